@@ -1,0 +1,185 @@
+/* api.c -- flat entry points of the oracle for the Python test harness.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Pointer-and-size functions so
+ * that ctypes can drive the restatement without knowing struct layouts.
+ */
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+void *or_api_bench_new(int log_n, uint64_t seed) {
+    or_p2_bench_t *b = (or_p2_bench_t *)malloc(sizeof *b);
+    or_p2_bench_setup(b, log_n, seed);
+    return b;
+}
+void or_api_bench_free(void *h) {
+    or_p2_bench_free((or_p2_bench_t *)h);
+    free(h);
+}
+const f_t *or_api_bench_trace(void *h) { return ((or_p2_bench_t *)h)->trace; }
+const f_t *or_api_bench_inputs(void *h) { return ((or_p2_bench_t *)h)->inputs; }
+/* AIR round constants: beginning[64], partial[20], ending[64] */
+void or_api_bench_air_consts(void *h, f_t *out) { memcpy(out, &((or_p2_bench_t *)h)->air, sizeof(or_air_consts_t)); }
+/* Poseidon2 constants: ext_init[4][W], ext_term[4][W], internal[rounds_p] (packed) */
+int or_api_bench_p2_consts(void *h, int width, f_t *out) {
+    or_p2_bench_t *b = (or_p2_bench_t *)h;
+    const or_p2_t *p = width == 16 ? &b->p16 : &b->p24;
+    int k = 0;
+    for (int r = 0; r < 4; r++) for (int i = 0; i < width; i++) out[k++] = p->ext_init[r][i];
+    for (int r = 0; r < 4; r++) for (int i = 0; i < width; i++) out[k++] = p->ext_term[r][i];
+    for (int r = 0; r < p->rounds_p; r++) out[k++] = p->internal[r];
+    return k;
+}
+void or_api_permute(void *h, int width, f_t *state) {
+    or_p2_bench_t *b = (or_p2_bench_t *)h;
+    or_p2_permute(width == 16 ? &b->p16 : &b->p24, state);
+}
+void or_api_hash_leaf(void *h, const f_t *row, size_t len, f_t *out) {
+    or_hash_leaf(&((or_p2_bench_t *)h)->p24, row, len, out);
+}
+void or_api_compress(void *h, const f_t *l, const f_t *r, f_t *out) {
+    or_compress(&((or_p2_bench_t *)h)->p16, l, r, out);
+}
+
+static void settings(or_air_settings_t *s, int sec, int f0, int f, int rate, int skips, int rs0) {
+    s->security_bits = sec;
+    s->folding_f0 = f0;
+    s->folding_f = f;
+    s->log_inv_rate = rate;
+    s->univariate_skips = skips;
+    s->rs_reduction0 = rs0;
+}
+
+size_t or_api_prove(void *h, int sec, int f0, int f, int rate, int skips, int rs0, f_t **out) {
+    or_air_settings_t s;
+    settings(&s, sec, f0, f, rate, skips, rs0);
+    return or_air_prove(&s, (or_p2_bench_t *)h, out);
+}
+int or_api_verify(void *h, int sec, int f0, int f, int rate, int skips, int rs0, const f_t *proof,
+                  size_t len) {
+    or_air_settings_t s;
+    settings(&s, sec, f0, f, rate, skips, rs0);
+    return or_air_verify(&s, (or_p2_bench_t *)h, proof, len);
+}
+void or_api_free(void *p) { free(p); }
+
+/* WHIR schedule as a flat int array: [n_rounds, final_sumcheck_rounds,
+ * commit_ood, starting_folding_pow, final_queries, final_pow, final_folding_pow,
+ * final_log_domain, then per round 8 ints] */
+int or_api_whir_config(int v, int sec, int max_pow, int f0, int f, int rate, int rs0, int *out) {
+    or_whir_cfg_t c;
+    if (or_whir_config(&c, v, sec, max_pow, f0, f, rate, rs0)) return -1;
+    int k = 0;
+    out[k++] = c.n_rounds; out[k++] = c.final_sumcheck_rounds; out[k++] = c.commit_ood_samples;
+    out[k++] = c.starting_folding_pow_bits; out[k++] = c.final_queries; out[k++] = c.final_pow_bits;
+    out[k++] = c.final_folding_pow_bits; out[k++] = c.final_log_domain_size;
+    for (int r = 0; r < c.n_rounds; r++) {
+        const or_whir_round_t *p = &c.rounds[r];
+        out[k++] = p->pow_bits; out[k++] = p->folding_pow_bits; out[k++] = p->num_queries;
+        out[k++] = p->ood_samples; out[k++] = p->log_inv_rate; out[k++] = p->num_variables;
+        out[k++] = p->folding_factor; out[k++] = p->log_domain_size;
+    }
+    return k;
+}
+
+/* ---- field helpers ---- */
+void or_api_ef_mul(const f_t *a, const f_t *b, f_t *out) {
+    ef_t x, y;
+    memcpy(x.c, a, 32); memcpy(y.c, b, 32);
+    ef_t r = ef_mul(x, y);
+    memcpy(out, r.c, 32);
+}
+void or_api_ef_inv(const f_t *a, f_t *out) {
+    ef_t x;
+    memcpy(x.c, a, 32);
+    ef_t r = ef_inv(x);
+    memcpy(out, r.c, 32);
+}
+f_t or_api_two_adic_generator(int bits) { return f_two_adic_generator(bits); }
+void or_api_rng_u32(uint64_t seed, uint32_t *out, size_t n) {
+    or_rng_t r;
+    or_rng_seed_u64(&r, seed);
+    for (size_t i = 0; i < n; i++) out[i] = or_rng_next_u32(&r);
+}
+void or_api_rng_f(uint64_t seed, f_t *out, size_t n) {
+    or_rng_t r;
+    or_rng_seed_u64(&r, seed);
+    for (size_t i = 0; i < n; i++) out[i] = or_rng_sample_f(&r);
+}
+void or_api_eval_eq(const f_t *point, int n, f_t *out) { or_eval_eq((const ef_t *)point, n, (ef_t *)out); }
+void or_api_mle_eval_f(const f_t *evals, int n, const f_t *point, f_t *out) {
+    ef_t r = or_mle_eval_f(evals, n, (const ef_t *)point);
+    memcpy(out, r.c, 32);
+}
+void or_api_mle_eval_ef(const f_t *evals, int n, const f_t *point, f_t *out) {
+    ef_t r = or_mle_eval_ef((const ef_t *)evals, n, (const ef_t *)point);
+    memcpy(out, r.c, 32);
+}
+void or_api_air_eval_f(void *h, const f_t *local, const f_t *alpha_pows, f_t *out) {
+    ef_t r = or_air_eval_f(&((or_p2_bench_t *)h)->air, local, (const ef_t *)alpha_pows);
+    memcpy(out, r.c, 32);
+}
+void or_api_air_eval_ef(void *h, const f_t *local, const f_t *alpha_pows, f_t *out) {
+    ef_t r = or_air_eval_ef(&((or_p2_bench_t *)h)->air, (const ef_t *)local, (const ef_t *)alpha_pows);
+    memcpy(out, r.c, 32);
+}
+void or_api_selectors_at_ef(int k, const f_t *x, f_t *out) {
+    ef_t xx;
+    memcpy(xx.c, x, 32);
+    or_selectors_at_ef(k, xx, (ef_t *)out);
+}
+void or_api_lagrange(const f_t *xs, const f_t *ys, int n, f_t *coeffs) {
+    or_lagrange_interp(xs, (const ef_t *)ys, n, (ef_t *)coeffs);
+}
+
+/* ---- folds (crates/utils/src/multilinear.rs) ---- */
+/* fold an F table (2^log_n) by 2^k EF scalars -> EF table (2^(log_n-k)) */
+void or_api_fold_f_large(const f_t *in, int log_n, const f_t *scalars, int k, f_t *out) {
+    size_t M = (size_t)1 << (log_n - k);
+    int K = 1 << k;
+    const ef_t *s = (const ef_t *)scalars;
+    ef_t *o = (ef_t *)out;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < M; i++) {
+        ef_t acc = ef_zero();
+        for (int j = 0; j < K; j++) acc = ef_add(acc, ef_mul_f(s[j], in[i + (size_t)j * M]));
+        o[i] = acc;
+    }
+}
+void or_api_fold_ef_large(const f_t *in, int log_n, const f_t *scalars, int k, f_t *out) {
+    size_t M = (size_t)1 << (log_n - k);
+    int K = 1 << k;
+    const ef_t *s = (const ef_t *)scalars, *x = (const ef_t *)in;
+    ef_t *o = (ef_t *)out;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < M; i++) {
+        ef_t acc = ef_zero();
+        for (int j = 0; j < K; j++) acc = ef_add(acc, ef_mul(s[j], x[i + (size_t)j * M]));
+        o[i] = acc;
+    }
+}
+
+/* ---- product sumcheck micro: sumcheck::prove(1, [a, b], a^2 b, 3, [1], Some(z), false, ...)
+ * over a transcript whose challenger uses the Poseidon16 of bench handle h with an
+ * empty domain separator.  Returns the proof length; proof copied to out. ---- */
+size_t or_api_sc_a2b(void *h, const f_t *a, const f_t *b, int n, const f_t *z, f_t *challenges,
+                     f_t *final_evals, f_t *final_sum, f_t **proof_out) {
+    or_p2_bench_t *bb = (or_p2_bench_t *)h;
+    or_ts_t ts;
+    or_ts_init(&ts, &bb->p16, NULL, 0);
+    /* claimed sum = sum_x eq(z, x) a(x)^2 b(x) */
+    size_t N = (size_t)1 << n;
+    ef_t *eq = (ef_t *)malloc(N * sizeof(ef_t));
+    or_eval_eq((const ef_t *)z, n, eq);
+    ef_t sum = ef_zero();
+    for (size_t x = 0; x < N; x++) sum = ef_add(sum, ef_mul_f(eq[x], f_mul(f_mul(a[x], a[x]), b[x])));
+    free(eq);
+    const void *mles[2] = {a, b};
+    or_sc_comp_t comp = {OR_SC_A2B, NULL, NULL, 0};
+    ef_t fs;
+    or_sc_prove(1, 2, 1, mles, n, &comp, 3, (const ef_t *)z, 0, &ts, sum, 128,
+                (ef_t *)challenges, (ef_t *)final_evals, &fs);
+    memcpy(final_sum, fs.c, 32);
+    *proof_out = ts.data;
+    return ts.len;
+}
