@@ -1,0 +1,148 @@
+/*
+ * whirlaway_hip.h -- C ABI of the MI355X (gfx950) Whirlaway prover.
+ *
+ * Drop-in boundary for the reference's prover hot path.  Each entry point
+ * names the reference interface it replaces (paths relative to the reference
+ * repository root).  Conventions:
+ *   - plain pointers and sizes; no torch or HIP types cross the boundary;
+ *   - field elements are canonical little-endian u32 in [0, p), p = 2^31 - 2^24 + 1;
+ *     an extension element is 8 consecutive u32 (coefficient 0 first);
+ *   - every call returns a status (WL_OK == 0) and never unwinds; the message of
+ *     the last failure on the calling thread is wl_last_error();
+ *   - one HIP stream per context; a context is not thread-safe, separate
+ *     contexts are.
+ */
+#ifndef WHIRLAWAY_HIP_H
+#define WHIRLAWAY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { WL_OK = 0, WL_ERR_ARG = 1, WL_ERR_DEVICE = 2, WL_ERR_BUFFER = 3 };
+
+typedef struct wl_ctx wl_ctx;
+typedef struct wl_table wl_table;
+typedef struct wl_witness wl_witness;
+
+/* AirSettings (crates/air/src/lib.rs:13-41) with
+ * FoldingFactor::ConstantFromSecondRound(folding_factor_first, folding_factor)
+ * and SecurityAssumption::CapacityBound (src/main.rs:16-23). */
+typedef struct {
+    uint32_t security_bits;               /* 128 */
+    uint32_t folding_factor_first;        /* 7 */
+    uint32_t folding_factor;              /* 4 */
+    uint32_t log_inv_rate;                /* 1 */
+    uint32_t univariate_skips;            /* 4 (must equal the table's) */
+    uint32_t rs_domain_initial_reduction; /* 5 */
+} wl_air_settings;
+
+/* Message of the last failed call on this thread. */
+const char *wl_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int wl_device_count(void);
+
+/* Context: owns the HIP stream, the device memory pool and twiddle tables.
+ * Replaces the implicit global rayon pool of the reference
+ * (crates/sumcheck/src/prove.rs:6, crates/utils/src/multilinear.rs:6). */
+wl_ctx *wl_create(int device);
+void wl_destroy(wl_ctx *ctx);
+
+/* AirTable::new for the Poseidon2-16 KoalaBear AIR (crates/air/src/table.rs:32-56,
+ * called at src/examples/poseidon2.rs:140-146).  round_constants: the AIR's
+ * RoundConstants (148 canonical u32: beginning[4][16], partial[20],
+ * ending[4][16]; p3-poseidon2-air RoundConstants::from_rng order). */
+wl_table *wl_air_table_new(wl_ctx *ctx, const uint32_t *round_constants, uint32_t log_length,
+                           uint32_t univariate_skips);
+void wl_air_table_free(wl_table *table);
+
+/* AirTable::prove (crates/air/src/prove.rs:49-226) including the transcript set
+ * up by DomainSeparator/to_prover_state (src/examples/poseidon2.rs:155-163).
+ * p16/p24: Poseidon2KoalaBear<16>/<24> constants (canonical): ext_init[4][W],
+ * ext_term[4][W], internal[20 | 23] (new_from_rng_128 order).  witness: 165
+ * columns x 2^log_length, column-major (Vec<EvaluationsList<F>> of
+ * prove.rs:55).  The proof is proof_data (Vec<F>, src/examples/poseidon2.rs:178):
+ * *proof_len always receives its length; proof_out may be NULL to query it. */
+int wl_air_prove(wl_table *table, const wl_air_settings *settings, const uint32_t *p16,
+                 const uint32_t *p24, const uint32_t *witness, uint32_t *proof_out,
+                 size_t proof_cap, size_t *proof_len);
+
+/* Witness resident in HBM (the commitment's packed polynomial,
+ * crates/utils/src/multilinear.rs:169-181, built on the device). */
+wl_witness *wl_witness_upload(wl_ctx *ctx, const uint32_t *witness, uint32_t log_length);
+void wl_witness_free(wl_witness *w);
+/* wl_air_prove on a resident witness (no host-to-device copy of the trace). */
+int wl_air_prove_resident(wl_table *table, const wl_air_settings *settings, const uint32_t *p16,
+                          const uint32_t *p24, const wl_witness *w, uint32_t *proof_out,
+                          size_t proof_cap, size_t *proof_len);
+
+/* Synthetic benchmark inputs exactly as src/examples/poseidon2.rs:104-151 draws
+ * them from StdRng::seed_from_u64(seed): air_rc (148, RoundConstants::from_rng),
+ * inputs (16 x 2^log_n, row-major), p16 (148) and p24 (215) Poseidon2
+ * constants (new_from_rng_128).  Any output pointer may be NULL. */
+int wl_p2_bench_inputs(uint64_t seed, uint32_t log_n, uint32_t *air_rc, uint32_t *inputs,
+                       uint32_t *p16, uint32_t *p24);
+
+/* generate_trace_rows (src/examples/poseidon2.rs:122-131) on the GPU: one
+ * Poseidon2 permutation per row with the table's round constants, written
+ * column-major into a resident witness. */
+wl_witness *wl_air_generate_witness(wl_table *table, const uint32_t *inputs);
+
+/* Copy a resident witness (165 x 2^n canonical, column-major) to the host. */
+int wl_witness_download(const wl_witness *w, uint32_t *out);
+
+/* Accumulated per-phase wall times in ms since the last reset: [commit,
+ * zerocheck round 0, zerocheck rounds >= 1, inner sums + batching, inner
+ * sumcheck, WHIR opening]. */
+void wl_phase_times(wl_ctx *ctx, double *out6, int reset);
+
+/* ---------------------------------------------------------------- kernels --
+ * crates/utils/src/multilinear.rs:90-123 fold_multilinear_in_large_field:
+ * out[i] = sum_j scalars[j] * in[i + j * 2^(log_n - log_k)], in F (base) or EF,
+ * 2^log_k EF scalars, EF output of 2^(log_n - log_k) elements. */
+int wl_fold_large(wl_ctx *ctx, const uint32_t *in, int in_is_ef, uint32_t log_n,
+                  const uint32_t *scalars, uint32_t log_k, uint32_t *out);
+
+/* Kernel micro (a), benches/folding_benchmarks.rs: fold 2^log_n base-field
+ * evaluations by n_folds successive EF challenges (1 - c, c), most-significant
+ * variable first; after each fold the intermediate table is checked by the
+ * caller through out_all (concatenated EF tables of sizes 2^(log_n-1), ...),
+ * or only the last table when out_all is NULL (final_out: 2^(log_n-n_folds) EF).
+ * Device time of the folds (ms) in *ms when non-NULL. */
+int wl_fold_chain(wl_ctx *ctx, const uint32_t *in, uint32_t log_n, const uint32_t *challenges,
+                  uint32_t n_folds, uint32_t *out_all, uint32_t *final_out, double *ms);
+
+/* Kernel micro (b): sumcheck::prove(1, [a, b], a^2 * b, degree 3, [1],
+ * Some(z), false, ...) (crates/sumcheck/src/prove.rs:22-93) over a transcript
+ * whose challenger is Poseidon2KoalaBear<16> with constants p16 and an empty
+ * domain separator.  a, b: 2^n_vars base-field evaluations; z: n_vars EF.
+ * Outputs: challenges (n_vars EF), final evaluations of a and b (2 EF), final
+ * sum (1 EF), proof_data (as wl_air_prove). */
+int wl_sumcheck_a2b(wl_ctx *ctx, const uint32_t *p16, const uint32_t *a, const uint32_t *b,
+                    uint32_t n_vars, const uint32_t *z, uint32_t *challenges,
+                    uint32_t *final_evals, uint32_t *final_sum, uint32_t *proof_out,
+                    size_t proof_cap, size_t *proof_len, double *ms);
+
+/* Poseidon2 permutations on the device (p3-poseidon2 Poseidon2KoalaBear<W>,
+ * src/examples/poseidon2.rs:22-26): n states of width 16 or 24, in place. */
+int wl_poseidon2_permute(wl_ctx *ctx, const uint32_t *p16, const uint32_t *p24, int width,
+                         uint32_t *states, size_t n);
+
+/* Reed-Solomon encoding of n_cols columns of 2^log_m evaluations (univariate
+ * embedding, natural-order NTT over the 2^log_rows-th roots of unity) and the
+ * Merkle root of its rows (PaddingFreeSponge<P24,24,16,8> leaves,
+ * TruncatedPermutation<P16,2,8,16> nodes) -- the CommitmentWriter::commit
+ * core called at crates/air/src/prove.rs:91.  codeword_out (n_cols x
+ * 2^log_rows, column-major) may be NULL. */
+int wl_rs_commit(wl_ctx *ctx, const uint32_t *p16, const uint32_t *p24, const uint32_t *cols,
+                 uint32_t n_cols, uint32_t log_m, uint32_t log_rows, uint32_t *codeword_out,
+                 uint32_t *root_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,199 @@
+"""ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only the test suite, the smoke entry and the benchmark's CPU-baseline leg load
+this module; the product (whirlaway_amd) never does.  build() compiles
+oracle/build/liboracle.so with the Makefile next to this file.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "liboracle.so")
+P = 0x7F000001
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "-j8"], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        vp = ctypes.c_void_p
+        L.or_api_bench_new.restype = vp
+        L.or_api_bench_new.argtypes = [ctypes.c_int, ctypes.c_uint64]
+        L.or_api_bench_free.argtypes = [vp]
+        L.or_api_bench_trace.restype = u32p
+        L.or_api_bench_trace.argtypes = [vp]
+        L.or_api_bench_inputs.restype = u32p
+        L.or_api_bench_inputs.argtypes = [vp]
+        L.or_api_bench_air_consts.argtypes = [vp, u32p]
+        L.or_api_bench_p2_consts.restype = ctypes.c_int
+        L.or_api_bench_p2_consts.argtypes = [vp, ctypes.c_int, u32p]
+        L.or_api_permute.argtypes = [vp, ctypes.c_int, u32p]
+        L.or_api_hash_leaf.argtypes = [vp, u32p, ctypes.c_size_t, u32p]
+        L.or_api_compress.argtypes = [vp, u32p, u32p, u32p]
+        L.or_api_prove.restype = ctypes.c_size_t
+        L.or_api_prove.argtypes = [vp] + [ctypes.c_int] * 6 + [ctypes.POINTER(u32p)]
+        L.or_api_verify.restype = ctypes.c_int
+        L.or_api_verify.argtypes = [vp] + [ctypes.c_int] * 6 + [u32p, ctypes.c_size_t]
+        L.or_api_free.argtypes = [vp]
+        L.or_api_whir_config.restype = ctypes.c_int
+        L.or_api_whir_config.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_int)]
+        L.or_api_ef_mul.argtypes = [u32p, u32p, u32p]
+        L.or_api_ef_inv.argtypes = [u32p, u32p]
+        L.or_api_two_adic_generator.restype = ctypes.c_uint32
+        L.or_api_two_adic_generator.argtypes = [ctypes.c_int]
+        L.or_api_rng_u32.argtypes = [ctypes.c_uint64, u32p, ctypes.c_size_t]
+        L.or_api_rng_f.argtypes = [ctypes.c_uint64, u32p, ctypes.c_size_t]
+        L.or_api_eval_eq.argtypes = [u32p, ctypes.c_int, u32p]
+        L.or_api_mle_eval_f.argtypes = [u32p, ctypes.c_int, u32p, u32p]
+        L.or_api_mle_eval_ef.argtypes = [u32p, ctypes.c_int, u32p, u32p]
+        L.or_api_air_eval_f.argtypes = [vp, u32p, u32p, u32p]
+        L.or_api_air_eval_ef.argtypes = [vp, u32p, u32p, u32p]
+        L.or_api_selectors_at_ef.argtypes = [ctypes.c_int, u32p, u32p]
+        L.or_api_lagrange.argtypes = [u32p, u32p, ctypes.c_int, u32p]
+        L.or_api_fold_f_large.argtypes = [u32p, ctypes.c_int, u32p, ctypes.c_int, u32p]
+        L.or_api_fold_ef_large.argtypes = [u32p, ctypes.c_int, u32p, ctypes.c_int, u32p]
+        L.or_api_sc_a2b.restype = ctypes.c_size_t
+        L.or_api_sc_a2b.argtypes = [vp, u32p, u32p, ctypes.c_int, u32p, u32p, u32p, u32p,
+                                    ctypes.POINTER(u32p)]
+        L.or_ntt.argtypes = [u32p, ctypes.c_int]
+        L.or_mobius.argtypes = [u32p, ctypes.c_int]
+        L.or_chacha_block.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, u32p]
+        L.or_chacha_block_ietf.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_int, u32p]
+        L.or_log2_up.restype = ctypes.c_size_t
+        L.or_log2_up.argtypes = [ctypes.c_size_t]
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+DEFAULT = dict(sec=128, f0=7, f=4, rate=1, skips=4, rs0=5)
+
+
+class Bench:
+    """src/examples/poseidon2.rs:85-151: seeded constants, inputs and trace."""
+
+    def __init__(self, log_n, seed=0):
+        self.log_n = log_n
+        self.h = lib().or_api_bench_new(log_n, seed)
+
+    def close(self):
+        if self.h:
+            lib().or_api_bench_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def trace(self):
+        N = 1 << self.log_n
+        p = lib().or_api_bench_trace(self.h)
+        return np.ctypeslib.as_array(p, shape=(165 * N,)).copy()
+
+    def inputs(self):
+        N = 1 << self.log_n
+        p = lib().or_api_bench_inputs(self.h)
+        return np.ctypeslib.as_array(p, shape=(16 * N,)).copy()
+
+    def air_consts(self):
+        out = np.zeros(148, dtype=np.uint32)
+        lib().or_api_bench_air_consts(self.h, ptr(out))
+        return out
+
+    def p2_consts(self, width):
+        out = np.zeros(4 * 24 * 2 + 23, dtype=np.uint32)
+        k = lib().or_api_bench_p2_consts(self.h, width, ptr(out))
+        return out[:k].copy()
+
+    def permute(self, width, state):
+        s = np.ascontiguousarray(state, dtype=np.uint32).copy()
+        lib().or_api_permute(self.h, width, ptr(s))
+        return s
+
+    def prove(self, **kw):
+        s = dict(DEFAULT, **kw)
+        p = ctypes.POINTER(ctypes.c_uint32)()
+        n = lib().or_api_prove(self.h, s["sec"], s["f0"], s["f"], s["rate"], s["skips"], s["rs0"],
+                               ctypes.byref(p))
+        arr = np.ctypeslib.as_array(p, shape=(n,)).copy()
+        lib().or_api_free(p)
+        return arr
+
+    def verify(self, proof, **kw):
+        s = dict(DEFAULT, **kw)
+        pr = np.ascontiguousarray(proof, dtype=np.uint32)
+        return lib().or_api_verify(self.h, s["sec"], s["f0"], s["f"], s["rate"], s["skips"],
+                                   s["rs0"], ptr(pr), pr.size)
+
+    def sc_a2b(self, a, b, z):
+        n = a.size.bit_length() - 1
+        ch = np.zeros(8 * n, dtype=np.uint32)
+        fe = np.zeros(16, dtype=np.uint32)
+        fs = np.zeros(8, dtype=np.uint32)
+        p = ctypes.POINTER(ctypes.c_uint32)()
+        ln = lib().or_api_sc_a2b(self.h, ptr(a), ptr(b), n, ptr(np.ascontiguousarray(z)),
+                                 ptr(ch), ptr(fe), ptr(fs), ctypes.byref(p))
+        proof = np.ctypeslib.as_array(p, shape=(ln,)).copy()
+        lib().or_api_free(p)
+        return {"challenges": ch.reshape(n, 8), "final_evals": fe.reshape(2, 8),
+                "final_sum": fs, "proof": proof}
+
+
+def whir_config(v, sec=128, max_pow=16, f0=7, f=4, rate=1, rs0=5):
+    out = (ctypes.c_int * 256)()
+    k = lib().or_api_whir_config(v, sec, max_pow, f0, f, rate, rs0, out)
+    vals = list(out[:k])
+    head = dict(zip(["n_rounds", "final_sumcheck_rounds", "commit_ood", "starting_folding_pow",
+                     "final_queries", "final_pow", "final_folding_pow", "final_log_domain"],
+                    vals[:8]))
+    rounds = []
+    for r in range(head["n_rounds"]):
+        rounds.append(dict(zip(["pow_bits", "folding_pow_bits", "num_queries", "ood_samples",
+                                "log_inv_rate", "num_variables", "folding_factor",
+                                "log_domain_size"], vals[8 + 8 * r: 16 + 8 * r])))
+    head["rounds"] = rounds
+    return head
+
+
+def rng_f(seed, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().or_api_rng_f(seed, ptr(out), n)
+    return out
+
+
+def rng_u32(seed, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().or_api_rng_u32(seed, ptr(out), n)
+    return out
+
+
+def fold_f_large(m, scalars, k):
+    log_n = m.size.bit_length() - 1
+    out = np.zeros((m.size >> k) * 8, dtype=np.uint32)
+    lib().or_api_fold_f_large(ptr(m), log_n, ptr(np.ascontiguousarray(scalars)), k, ptr(out))
+    return out
+
+
+def fold_ef_large(m, scalars, k):
+    log_n = (m.size // 8).bit_length() - 1
+    out = np.zeros(((m.size // 8) >> k) * 8, dtype=np.uint32)
+    lib().or_api_fold_ef_large(ptr(m), log_n, ptr(np.ascontiguousarray(scalars)), k, ptr(out))
+    return out

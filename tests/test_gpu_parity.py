@@ -1,0 +1,169 @@
+"""GPU parity: every HIP entry point against the CPU oracle on the same inputs.
+
+Integer/field work, so the bar is bit-exact equality.  Full proofs are compared
+byte for byte (proof_data) for n up to 12 and checked against the oracle's
+verifier at the benchmark size n = 17.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import pyfield as pf
+
+pytestmark = pytest.mark.gpu
+P = pf.P
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_poseidon2_permutations(gpu_ctx, oracle):
+    from whirlaway_amd import utils
+    b = oracle.Bench(5, seed=3)
+    p16, p24 = b.p2_consts(16), b.p2_consts(24)
+    rng = np.random.default_rng(5)
+    for width in (16, 24):
+        _say(f"poseidon2 width {width}")
+        st = rng.integers(0, P, (257, width), dtype=np.uint32)
+        got = utils.poseidon2_permute(gpu_ctx, p16, p24, width, st).reshape(-1, width)
+        for i in range(0, 257, 32):
+            assert np.array_equal(got[i], b.permute(width, st[i])), (width, i)
+
+
+@pytest.mark.parametrize("log_n,k", [(10, 1), (12, 4), (16, 4), (18, 7)])
+def test_fold_large(gpu_ctx, oracle, log_n, k):
+    from whirlaway_amd import utils
+    _say(f"fold_large log_n={log_n} k={k}")
+    rng = np.random.default_rng(log_n * 31 + k)
+    m = rng.integers(0, P, 1 << log_n, dtype=np.uint32)
+    sc = rng.integers(0, P, 8 << k, dtype=np.uint32)
+    assert np.array_equal(utils.fold_multilinear_in_large_field(gpu_ctx, m, sc),
+                          oracle.fold_f_large(m, sc, k))
+    me = rng.integers(0, P, (1 << (log_n - 2)) * 8, dtype=np.uint32)
+    assert np.array_equal(utils.fold_multilinear_in_large_field(gpu_ctx, me, sc, is_ef=True),
+                          oracle.fold_ef_large(me, sc, k))
+
+
+def test_fold_chain_per_round(gpu_ctx, oracle):
+    """Kernel micro (a) shape at reduced size: every intermediate table bit-exact."""
+    from whirlaway_amd import utils
+    log_n, folds = 14, 14
+    _say(f"fold_chain log_n={log_n}")
+    rng = np.random.default_rng(7)
+    m = rng.integers(0, P, 1 << log_n, dtype=np.uint32)
+    ch = rng.integers(0, P, (folds, 8), dtype=np.uint32)
+    final, allt, _ = utils.fold_chain(gpu_ctx, m, ch, all_tables=True)
+    cur, off = None, 0
+    for i in range(folds):
+        sc = np.concatenate([(np.array(pf.esub(pf.ef(1), pf.ef(ch[i])), dtype=np.uint32)), ch[i]])
+        cur = oracle.fold_f_large(m, sc, 1) if i == 0 else oracle.fold_ef_large(cur, sc, 1)
+        ln = cur.size
+        assert np.array_equal(allt[off: off + ln], cur), i
+        off += ln
+    assert np.array_equal(final, cur)
+
+
+@pytest.mark.parametrize("n", [4, 9, 14])
+def test_sumcheck_a2b(gpu_ctx, oracle, n):
+    """Kernel micro (b) shape at reduced size: transcript, challenges and final values."""
+    from whirlaway_amd import sumcheck
+    _say(f"sumcheck a2b n={n}")
+    b = oracle.Bench(5, seed=11)
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, P, 1 << n, dtype=np.uint32)
+    bb = rng.integers(0, P, 1 << n, dtype=np.uint32)
+    z = rng.integers(0, P, (n, 8), dtype=np.uint32)
+    want = b.sc_a2b(a, bb, z.reshape(-1).copy())
+    got = sumcheck.prove_a2b(gpu_ctx, b.p2_consts(16), a, bb, z)
+    assert np.array_equal(got["proof"], want["proof"])
+    assert np.array_equal(got["challenges"], want["challenges"])
+    assert np.array_equal(got["final_evals"], want["final_evals"])
+    assert np.array_equal(got["final_sum"], want["final_sum"])
+
+
+@pytest.mark.parametrize("n_cols,log_m,log_rows", [(4, 5, 6), (8, 10, 12), (3, 10, 13), (2, 13, 16), (16, 6, 15)])
+def test_rs_commit(gpu_ctx, oracle, n_cols, log_m, log_rows):
+    from whirlaway_amd import utils
+    _say(f"rs_commit {n_cols}x2^{log_m} -> 2^{log_rows}")
+    b = oracle.Bench(5, seed=13)
+    rng = np.random.default_rng(log_rows)
+    cols = rng.integers(0, P, (n_cols, 1 << log_m), dtype=np.uint32)
+    root, cw = utils.rs_commit(gpu_ctx, b.p2_consts(16), b.p2_consts(24), cols, log_rows, True)
+    L = oracle.lib()
+    cw = cw.reshape(n_cols, 1 << log_rows)
+    for c in range(n_cols):
+        col = np.zeros(1 << log_rows, dtype=np.uint32)
+        col[: 1 << log_m] = cols[c]
+        L.or_mobius(oracle.ptr(col), log_m)
+        L.or_ntt(oracle.ptr(col), log_rows)
+        assert np.array_equal(cw[c], col), c
+    # Merkle root from the oracle's sponge/compression
+    rows = cw.T.copy()
+    level = []
+    for r in range(rows.shape[0]):
+        out = np.zeros(8, dtype=np.uint32)
+        L.or_api_hash_leaf(b.h, oracle.ptr(np.ascontiguousarray(rows[r])), n_cols, oracle.ptr(out))
+        level.append(out)
+    while len(level) > 1:
+        nxt = []
+        for i in range(0, len(level), 2):
+            out = np.zeros(8, dtype=np.uint32)
+            L.or_api_compress(b.h, oracle.ptr(level[i]), oracle.ptr(level[i + 1]), oracle.ptr(out))
+            nxt.append(out)
+        level = nxt
+    assert np.array_equal(root, level[0])
+
+
+@pytest.mark.parametrize("n", [5, 6, 8, 10, 12])
+def test_proof_bytes_identical(gpu_ctx, oracle, n):
+    import whirlaway_amd as wa
+    _say(f"full proof n={n}")
+    b = oracle.Bench(n, seed=0)
+    want = b.prove()
+    table = wa.AirTable(gpu_ctx, b.air_consts(), n)
+    got = table.prove(wa.AirSettings(), b.p2_consts(16), b.p2_consts(24), b.trace())
+    table.close()
+    assert got.size == want.size
+    diff = np.nonzero(got != want)[0]
+    assert diff.size == 0, f"first mismatch at {diff[:5]}"
+    assert b.verify(got) == 0
+
+
+def test_proof_benchmark_size_verifies(gpu_ctx, oracle):
+    """n = 17 (the benchmark workload): the oracle verifier accepts the GPU proof,
+    and a resident-witness proof is byte-identical to the host-witness proof."""
+    import whirlaway_amd as wa
+    from whirlaway_amd.air import Witness
+    n = 17
+    _say("full proof n=17")
+    b = oracle.Bench(n, seed=0)
+    table = wa.AirTable(gpu_ctx, b.air_consts(), n)
+    tr = b.trace()
+    got = table.prove(wa.AirSettings(), b.p2_consts(16), b.p2_consts(24), tr)
+    assert b.verify(got) == 0
+    w = Witness(gpu_ctx, tr, n)
+    again = table.prove(wa.AirSettings(), b.p2_consts(16), b.p2_consts(24), w)
+    w.close()
+    table.close()
+    assert np.array_equal(got, again)
+    bad = got.copy()
+    bad[len(bad) // 2] = (int(bad[len(bad) // 2]) + 1) % P
+    assert b.verify(bad) != 0
+
+
+@pytest.mark.parametrize("n", [5, 11, 17])
+def test_trace_generation(gpu_ctx, oracle, n):
+    """generate_trace_rows on the GPU equals the oracle's trace."""
+    import whirlaway_amd as wa
+    from whirlaway_amd.air import bench_inputs, download_witness, generate_witness
+    _say(f"trace generation n={n}")
+    rc, inp, p16, p24 = bench_inputs(0, n)
+    table = wa.AirTable(gpu_ctx, rc, n)
+    w = generate_witness(table, inp)
+    got = download_witness(w)
+    w.close()
+    table.close()
+    assert np.array_equal(got, oracle.Bench(n, 0).trace())

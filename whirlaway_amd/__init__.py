@@ -1,0 +1,81 @@
+"""MI355X-native Whirlaway prover hot path.
+
+The product is the HIP/C++ library ``libwhirlaway_hip.so`` (C ABI in
+``include/whirlaway_hip.h``); this package is a thin ctypes mirror of the
+reference's Rust API (``crates/air``: AirSettings, AirTable::prove;
+``crates/sumcheck``: prove; ``crates/utils``: folds) for tests and the
+benchmark.  There is no CPU fallback: importing a compute entry point without
+the built library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwhirlaway_hip.so")
+P = 0x7F000001
+
+_lib = None
+
+
+class WhirlawayError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP extension; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise WhirlawayError(
+                f"{LIB_PATH} is missing: build it with `python -m whirlaway_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        L.wl_last_error.restype = ctypes.c_char_p
+        L.wl_device_count.restype = ctypes.c_int
+        L.wl_create.restype = vp
+        L.wl_create.argtypes = [ctypes.c_int]
+        L.wl_destroy.argtypes = [vp]
+        L.wl_air_table_new.restype = vp
+        L.wl_air_table_new.argtypes = [vp, u32p, ctypes.c_uint32, ctypes.c_uint32]
+        L.wl_air_table_free.argtypes = [vp]
+        L.wl_air_prove.argtypes = [vp, vp, u32p, u32p, u32p, u32p, ctypes.c_size_t, szp]
+        L.wl_witness_upload.restype = vp
+        L.wl_witness_upload.argtypes = [vp, u32p, ctypes.c_uint32]
+        L.wl_witness_free.argtypes = [vp]
+        L.wl_air_prove_resident.argtypes = [vp, vp, u32p, u32p, vp, u32p, ctypes.c_size_t, szp]
+        L.wl_phase_times.argtypes = [vp, dp, ctypes.c_int]
+        L.wl_fold_large.argtypes = [vp, u32p, ctypes.c_int, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p]
+        L.wl_fold_chain.argtypes = [vp, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p, dp]
+        L.wl_sumcheck_a2b.argtypes = [vp, u32p, u32p, u32p, ctypes.c_uint32, u32p, u32p, u32p, u32p,
+                                      u32p, ctypes.c_size_t, szp, dp]
+        L.wl_poseidon2_permute.argtypes = [vp, u32p, u32p, ctypes.c_int, u32p, ctypes.c_size_t]
+        L.wl_rs_commit.argtypes = [vp, u32p, u32p, u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.c_uint32, u32p, u32p]
+        L.wl_p2_bench_inputs.argtypes = [ctypes.c_uint64, ctypes.c_uint32, u32p, u32p, u32p, u32p]
+        L.wl_air_generate_witness.restype = vp
+        L.wl_air_generate_witness.argtypes = [vp, u32p]
+        L.wl_witness_download.argtypes = [vp, u32p]
+        _lib = L
+    return _lib
+
+
+def u32ptr(a):
+    """Pointer to a C-contiguous uint32 numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"], "expected contiguous uint32"
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def check(rc):
+    if rc != 0:
+        raise WhirlawayError(lib().wl_last_error().decode() or f"error {rc}")
+
+
+from .air import AirSettings, AirTable, Context  # noqa: E402
+from . import sumcheck, utils  # noqa: E402,F401

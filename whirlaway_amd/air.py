@@ -1,0 +1,146 @@
+"""crates/air counterpart: AirSettings and AirTable::prove for the Poseidon2 AIR.
+
+Mirrors crates/air/src/lib.rs:13-41 (AirSettings::new) and
+crates/air/src/table.rs:32-56 / prove.rs:49-226 (AirTable::new / prove).  The
+reference prove() panics on failure; here a WhirlawayError is raised.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import check, lib, u32ptr, WhirlawayError
+
+NCOLS = 165  # Poseidon2Air<16, 3, 0, 4, 20> width
+
+
+@dataclass
+class AirSettings:
+    """AirSettings::new(security_bits, CapacityBound, ConstantFromSecondRound(f0, f),
+    log_inv_rate, univariate_skips, initial_domain_reduction) -- src/main.rs:16-23."""
+    security_bits: int = 128
+    folding_factor_first: int = 7
+    folding_factor: int = 4
+    log_inv_rate: int = 1
+    univariate_skips: int = 4
+    rs_domain_initial_reduction: int = 5
+
+    def c_struct(self):
+        return (ctypes.c_uint32 * 6)(self.security_bits, self.folding_factor_first,
+                                      self.folding_factor, self.log_inv_rate,
+                                      self.univariate_skips, self.rs_domain_initial_reduction)
+
+
+class Context:
+    """One HIP stream + memory pool on one device (wl_create)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        self.ptr = L.wl_create(device)
+        if not self.ptr:
+            raise WhirlawayError(L.wl_last_error().decode())
+
+    def close(self):
+        if self.ptr:
+            lib().wl_destroy(self.ptr)
+            self.ptr = None
+
+    def phase_times(self, reset=True):
+        out = (ctypes.c_double * 6)()
+        lib().wl_phase_times(self.ptr, out, int(reset))
+        return list(out)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class AirTable:
+    """AirTable::<KoalaBear, EF8, Poseidon2Air>::new(air, log_length, skips, [], 3)."""
+
+    def __init__(self, ctx, round_constants, log_length, univariate_skips=4):
+        rc = np.ascontiguousarray(round_constants, dtype=np.uint32)
+        if rc.size != 148:
+            raise ValueError("Poseidon2 AIR round constants: expected 148 values")
+        self.ctx = ctx
+        self.log_length = log_length
+        self._rc = rc
+        self.ptr = lib().wl_air_table_new(ctx.ptr, u32ptr(rc), log_length, univariate_skips)
+        if not self.ptr:
+            raise WhirlawayError(lib().wl_last_error().decode())
+
+    def close(self):
+        if self.ptr:
+            lib().wl_air_table_free(self.ptr)
+            self.ptr = None
+
+    def prove(self, settings, p16, p24, witness):
+        """Returns proof_data (uint32 canonical field elements).  witness: 165 x 2^n
+        column-major (uint32) or a resident Witness."""
+        p16 = np.ascontiguousarray(p16, dtype=np.uint32)
+        p24 = np.ascontiguousarray(p24, dtype=np.uint32)
+        st = settings.c_struct()
+        n = ctypes.c_size_t(0)
+        L = lib()
+        cap = 1 << 20
+        out = np.zeros(cap, dtype=np.uint32)
+        if isinstance(witness, Witness):
+            check(L.wl_air_prove_resident(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24),
+                                          witness.ptr, u32ptr(out), cap, ctypes.byref(n)))
+        else:
+            w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1)
+            if w.size != NCOLS << self.log_length:
+                raise ValueError("witness must hold 165 columns of 2^log_length values")
+            check(L.wl_air_prove(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24), u32ptr(w),
+                                 u32ptr(out), cap, ctypes.byref(n)))
+        return out[: n.value].copy()
+
+
+class Witness:
+    """Trace resident in HBM (wl_witness_upload)."""
+
+    def __init__(self, ctx, witness, log_length):
+        w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1)
+        if w.size != NCOLS << log_length:
+            raise ValueError("witness must hold 165 columns of 2^log_length values")
+        self.log_length = log_length
+        self.ptr = lib().wl_witness_upload(ctx.ptr, u32ptr(w), log_length)
+        if not self.ptr:
+            raise WhirlawayError(lib().wl_last_error().decode())
+
+    def close(self):
+        if self.ptr:
+            lib().wl_witness_free(self.ptr)
+            self.ptr = None
+
+
+def bench_inputs(seed, log_n, with_inputs=True):
+    """src/examples/poseidon2.rs:104-151: (air round constants, inputs, p16, p24)."""
+    rc = np.zeros(148, dtype=np.uint32)
+    p16 = np.zeros(148, dtype=np.uint32)
+    p24 = np.zeros(215, dtype=np.uint32)
+    inputs = np.zeros(16 << log_n, dtype=np.uint32) if with_inputs else None
+    check(lib().wl_p2_bench_inputs(seed, log_n, u32ptr(rc), u32ptr(inputs), u32ptr(p16), u32ptr(p24)))
+    return rc, inputs, p16, p24
+
+
+def generate_witness(table, inputs):
+    """generate_trace_rows on the GPU into a resident Witness."""
+    inputs = np.ascontiguousarray(inputs, dtype=np.uint32)
+    if inputs.size != 16 << table.log_length:
+        raise ValueError("inputs must hold 16 x 2^log_length values")
+    ptr = lib().wl_air_generate_witness(table.ptr, u32ptr(inputs))
+    if not ptr:
+        raise WhirlawayError(lib().wl_last_error().decode())
+    w = Witness.__new__(Witness)
+    w.ptr = ptr
+    w.log_length = table.log_length
+    return w
+
+
+def download_witness(w):
+    out = np.zeros(NCOLS << w.log_length, dtype=np.uint32)
+    check(lib().wl_witness_download(w.ptr, u32ptr(out)))
+    return out

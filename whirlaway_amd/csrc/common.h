@@ -1,0 +1,84 @@
+// common.h -- device helpers shared by the prover kernels: compile-time
+// Poseidon2 diagonals, wave/block reductions of extension-field values and
+// the host-side launch/check macros.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include "kb.h"
+
+namespace kb {
+
+constexpr uint32_t cmont(uint64_t c) { return (uint32_t)((c << 32) % P); }
+constexpr uint64_t cinv2k(int k) {
+    uint64_t x = 1;
+    for (int i = 0; i < k; i++) x = (x & 1) ? (x + P) / 2 : x / 2;
+    return x;
+}
+constexpr uint64_t cneg(uint64_t x) { return x ? P - x : 0; }
+
+struct DiagTables {
+    uint32_t d16[16];
+    uint32_t d24[24];
+};
+
+constexpr DiagTables make_diag() {
+    DiagTables t{};
+    const uint64_t c16[16] = {cneg(2), 1, 2, cinv2k(1), 3, 4, cneg(cinv2k(1)), cneg(3), cneg(4),
+                              cinv2k(8), cinv2k(3), cinv2k(24), cneg(cinv2k(8)), cneg(cinv2k(3)),
+                              cneg(cinv2k(4)), cneg(cinv2k(24))};
+    const uint64_t c24[24] = {cneg(2), 1, 2, cinv2k(1), 3, 4, cneg(cinv2k(1)), cneg(3), cneg(4),
+                              cinv2k(8), cinv2k(2), cinv2k(3), cinv2k(4), cinv2k(5), cinv2k(6),
+                              cinv2k(24), cneg(cinv2k(8)), cneg(cinv2k(3)), cneg(cinv2k(4)),
+                              cneg(cinv2k(5)), cneg(cinv2k(6)), cneg(cinv2k(7)), cneg(cinv2k(9)),
+                              cneg(cinv2k(24))};
+    for (int i = 0; i < 16; i++) t.d16[i] = cmont(c16[i]);
+    for (int i = 0; i < 24; i++) t.d24[i] = cmont(c24[i]);
+    return t;
+}
+
+constexpr DiagTables DIAG = make_diag();
+__device__ __constant__ static const DiagTables DDIAG = make_diag();
+
+// Poseidon2 with the fixed KoalaBear diagonals.
+template <typename C>
+__device__ __forceinline__ void p16_permute(F *s, const C &k) { p2_permute<16, 20>(s, k, DDIAG.d16); }
+template <typename C>
+__device__ __forceinline__ void p24_permute(F *s, const C &k) { p2_permute<24, 23>(s, k, DDIAG.d24); }
+
+// ---------------------------------------------------------- reductions --
+__device__ __forceinline__ EF wave_sum(EF v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) v.c[c] = fadd(v.c[c], F{(uint32_t)__shfl_xor((int)v.c[c].v, off, 64)});
+    }
+    return v;
+}
+
+// Sum over the block (blockDim.x multiple of 64, at most 1024); result valid in thread 0.
+__device__ __forceinline__ EF block_sum(EF v, EF *lds /* >= 16 entries */) {
+    v = wave_sum(v);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    EF r = ezero();
+    if (threadIdx.x == 0) {
+        int nw = blockDim.x >> 6;
+        for (int i = 0; i < nw; i++) r = eadd(r, lds[i]);
+    }
+    return r;
+}
+
+}  // namespace kb
+
+#define HIP_CHECK(x)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) {                                                           \
+            fprintf(stderr, "HIP error %s at %s:%d: %s\n", #x, __FILE__, __LINE__,        \
+                    hipGetErrorString(e_));                                               \
+            throw std::runtime_error(hipGetErrorString(e_));                              \
+        }                                                                                 \
+    } while (0)

@@ -1,0 +1,426 @@
+// k_whir.hip -- HIP kernels for the WHIR commitment and opening: Reed-Solomon
+// encoding (Mobius transform + radix-2 NTT, LDS-staged four-step), Poseidon2
+// Merkle trees, proof-of-work grinding, and the WHIR sumcheck reductions.
+//
+// Behaviour follows the reference's WHIR call sites (crates/air/src/prove.rs:82-91,
+// 216-225) with the conventions stated in oracle/whir.c.  The NTT uses
+// n = N2*n1 + n2, k = k1 + N1*k2:
+//   step A (kernel k_ntt_a): Mobius over the evaluation bits of n1, DFT along n1
+//   step B (kernel k_ntt_b): Mobius over the evaluation bits of n2, twiddle
+//                            w_N^(n2*k1), DFT along n2, store X[k1 + N1*k2]
+// Mobius along one axis commutes with the DFT along the other, so the fused
+// pipeline equals Mobius-then-NTT on the whole column.
+#include "common.h"
+#include "kernels.h"
+
+namespace wk {
+using namespace kb;
+
+struct P2DevT {
+    P2Consts16 p16;
+    P2Consts24 p24;
+};
+
+static inline unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+__device__ __forceinline__ uint32_t brev(uint32_t x, int bits) { return bits ? __brev(x) >> (32 - bits) : 0; }
+
+// ------------------------------------------------------------------ NTT ----
+__global__ void k_twiddles(F *out, int L, F w) {
+    // out[i] = w^i for i < 2^L; each thread computes a chunk by repeated multiplication
+    const size_t N = (size_t)1 << L;
+    const size_t chunk = 64;
+    size_t start = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * chunk;
+    if (start >= N) return;
+    // w^start by square-and-multiply
+    F acc = fone(), b = w;
+    size_t e = start;
+    while (e) { if (e & 1) acc = fmul(acc, b); b = fmul(b, b); e >>= 1; }
+    for (size_t i = start; i < start + chunk && i < N; i++) { out[i] = acc; acc = fmul(acc, w); }
+}
+void twiddles(F *out, int L, hipStream_t s) {
+    // generator of order 2^L: 3^((p-1)/2^L)
+    F g = fpow(fsmall(3), (uint64_t)(P - 1) >> L);
+    size_t N = (size_t)1 << L;
+    k_twiddles<<<nblk((N + 63) / 64), 256, 0, s>>>(out, L, g);
+}
+
+// In-LDS helpers on a tile of T rows, row stride `ld`, row length 2^lg, data held
+// in bit-reversed positions.
+__device__ __forceinline__ void lds_mobius(F *tile, int T, int ld, int lg, uint32_t natural_mask) {
+    for (int b = 0; b < lg; b++) {
+        if (!((natural_mask >> b) & 1)) continue;
+        int pb = lg - 1 - b;
+        int half = 1 << (lg - 1);
+        for (int i = threadIdx.x; i < T * half; i += blockDim.x) {
+            int t = i >> (lg - 1), j = i & (half - 1);
+            int lo = ((j >> pb) << (pb + 1)) | (j & ((1 << pb) - 1));
+            int hi = lo | (1 << pb);
+            F *row = tile + t * ld;
+            row[hi] = fsub(row[hi], row[lo]);
+        }
+        __syncthreads();
+    }
+}
+__device__ __forceinline__ void lds_dit(F *tile, int T, int ld, int lg, const F *tw /* w_{2^lg}^k, k < 2^(lg-1) */) {
+    int halfN = 1 << (lg - 1);
+    for (int s = 1; s <= lg; s++) {
+        int half = 1 << (s - 1);
+        int tstride = 1 << (lg - s);
+        for (int i = threadIdx.x; i < T * halfN; i += blockDim.x) {
+            int t = i >> (lg - 1), j = i & (halfN - 1);
+            int k = j & (half - 1);
+            int i0 = ((j >> (s - 1)) << s) | k;
+            int i1 = i0 + half;
+            F *row = tile + t * ld;
+            F u = row[i0], v = fmul(row[i1], tw[k * tstride]);
+            row[i0] = fadd(u, v);
+            row[i1] = fsub(u, v);
+        }
+        __syncthreads();
+    }
+}
+
+// Whole column in LDS (2^L <= 4096): Mobius over the log_m evaluation bits, NTT.
+__global__ void k_ntt_small(const F *cols, int log_m, int L, const F *tw_full, F *cw) {
+    extern __shared__ F sm[];
+    const int N = 1 << L;
+    F *tile = sm;
+    F *tw = sm + N;
+    const F *col = cols + ((size_t)blockIdx.x << log_m);
+    F *out = cw + ((size_t)blockIdx.x << L);
+    const size_t m = (size_t)1 << log_m;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) tile[brev(i, L)] = (size_t)i < m ? col[i] : F{0};
+    for (int i = threadIdx.x; i < N / 2; i += blockDim.x) tw[i] = tw_full[i];
+    __syncthreads();
+    lds_mobius(tile, 1, N, L, (uint32_t)(m - 1));
+    lds_dit(tile, 1, N, L, tw);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) out[i] = tile[i];
+}
+
+__global__ void k_ntt_a(const F *cols, int log_m, int L1, int L2, int T, const F *tw_full, F *Y) {
+    extern __shared__ F sm[];
+    const int N1 = 1 << L1, N2 = 1 << L2, ld = N1 + 1;
+    const int L = L1 + L2;
+    F *tile = sm;
+    F *tw = sm + T * ld;
+    const size_t m = (size_t)1 << log_m;
+    const F *col = cols + ((size_t)blockIdx.y << log_m);
+    F *y = Y + ((size_t)blockIdx.y << L);
+    const int n2_0 = blockIdx.x * T;
+    for (int i = threadIdx.x; i < T * N1; i += blockDim.x) {
+        int n1 = i / T, t = i % T;
+        size_t n = (size_t)N2 * n1 + n2_0 + t;
+        tile[t * ld + brev(n1, L1)] = n < m ? col[n] : F{0};
+    }
+    for (int i = threadIdx.x; i < N1 / 2; i += blockDim.x) tw[i] = tw_full[(size_t)i << L2];
+    __syncthreads();
+    // evaluation bits of n1: natural bit b of n1 is bit L2 + b of n
+    uint32_t mask = 0;
+    for (int b = 0; b < L1; b++) if (L2 + b < log_m) mask |= 1u << b;
+    lds_mobius(tile, T, ld, L1, mask);
+    lds_dit(tile, T, ld, L1, tw);
+    for (int i = threadIdx.x; i < T * N1; i += blockDim.x) {
+        int k1 = i / T, t = i % T;
+        y[(size_t)k1 * N2 + n2_0 + t] = tile[t * ld + k1];
+    }
+}
+
+__global__ void k_ntt_b(const F *Y, int log_m, int L1, int L2, int T, const F *tw_full, F *cw) {
+    extern __shared__ F sm[];
+    const int N1 = 1 << L1, N2 = 1 << L2, ld = N2 + 1;
+    const int L = L1 + L2;
+    const size_t Nmask = ((size_t)1 << L) - 1;
+    F *tile = sm;
+    F *tw = sm + T * ld;
+    const F *y = Y + ((size_t)blockIdx.y << L);
+    F *out = cw + ((size_t)blockIdx.y << L);
+    const int k1_0 = blockIdx.x * T;
+    for (int i = threadIdx.x; i < T * N2; i += blockDim.x) {
+        int t = i / N2, n2 = i % N2;
+        tile[t * ld + brev(n2, L2)] = y[(size_t)(k1_0 + t) * N2 + n2];
+    }
+    for (int i = threadIdx.x; i < N2 / 2; i += blockDim.x) tw[i] = tw_full[(size_t)i << L1];
+    __syncthreads();
+    uint32_t mask = 0;
+    for (int b = 0; b < L2; b++) if (b < log_m) mask |= 1u << b;
+    lds_mobius(tile, T, ld, L2, mask);
+    for (int i = threadIdx.x; i < T * N2; i += blockDim.x) {
+        int t = i / N2, p = i % N2;
+        int n2 = brev(p, L2);
+        size_t e = ((size_t)n2 * (size_t)(k1_0 + t)) & Nmask;
+        tile[t * ld + p] = fmul(tile[t * ld + p], tw_full[e]);
+    }
+    __syncthreads();
+    lds_dit(tile, T, ld, L2, tw);
+    for (int i = threadIdx.x; i < T * N2; i += blockDim.x) {
+        int k2 = i / T, t = i % T;
+        out[(size_t)(k1_0 + t) + (size_t)N1 * k2] = tile[t * ld + k2];
+    }
+}
+
+void rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int L, F *cw, F *tmp,
+               const F *tw_full, hipStream_t s) {
+    size_t rows = (size_t)1 << L;
+    if (n_nonzero < n_cols) fill_zero(cw + (size_t)n_nonzero * rows, (size_t)(n_cols - n_nonzero) * rows * 4, s);
+    if (n_nonzero <= 0) return;
+    if (L <= 12) {
+        size_t sh = ((size_t)1 << L) * 4 + ((size_t)1 << L) / 2 * 4 + 16;
+        k_ntt_small<<<n_nonzero, 256, sh, s>>>(cols, log_m, L, tw_full, cw);
+        return;
+    }
+    int L1 = L / 2, L2 = L - L1;
+    // tiles of T transforms: T * (len + 1) + len / 2 words must fit 64 KiB of LDS
+    auto tile_count = [](int lg, int cap) {
+        int len = 1 << lg, T = 64;
+        while (T > 1 && (size_t)T * (len + 1) + len / 2 > 16384) T >>= 1;
+        return T < cap ? T : cap;
+    };
+    int Ta = tile_count(L1, 1 << L2);
+    int Tb = tile_count(L2, 1 << L1);
+    size_t sha = ((size_t)Ta * ((1 << L1) + 1) + (1 << L1) / 2) * 4;
+    size_t shb = ((size_t)Tb * ((1 << L2) + 1) + (1 << L2) / 2) * 4;
+    k_ntt_a<<<dim3((1 << L2) / Ta, n_nonzero), 256, sha, s>>>(cols, log_m, L1, L2, Ta, tw_full, tmp);
+    k_ntt_b<<<dim3((1 << L1) / Tb, n_nonzero), 256, shb, s>>>(tmp, log_m, L1, L2, Tb, tw_full, cw);
+}
+
+// ---------------------------------------------------------------- Merkle ---
+__global__ void __launch_bounds__(256) k_merkle_leaves(const F *cw, int width, size_t rows,
+                                                      const P2DevT *pc, F *nodes) {
+    size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    F s[24];
+#pragma unroll
+    for (int i = 0; i < 24; i++) s[i] = F{0};
+    for (int off = 0; off < width; off += 16) {
+        int n = width - off < 16 ? width - off : 16;
+        if (n == 16) {
+#pragma unroll
+            for (int j = 0; j < 16; j++) s[j] = cw[(size_t)(off + j) * rows + r];
+        } else {
+            for (int j = 0; j < n; j++) s[j] = cw[(size_t)(off + j) * rows + r];
+        }
+        p24_permute(s, pc->p24);
+    }
+    F *o = nodes + (rows + r) * 8;
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = s[i];
+}
+
+__device__ __forceinline__ void compress_node(F *nodes, size_t i, const P2DevT *pc) {
+    F s[16];
+    const F *l = nodes + 2 * i * 8;
+#pragma unroll
+    for (int j = 0; j < 16; j++) s[j] = l[j];
+    p16_permute(s, pc->p16);
+    F *o = nodes + i * 8;
+#pragma unroll
+    for (int j = 0; j < 8; j++) o[j] = s[j];
+}
+
+__global__ void __launch_bounds__(256) k_merkle_level(F *nodes, size_t cnt, const P2DevT *pc) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    compress_node(nodes, cnt + i, pc);
+}
+
+__global__ void __launch_bounds__(256) k_merkle_top(F *nodes, size_t cnt, const P2DevT *pc) {
+    // levels with parents [cnt, 2cnt), [cnt/2, cnt), ... , [1, 2) in one block
+    for (size_t c = cnt; c >= 1; c >>= 1) {
+        for (size_t i = threadIdx.x; i < c; i += blockDim.x) compress_node(nodes, c + i, pc);
+        __syncthreads();
+        __threadfence_block();
+    }
+}
+
+void merkle_build(const F *cw, int width, int log_rows, const void *pc, const void *unused, F *nodes,
+                  hipStream_t s) {
+    (void)unused;
+    const P2DevT *p = (const P2DevT *)pc;
+    size_t rows = (size_t)1 << log_rows;
+    k_merkle_leaves<<<nblk(rows), 256, 0, s>>>(cw, width, rows, p, nodes);
+    size_t cnt = rows >> 1;
+    while (cnt > 256) {
+        k_merkle_level<<<nblk(cnt), 256, 0, s>>>(nodes, cnt, p);
+        cnt >>= 1;
+    }
+    if (cnt >= 1) k_merkle_top<<<1, 256, 0, s>>>(nodes, cnt, p);
+}
+
+__global__ void k_merkle_open(const F *cw, const F *nodes, int width, int log_rows,
+                              const uint32_t *idx, F *rows_out, F *paths_out) {
+    int q = blockIdx.x;
+    size_t rows = (size_t)1 << log_rows;
+    uint32_t r = idx[q];
+    for (int c = threadIdx.x; c < width; c += blockDim.x) rows_out[(size_t)q * width + c] = cw[(size_t)c * rows + r];
+    for (int t = threadIdx.x; t < log_rows * 8; t += blockDim.x) {
+        int l = t / 8, j = t % 8;
+        size_t node = ((rows + r) >> l) ^ 1;
+        paths_out[((size_t)q * log_rows + l) * 8 + j] = nodes[node * 8 + j];
+    }
+}
+void merkle_open(const F *cw, const F *nodes, int width, int log_rows, const uint32_t *idx, int nq,
+                 F *rows_out, F *paths_out, hipStream_t s) {
+    if (nq > 0) k_merkle_open<<<nq, 128, 0, s>>>(cw, nodes, width, log_rows, idx, rows_out, paths_out);
+}
+
+// ----------------------------------------------------------------- grinding --
+__global__ void __launch_bounds__(256) k_grind(const P2DevT *pc, const F *st, int pos, uint32_t mask,
+                                              uint32_t base, uint32_t count, uint32_t *found) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    uint32_t w = base + g;
+    if (w >= P) return;
+    F s[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) s[i] = st[i];
+    s[pos] = to_mont(w);
+    p16_permute(s, pc->p16);
+    if ((from_mont(s[7]) & mask) == 0) atomicMin(found, w);
+}
+void grind(const void *pc, const F *state16, int pos, uint32_t mask, uint32_t base, uint32_t count,
+           uint32_t *found, hipStream_t s) {
+    k_grind<<<nblk(count), 256, 0, s>>>((const P2DevT *)pc, state16, pos, mask, base, count, found);
+}
+
+__global__ void k_p2_batch(const P2DevT *pc, int width, F *st, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (width == 16) {
+        F s[16];
+        for (int j = 0; j < 16; j++) s[j] = st[(size_t)i * 16 + j];
+        p16_permute(s, pc->p16);
+        for (int j = 0; j < 16; j++) st[(size_t)i * 16 + j] = s[j];
+    } else {
+        F s[24];
+        for (int j = 0; j < 24; j++) s[j] = st[(size_t)i * 24 + j];
+        p24_permute(s, pc->p24);
+        for (int j = 0; j < 24; j++) st[(size_t)i * 24 + j] = s[j];
+    }
+}
+void p2_permute_batch(const void *pc, int width, F *states, int n, hipStream_t s) {
+    k_p2_batch<<<nblk(n), 256, 0, s>>>((const P2DevT *)pc, width, states, n);
+}
+
+// ------------------------------------------------------- WHIR reductions ----
+// Lazy EF x F accumulator (64-bit per coefficient, reduced every 4 terms).
+struct AccF2 {
+    uint64_t s[8];
+    EF r;
+    int n;
+    __device__ __forceinline__ AccF2() : n(0) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) s[i] = 0;
+        r = ezero();
+    }
+    __device__ __forceinline__ void flush() {
+#pragma unroll
+        for (int i = 0; i < 8; i++) { r.c[i] = fadd(r.c[i], F{redc64(s[i])}); s[i] = 0; }
+        n = 0;
+    }
+    __device__ __forceinline__ void add(const EF &a, F b) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) s[i] += (uint64_t)a.c[i].v * b.v;
+        if (++n == 4) flush();
+    }
+    __device__ __forceinline__ EF get() { flush(); return r; }
+};
+
+constexpr int T_JG = 2;
+__global__ void k_whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts) {
+    const size_t X = (size_t)1 << (m - k);
+    const int U = 1 << k;
+    const int u = blockIdx.y;
+    const int j0 = blockIdx.z * T_JG;
+    AccF2 acc[T_JG];
+    const size_t st = (size_t)gridDim.x * blockDim.x;
+    const F *fu = f + (size_t)u * X;
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < X; x += st) {
+        F v = fu[x];
+#pragma unroll
+        for (int jj = 0; jj < T_JG; jj++)
+            if (j0 + jj < nE) acc[jj].add(E[(size_t)(j0 + jj) * X + x], v);
+    }
+    __shared__ EF lds[16];
+#pragma unroll
+    for (int jj = 0; jj < T_JG; jj++) {
+        EF t = block_sum(acc[jj].get(), lds);
+        if (threadIdx.x == 0 && j0 + jj < nE) parts[((size_t)(j0 + jj) * U + u) * gridDim.x + blockIdx.x] = t;
+    }
+}
+int whir_T_chunks(int m, int k) {
+    size_t X = (size_t)1 << (m - k);
+    unsigned nc = nblk(X, 256 * 16);
+    return nc < 1 ? 1 : (int)nc;
+}
+int whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts, hipStream_t s) {
+    unsigned nc = (unsigned)whir_T_chunks(m, k);
+    k_whir_T_f<<<dim3(nc, 1 << k, (nE + T_JG - 1) / T_JG), 256, 0, s>>>(f, m, k, E, nE, parts);
+    return (int)nc;
+}
+
+__global__ void k_whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out) {
+    const size_t X = (size_t)1 << (m - k);
+    size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= X) return;
+    AccF2 acc;
+    for (int u = 0; u < (1 << k); u++) acc.add(eqr[u], f[(size_t)u * X + x]);
+    out[x] = acc.get();
+}
+void whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, hipStream_t s) {
+    size_t X = (size_t)1 << (m - k);
+    k_whir_fold_f<<<nblk(X), 256, 0, s>>>(f, m, k, eqr, out);
+}
+
+__global__ void k_build_w(EF *W, size_t Nw, int lo, int nF, const EF *scalF, const F *hiF,
+                          const F *loF, int nE, const EF *scalE, const EF *hiE, const EF *loE,
+                          bool accumulate) {
+    size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= Nw) return;
+    const size_t LO = (size_t)1 << lo, HI = Nw >> lo;
+    size_t xh = x >> lo, xl = x & (LO - 1);
+    AccF2 acc;
+    for (int j = 0; j < nF; j++) acc.add(scalF[j], fmul(hiF[(size_t)j * HI + xh], loF[(size_t)j * LO + xl]));
+    EF r = acc.get();
+    for (int j = 0; j < nE; j++)
+        r = eadd(r, emul(scalE[j], emul(hiE[(size_t)j * HI + xh], loE[(size_t)j * LO + xl])));
+    W[x] = accumulate ? eadd(W[x], r) : r;
+}
+void whir_build_w(EF *W, int m, int lo, int nF, const EF *scalF, const F *hiF, const F *loF, int nE,
+                  const EF *scalE, const EF *hiE, const EF *loE, bool accumulate, hipStream_t s) {
+    size_t Nw = (size_t)1 << m;
+    k_build_w<<<nblk(Nw), 256, 0, s>>>(W, Nw, lo, nF, scalF, hiF, loF, nE, scalE, hiE, loE, accumulate);
+}
+
+__global__ void k_eq_univ_f(const F *ys, int nY, int n, F *out) {
+    size_t N = (size_t)1 << n;
+    size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (x >= N || j >= nY) return;
+    F y = ys[j];
+    F acc = fone();
+    for (int b = 0; b < n; b++) {
+        acc = fmul(acc, ((x >> b) & 1) ? y : fsub(fone(), y));
+        y = fmul(y, y);
+    }
+    out[(size_t)j * N + x] = acc;
+}
+void eq_univ_f_tables(const F *ys, int nY, int n, F *out, hipStream_t s) {
+    if (nY <= 0) return;
+    size_t N = (size_t)1 << n;
+    k_eq_univ_f<<<dim3(nblk(N), nY), 256, 0, s>>>(ys, nY, n, out);
+}
+
+__global__ void k_ef_to_cols(const EF *in, size_t X, int K, F *out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int b = blockIdx.y;
+    if (i >= X) return;
+    EF v = in[(size_t)b * X + i];
+#pragma unroll
+    for (int c = 0; c < 8; c++) out[((size_t)b * 8 + c) * X + i] = v.c[c];
+}
+void ef_to_cols(const EF *in, int m, int k, F *out, hipStream_t s) {
+    size_t X = (size_t)1 << (m - k);
+    k_ef_to_cols<<<dim3(nblk(X), 1 << k), 256, 0, s>>>(in, X, 1 << k, out);
+}
+
+}  // namespace wk

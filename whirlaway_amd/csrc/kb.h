@@ -1,0 +1,176 @@
+// kb.h -- KoalaBear field, degree-8 extension and Poseidon2, shared by host and
+// device code of the MI355X prover.
+//
+// F: p = 2^31 - 2^24 + 1 held in Montgomery form (R = 2^32) in one 32-bit VALU
+// lane; products are 64-bit (v_mul_lo/v_mul_hi) and reduced by REDC, which for
+// this p needs no division.  Sums of up to four products are accumulated in
+// 64 bits before one reduction (redc64).
+// EF: F[X]/(X^8 - 3), eight consecutive u32 (coefficient 0 first), matching
+// BinomialExtensionField<KoalaBear, 8> named at src/examples/poseidon2.rs:31.
+// Poseidon2: Plonky3 layer structure, S-box x^3, width 16 (20 partial rounds)
+// and 24 (23 partial rounds); constants live in a P2Consts block (Montgomery
+// form) read with uniform (scalar) loads.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KB_HD __host__ __device__ __forceinline__
+
+namespace kb {
+
+constexpr uint32_t P = 0x7f000001u;
+constexpr uint32_t MU = 0x81000001u;          // P^-1 mod 2^32
+constexpr uint32_t R2 = 0x17f7efe4u;          // 2^64 mod P (to Montgomery)
+constexpr uint32_t ONE_M = 0x01fffffeu;       // 2^32 mod P (Montgomery 1)
+constexpr uint32_t W_EXT = 3u;                // X^8 = W
+
+struct F { uint32_t v; };
+
+KB_HD uint32_t redc(uint64_t x) {  // x < 2^32 * P  ->  x / 2^32 mod P
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    uint32_t m = lo * MU;
+    uint32_t mh = (uint32_t)(((uint64_t)m * P) >> 32);
+    uint32_t r = hi - mh;
+    return hi < mh ? r + P : r;
+}
+
+KB_HD uint32_t redc64(uint64_t x) {  // any x < 2^64
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    uint32_t m = lo * MU;
+    uint32_t mh = (uint32_t)(((uint64_t)m * P) >> 32);
+    uint32_t r = hi - mh;
+    if (hi < mh) return r + P;
+    if (r >= P) r -= P;
+    if (r >= P) r -= P;
+    return r;
+}
+
+KB_HD F fadd(F a, F b) { uint32_t s = a.v + b.v; return F{s >= P ? s - P : s}; }
+KB_HD F fsub(F a, F b) { uint32_t d = a.v - b.v; return F{a.v < b.v ? d + P : d}; }
+KB_HD F fneg(F a) { return F{a.v ? P - a.v : 0u}; }
+KB_HD F fdbl(F a) { return fadd(a, a); }
+KB_HD F fmul(F a, F b) { return F{redc((uint64_t)a.v * b.v)}; }
+KB_HD F fzero() { return F{0}; }
+KB_HD F fone() { return F{ONE_M}; }
+KB_HD F to_mont(uint32_t canon) { return F{redc((uint64_t)canon * R2)}; }
+KB_HD uint32_t from_mont(F a) { return redc((uint64_t)a.v); }
+KB_HD F fcube(F x) { return fmul(fmul(x, x), x); }
+KB_HD bool feq(F a, F b) { return a.v == b.v; }
+
+inline F fpow(F a, uint64_t e) {
+    F r = fone();
+    while (e) { if (e & 1) r = fmul(r, a); a = fmul(a, a); e >>= 1; }
+    return r;
+}
+inline F finv(F a) { return fpow(a, P - 2); }
+inline F fsmall(uint32_t c) { return to_mont(c % P); }
+
+struct EF { F c[8]; };
+
+KB_HD EF ezero() { EF r; for (int i = 0; i < 8; i++) r.c[i] = F{0}; return r; }
+KB_HD EF eone() { EF r = ezero(); r.c[0] = fone(); return r; }
+KB_HD EF efrom(F a) { EF r = ezero(); r.c[0] = a; return r; }
+KB_HD EF eadd(const EF &a, const EF &b) { EF r; for (int i = 0; i < 8; i++) r.c[i] = fadd(a.c[i], b.c[i]); return r; }
+KB_HD EF esub(const EF &a, const EF &b) { EF r; for (int i = 0; i < 8; i++) r.c[i] = fsub(a.c[i], b.c[i]); return r; }
+KB_HD EF eneg(const EF &a) { EF r; for (int i = 0; i < 8; i++) r.c[i] = fneg(a.c[i]); return r; }
+KB_HD EF edbl(const EF &a) { return eadd(a, a); }
+KB_HD EF emulf(const EF &a, F b) { EF r; for (int i = 0; i < 8; i++) r.c[i] = fmul(a.c[i], b); return r; }
+KB_HD bool eeq(const EF &a, const EF &b) { for (int i = 0; i < 8; i++) if (a.c[i].v != b.c[i].v) return false; return true; }
+
+KB_HD F ftimes3(F a) { return fadd(fdbl(a), a); }
+
+// c_k = sum_{i<=k} a_i b_{k-i} + sum_{i>k} a_i (3 b_{k+8-i}); two 4-term
+// 64-bit partial sums per coefficient, each reduced once.
+KB_HD EF emul(const EF &a, const EF &b) {
+    uint32_t b3[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) b3[j] = ftimes3(b.c[j]).v;
+    EF r;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        uint64_t s0 = 0, s1 = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            uint32_t bb = i <= k ? b.c[k - i].v : b3[k + 8 - i];
+            uint64_t p = (uint64_t)a.c[i].v * bb;
+            if (i < 4) s0 += p; else s1 += p;
+        }
+        r.c[k] = fadd(F{redc64(s0)}, F{redc64(s1)});
+    }
+    return r;
+}
+
+KB_HD EF esqr(const EF &a) { return emul(a, a); }
+KB_HD EF ecube(const EF &a) { return emul(esqr(a), a); }
+
+// ---------------------------------------------------------------- Poseidon2 --
+// Constants block (Montgomery form): ext_init[4][W], ext_term[4][W], internal[RP].
+struct P2Consts16 { F ext_init[4][16]; F ext_term[4][16]; F internal[20]; };
+struct P2Consts24 { F ext_init[4][24]; F ext_term[4][24]; F internal[23]; };
+
+// Internal diagonal (canonical): 16: [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, 1/2^8,
+// 1/8, 1/2^24, -1/2^8, -1/8, -1/16, -1/2^24]; 24: [-2, 1, 2, 1/2, 3, 4, -1/2,
+// -3, -4, 1/2^8, 1/4, 1/8, 1/16, 1/32, 1/64, 1/2^24, -1/2^8, -1/8, -1/16,
+// -1/32, -1/64, -1/2^7, -1/2^9, -1/2^24].  Montgomery images precomputed.
+struct Diag { uint32_t d16[16]; uint32_t d24[24]; };
+
+KB_HD void mat4(F *x) {
+    F t01 = fadd(x[0], x[1]), t23 = fadd(x[2], x[3]);
+    F t0123 = fadd(t01, t23);
+    F t01123 = fadd(t0123, x[1]);
+    F t01233 = fadd(t0123, x[3]);
+    F y3 = fadd(t01233, fdbl(x[0]));
+    F y1 = fadd(t01123, fdbl(x[2]));
+    F y0 = fadd(t01123, t01);
+    F y2 = fadd(t01233, t23);
+    x[0] = y0; x[1] = y1; x[2] = y2; x[3] = y3;
+}
+
+template <int W>
+KB_HD void mds_light(F *s) {
+#pragma unroll
+    for (int c = 0; c < W; c += 4) mat4(s + c);
+    F sums[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        F acc = s[k];
+#pragma unroll
+        for (int j = k + 4; j < W; j += 4) acc = fadd(acc, s[j]);
+        sums[k] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < W; i++) s[i] = fadd(s[i], sums[i & 3]);
+}
+
+template <int W>
+KB_HD void internal_layer(F *s, const uint32_t *diag) {
+    F sum = s[0];
+#pragma unroll
+    for (int i = 1; i < W; i++) sum = fadd(sum, s[i]);
+#pragma unroll
+    for (int i = 0; i < W; i++) s[i] = fadd(sum, fmul(s[i], F{diag[i]}));
+}
+
+template <int W, int RP, typename C>
+KB_HD void p2_permute(F *s, const C &k, const uint32_t *diag) {
+    mds_light<W>(s);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < W; i++) s[i] = fcube(fadd(s[i], k.ext_init[r][i]));
+        mds_light<W>(s);
+    }
+#pragma unroll 4
+    for (int r = 0; r < RP; r++) {
+        s[0] = fcube(fadd(s[0], k.internal[r]));
+        internal_layer<W>(s, diag);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < W; i++) s[i] = fcube(fadd(s[i], k.ext_term[r][i]));
+        mds_light<W>(s);
+    }
+}
+
+}  // namespace kb

@@ -1,0 +1,120 @@
+// kernels.h -- host-side launchers for the HIP kernels of the prover.
+// All launchers enqueue on the given stream and never synchronize.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include "kb.h"
+
+namespace wk {
+using kb::EF;
+using kb::F;
+
+constexpr int AIR_NCOLS = 165;
+constexpr int AIR_NCONS = 149;
+constexpr int AIR_NRC = 4 * 16 + 20 + 4 * 16;  // beginning, partial, ending
+
+struct AirConsts { F rc[AIR_NRC]; };  // Montgomery form, beginning[4][16], partial[20], ending[4][16]
+
+// -------------------------------------------------------------- basic --
+void convert_to_mont(uint32_t *d, size_t n, hipStream_t s);
+void convert_from_mont(uint32_t *d, size_t n, hipStream_t s);
+void fill_zero(void *d, size_t bytes, hipStream_t s);
+// out[i] = sum_p in[i * n_parts + p]  (n_items outputs)
+void reduce_parts(const EF *in, int n_items, int n_parts, EF *out, hipStream_t s);
+
+// eq(point, .) over n variables, big-endian, times scale; point on device.
+void eq_table(const EF *d_point, int n, const EF *d_scale, EF *out, hipStream_t s);
+
+// ------------------------------------------------------------ zerocheck --
+// Round 0 (univariate skip, 2^skips nodes): for each of nz evaluation points z
+// (selector rows sel[nz][2^skips], F), partial sums over blocks of
+//   sum_x eq[x] * AIR(fold_z(up columns))(x)
+// parts: [nz][n_blocks] EF; returns n_blocks.
+int zc_round0(const F *trace, int log_n, int skips, const F *sel, int nz, const AirConsts *air,
+              const EF *alpha_pows, const EF *eq, EF *parts, hipStream_t s);
+// U[c][x] = sum_j S_j * up_c[j*M + x]  (S: 2^skips EF on device)
+void zc_fold0(const F *trace, int log_n, int skips, const EF *S, EF *U, hipStream_t s);
+// Rounds >= 1: U holds 165 EF columns (column c at U + c * stride) of current
+// length 2^l.  For z in {0,2,3}: parts[zi][block] partial sums of
+// eq[x] * AIR(U_z)(x), x < 2^(l-1).
+int zc_round_strided(const EF *U, size_t stride, int l, const AirConsts *air,
+                     const EF *alpha_pows, const EF *eq, EF *parts, hipStream_t s);
+// In-place fold of n_cols EF columns by (1-c, c): col[x] += c (col[x+h] - col[x]);
+// the column stride is unchanged.
+void fold_cols_strided(EF *U, int n_cols, size_t stride, int l, const EF *c, hipStream_t s);
+// down inner sums: parts[c][j] = sum_{x'} eqr[x'] * down_c(j*M + x'), M = 2^(log_n-skips)
+void zc_down_parts(const F *trace, int log_n, int skips, const EF *eqr, EF *parts, hipStream_t s);
+// out[c] = sum_j S[j] * parts[c][j]
+void zc_down_combine(const EF *parts, int skips, const EF *S, EF *out, hipStream_t s);
+
+// ------------------------------------------------------------- batching --
+// B[x] = sum_{c<165} eqc[c] * col_c[x]
+void lincomb_cols(const F *trace, int log_n, int n_cols, const EF *eqc, EF *B, hipStream_t s);
+// mixed[x] = up(B)[x] + a * down(B)[x]
+void mix_updown(const EF *B, int log_n, const EF *a, EF *mixed, hipStream_t s);
+// M[y] = matrix_up_folded + a * matrix_down_folded, from E = eq(point, .)
+void build_matrix(const EF *E, int log_n, const EF *a, EF *M, hipStream_t s);
+// parts[j][b]: block partials of sum_{i<len} A[j*len + i] * E[i], j < nseg; returns n_blocks
+int seg_dot(const EF *A, const EF *E, size_t len, int nseg, EF *parts, hipStream_t s);
+int seg_dot_f(const F *A, const EF *E, size_t len, int nseg, EF *parts, hipStream_t s);
+
+// number of block partials produced by seg_dot / prod_round / a2b_round over len items
+int dot_blocks(size_t len);
+// number of x-chunks produced by whir_T_f
+int whir_T_chunks(int m, int k);
+
+// ------------------------------------------------------ product sumcheck --
+// parts[0][b] = sum_{x<h} A[x] B[x], parts[1][b] = sum (2A[x+h]-A[x])(2B[x+h]-B[x]); h = 2^(l-1)
+int prod_round(const EF *A, const EF *B, int l, EF *parts, hipStream_t s);
+// in-place fold of one EF table by (1-c, c)
+void fold_ef(EF *A, int l, const EF *c, hipStream_t s);
+
+// -------------------------------------------------------------- micro --
+// fold an F table (2^l) by 2^k EF scalars: out[i] = sum_j s_j in[i + j 2^(l-k)]
+void fold_f_large(const F *in, int l, const EF *sc, int k, EF *out, hipStream_t s);
+void fold_ef_large(const EF *in, int l, const EF *sc, int k, EF *out, hipStream_t s);
+// eq-weighted a^2 b sumcheck round (degree 3, z in {0,2,3}, or {0,1,2,3} when
+// all4); F inputs (base) or EF; parts [z][block]
+int a2b_round_f(const F *a, const F *b, int l, const EF *eq, EF *parts, hipStream_t s, bool all4);
+int a2b_round_ef(const EF *a, const EF *b, int l, const EF *eq, EF *parts, hipStream_t s);
+// fold an F table by (1-c, c) into an EF table of half size
+void fold_f_to_ef(const F *in, int l, const EF *c, EF *out, hipStream_t s);
+
+// ----------------------------------------------------------------- WHIR --
+struct P2Dev;  // device Poseidon2 constants (16 and 24)
+
+// T[j][u] partial sums: sum_x f(u*2^(m-k) + x) * E_j(x), E_j EF tables (stride 2^(m-k)).
+// parts layout [j][u][chunk]; returns n_chunks.
+int whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts, hipStream_t s);
+// f'(x) = sum_u eqr[u] f(u*2^(m-k) + x)
+void whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, hipStream_t s);
+// W[x] = sum_j scal[j] * Ehi_j[x >> lo] * Elo_j[x & mask]; F-point tables and EF-point tables
+// (accumulate: W += ..., else W = ...)
+void whir_build_w(EF *W, int m, int lo, int nF, const EF *scalF, const F *hiF, const F *loF,
+                  int nE, const EF *scalE, const EF *hiE, const EF *loE, bool accumulate,
+                  hipStream_t s);
+// eq(expand(y_j, n), .) tables over F for nY points y (device), out [j][2^n]
+void eq_univ_f_tables(const F *ys, int nY, int n, F *out, hipStream_t s);
+// AoS EF (2^m) -> SoA F columns: out[(b*8 + c)*2^(m-k) + i] = in[b*2^(m-k) + i].c[c]
+void ef_to_cols(const EF *in, int m, int k, F *out, hipStream_t s);
+// RS encode n_cols columns of 2^log_m evaluations (contiguous) into columns of
+// 2^log_rows natural-order NTT values of the univariate embedding.
+void rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int log_rows, F *cw,
+               F *tmp, const F *tw_full, hipStream_t s);
+// twiddle table w_{2^L}^i for i < 2^L
+void twiddles(F *out, int L, hipStream_t s);
+// Merkle tree over rows of a column-major codeword (width F per leaf), heap layout nodes[2*rows][8]
+void merkle_build(const F *cw, int width, int log_rows, const void *p16, const void *p24, F *nodes,
+                  hipStream_t s);
+// gather rows (width F each) and sibling paths for sorted query indices
+void merkle_open(const F *cw, const F *nodes, int width, int log_rows, const uint32_t *idx, int nq,
+                 F *rows_out, F *paths_out, hipStream_t s);
+// minimal w in [base, base + count) with P16(state with w at position pos)[7] & mask == 0,
+// result via atomicMin into *found (init 0xffffffff)
+void grind(const void *p16, const F *state16, int pos, uint32_t mask, uint32_t base, uint32_t count,
+           uint32_t *found, hipStream_t s);
+// device Poseidon2 (tests): permute n states of width W in place
+void p2_permute_batch(const void *consts, int width, F *states, int n, hipStream_t s);
+
+}  // namespace wk

@@ -1,0 +1,221 @@
+// micro.hip -- C-ABI entry points for the kernel-level operations: folds
+// (crates/utils/src/multilinear.rs:90-123, benches/folding_benchmarks.rs), the
+// eq-weighted product sumcheck (crates/sumcheck/src/prove.rs:22-254 with
+// skips = 1), Poseidon2 permutations and the Reed-Solomon/Merkle commitment core.
+#include <hip/hip_runtime.h>
+#include "../../include/whirlaway_hip.h"
+#include "ctx.h"
+
+using namespace wl;
+
+namespace {
+
+void upload_canon(wl_ctx *c, uint32_t *dst, const uint32_t *src, size_t n) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, n * 4, hipMemcpyHostToDevice, c->stream));
+    wk::convert_to_mont(dst, n, c->stream);
+}
+void download_canon(wl_ctx *c, uint32_t *dst, uint32_t *src_dev, size_t n) {
+    wk::convert_from_mont(src_dev, n, c->stream);
+    c->d2h(dst, src_dev, n);
+}
+std::vector<EF> host_ef(const uint32_t *canon, size_t n) {
+    std::vector<EF> v(n);
+    for (size_t i = 0; i < n; i++) v[i] = wh::to_mont_ef(canon + 8 * i);
+    return v;
+}
+
+template <class Fn>
+int guard(wl_ctx *c, Fn fn) {
+    if (!c) { wl_set_error("null context"); return WL_ERR_ARG; }
+    try {
+        HIP_CHECK(hipSetDevice(c->device));
+        return fn();
+    } catch (const std::exception &e) {
+        wl_set_error(e.what());
+        return WL_ERR_DEVICE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int wl_fold_large(wl_ctx *c, const uint32_t *in, int in_is_ef, uint32_t log_n, const uint32_t *scalars,
+                  uint32_t log_k, uint32_t *out) {
+    if (!in || !scalars || !out || log_k > log_n) { wl_set_error("bad arguments"); return WL_ERR_ARG; }
+    return guard(c, [&]() {
+        size_t N = (size_t)1 << log_n, M = N >> log_k, K = (size_t)1 << log_k;
+        size_t w = in_is_ef ? 8 : 1;
+        DBuf<uint32_t> din(c->pool, N * w);
+        upload_canon(c, din, in, N * w);
+        DBuf<EF> dsc = upload_ef(c, host_ef(scalars, K));
+        DBuf<EF> dout(c->pool, M);
+        if (in_is_ef) wk::fold_ef_large((const EF *)din.p, (int)log_n, dsc, (int)log_k, dout, c->stream);
+        else wk::fold_f_large((const F *)din.p, (int)log_n, dsc, (int)log_k, dout, c->stream);
+        download_canon(c, out, (uint32_t *)dout.p, M * 8);
+        return WL_OK;
+    });
+}
+
+int wl_fold_chain(wl_ctx *c, const uint32_t *in, uint32_t log_n, const uint32_t *challenges,
+                  uint32_t n_folds, uint32_t *out_all, uint32_t *final_out, double *ms) {
+    if (!in || !challenges || n_folds < 1 || n_folds > log_n) { wl_set_error("bad arguments"); return WL_ERR_ARG; }
+    return guard(c, [&]() {
+        size_t N = (size_t)1 << log_n;
+        DBuf<F> din(c->pool, N);
+        upload_canon(c, (uint32_t *)din.p, in, N);
+        std::vector<EF> ch = host_ef(challenges, n_folds);
+        DBuf<EF> dch = upload_ef(c, ch);
+        DBuf<EF> cur(c->pool, N / 2);
+        HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+        wk::fold_f_to_ef(din, (int)log_n, dch.p, cur, c->stream);
+        for (uint32_t i = 1; i < n_folds; i++) wk::fold_ef(cur, (int)(log_n - i), dch.p + i, c->stream);
+        HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+        HIP_CHECK(hipEventSynchronize(c->ev1));
+        float t = 0;
+        HIP_CHECK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+        if (ms) *ms = t;
+        if (out_all) {
+            // recompute stage by stage so every intermediate table is returned
+            DBuf<EF> st(c->pool, N / 2);
+            wk::fold_f_to_ef(din, (int)log_n, dch.p, st, c->stream);
+            size_t off = 0;
+            for (uint32_t i = 0; i < n_folds; i++) {
+                if (i > 0) wk::fold_ef(st, (int)(log_n - i), dch.p + i, c->stream);
+                size_t len = (size_t)1 << (log_n - 1 - i);
+                DBuf<EF> cp(c->pool, len);
+                HIP_CHECK(hipMemcpyAsync(cp.p, st.p, len * sizeof(EF), hipMemcpyDeviceToDevice, c->stream));
+                download_canon(c, out_all + off * 8, (uint32_t *)cp.p, len * 8);
+                off += len;
+            }
+        }
+        if (final_out) download_canon(c, final_out, (uint32_t *)cur.p, ((size_t)1 << (log_n - n_folds)) * 8);
+        return WL_OK;
+    });
+}
+
+int wl_sumcheck_a2b(wl_ctx *c, const uint32_t *p16, const uint32_t *a, const uint32_t *b, uint32_t n,
+                    const uint32_t *z, uint32_t *challenges, uint32_t *final_evals, uint32_t *final_sum,
+                    uint32_t *proof_out, size_t proof_cap, size_t *proof_len, double *ms) {
+    if (!p16 || !a || !b || !z || n < 1) { wl_set_error("bad arguments"); return WL_ERR_ARG; }
+    return guard(c, [&]() {
+        size_t N = (size_t)1 << n;
+        P2Dev ph;
+        load_p2(p16, nullptr, ph);
+        DBuf<P2Dev> pd = upload_p2(c, ph);
+        wh::Transcript ts(&ph.p16);
+        std::vector<EF> zz = host_ef(z, n);
+        DBuf<F> da(c->pool, N), db(c->pool, N);
+        upload_canon(c, (uint32_t *)da.p, a, N);
+        upload_canon(c, (uint32_t *)db.p, b, N);
+        HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+        // claimed sum: (1 - z0) S(0) + z0 S(1) over the top variable
+        EF sum;
+        {
+            DBuf<EF> eq = eq_dev(c, std::vector<EF>(zz.begin() + 1, zz.end()), eone());
+            int nb = wk::dot_blocks(N / 2);
+            DBuf<EF> parts(c->pool, 4 * (size_t)nb);
+            wk::a2b_round_f(da, db, (int)n, eq, parts, c->stream, true);
+            std::vector<EF> s4 = reduce_many(c, parts, 4, nb);
+            sum = eadd(emul(esub(eone(), zz[0]), s4[0]), emul(zz[0], s4[1]));
+        }
+        EF missing = eone();
+        bool has_missing = false;
+        std::vector<EF> ch;
+        DBuf<EF> ea(c->pool, N / 2), eb(c->pool, N / 2);
+        for (uint32_t j = 0; j < n; j++) {
+            int l = (int)(n - j);
+            EF r = zz[j];
+            DBuf<EF> eq = eq_dev(c, std::vector<EF>(zz.begin() + 1 + j, zz.end()), eone());
+            int nb = wk::dot_blocks((size_t)1 << (l - 1));
+            DBuf<EF> parts(c->pool, 3 * (size_t)nb);
+            if (j == 0) wk::a2b_round_f(da, db, l, eq, parts, c->stream, false);
+            else wk::a2b_round_ef(ea, eb, l, eq, parts, c->stream);
+            std::vector<EF> v3 = reduce_many(c, parts, 3, nb);
+            EF e0 = v3[0], e2 = v3[1], e3 = v3[2];
+            if (has_missing) { e0 = emul(e0, missing); e2 = emul(e2, missing); e3 = emul(e3, missing); }
+            EF e1 = emul(esub(sum, emul(e0, esub(eone(), r))), wh::einv(r));
+            std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2), fsmall(3)};
+            std::vector<EF> p = wh::lagrange(xs, {e0, e1, e2, e3});
+            p = wh::pmul(p, {esub(eone(), r), esub(edbl(r), eone())});
+            ts.add_ef(p.data(), p.size());
+            EF cj = ts.sample_ef();
+            ch.push_back(cj);
+            sum = wh::peval(p, cj);
+            grind(c, ts, pd.p, pow_bits_sumcheck(128, 3 + 1));
+            EF f = eadd(emul(esub(eone(), r), esub(eone(), cj)), emul(r, cj));
+            missing = has_missing ? emul(f, missing) : f;
+            has_missing = true;
+            DBuf<EF> dc = upload_ef(c, std::vector<EF>(1, cj));
+            if (j == 0) {
+                wk::fold_f_to_ef(da, l, dc, ea, c->stream);
+                wk::fold_f_to_ef(db, l, dc, eb, c->stream);
+            } else {
+                wk::fold_ef(ea, l, dc, c->stream);
+                wk::fold_ef(eb, l, dc, c->stream);
+            }
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+        HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+        HIP_CHECK(hipEventSynchronize(c->ev1));
+        float t = 0;
+        HIP_CHECK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+        if (ms) *ms = t;
+        EF fa, fb;
+        c->d2h(&fa, ea.p, 1);
+        c->d2h(&fb, eb.p, 1);
+        for (uint32_t j = 0; j < n; j++) wh::from_mont_ef(ch[j], challenges + 8 * j);
+        wh::from_mont_ef(fa, final_evals);
+        wh::from_mont_ef(fb, final_evals + 8);
+        wh::from_mont_ef(sum, final_sum);
+        if (proof_len) *proof_len = ts.data.size();
+        if (proof_out) {
+            if (proof_cap < ts.data.size()) { wl_set_error("proof buffer too small"); return WL_ERR_BUFFER; }
+            memcpy(proof_out, ts.data.data(), ts.data.size() * 4);
+        }
+        return WL_OK;
+    });
+}
+
+int wl_poseidon2_permute(wl_ctx *c, const uint32_t *p16, const uint32_t *p24, int width, uint32_t *states,
+                         size_t n) {
+    if ((width != 16 && width != 24) || !states || (width == 16 ? !p16 : !p24)) {
+        wl_set_error("bad arguments");
+        return WL_ERR_ARG;
+    }
+    return guard(c, [&]() {
+        P2Dev ph;
+        load_p2(p16, p24, ph);
+        DBuf<P2Dev> pd = upload_p2(c, ph);
+        DBuf<uint32_t> ds(c->pool, n * width);
+        upload_canon(c, ds, states, n * width);
+        wk::p2_permute_batch(pd.p, width, (F *)ds.p, (int)n, c->stream);
+        download_canon(c, states, ds, n * width);
+        return WL_OK;
+    });
+}
+
+int wl_rs_commit(wl_ctx *c, const uint32_t *p16, const uint32_t *p24, const uint32_t *cols, uint32_t n_cols,
+                 uint32_t log_m, uint32_t log_rows, uint32_t *codeword_out, uint32_t *root_out) {
+    if (!p16 || !p24 || !cols || !root_out || log_rows < log_m || n_cols < 1) {
+        wl_set_error("bad arguments");
+        return WL_ERR_ARG;
+    }
+    return guard(c, [&]() {
+        P2Dev ph;
+        load_p2(p16, p24, ph);
+        DBuf<P2Dev> pd = upload_p2(c, ph);
+        size_t m = (size_t)1 << log_m, rows = (size_t)1 << log_rows;
+        DBuf<uint32_t> dcols(c->pool, n_cols * m);
+        upload_canon(c, dcols, cols, n_cols * m);
+        DBuf<F> cw(c->pool, n_cols * rows), tmp(c->pool, n_cols * rows), nodes(c->pool, 2 * rows * 8);
+        wk::rs_encode((const F *)dcols.p, (int)n_cols, (int)n_cols, (int)log_m, (int)log_rows, cw, tmp,
+                      c->twiddle((int)log_rows), c->stream);
+        wk::merkle_build(cw, (int)n_cols, (int)log_rows, pd.p, nullptr, nodes, c->stream);
+        download_canon(c, root_out, (uint32_t *)nodes.p + 8, 8);
+        if (codeword_out) download_canon(c, codeword_out, (uint32_t *)cw.p, n_cols * rows);
+        return WL_OK;
+    });
+}
+
+}  // extern "C"

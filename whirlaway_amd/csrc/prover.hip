@@ -1,0 +1,782 @@
+// prover.hip -- MI355X prover for the Poseidon2 AIR (Whirlaway hot path) and
+// the C ABI declared in include/whirlaway_hip.h.
+//
+// Protocol order follows crates/air/src/prove.rs:49-226 step by step:
+//   WHIR commit (prove.rs:80-91) -> alpha (:93-100) -> zerocheck challenges
+//   (:102-107) -> zerocheck sumcheck with univariate skip (:114-131,
+//   crates/sumcheck/src/prove.rs:22-254) -> inner sums (:135-145) -> column
+//   batching (:147-160) -> alpha2, sub_evals (:162-173) -> epsilons, inner
+//   sumcheck (:175-208) -> WHIR opening (:216-225).
+// The host side runs only the transcript and polylog-sized algebra; every pass
+// over trace- or codeword-sized data is a HIP kernel on the context's stream.
+#include <hip/hip_runtime.h>
+#include <chrono>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+#include "../../include/whirlaway_hip.h"
+#include "ctx.h"
+
+using namespace kb;
+using wh::Transcript;
+using wh::WhirCfg;
+
+using namespace wl;
+
+namespace {
+
+using wk::AIR_NCOLS;
+using wk::AIR_NCONS;
+
+// ------------------------------------------------------------------ WHIR --
+struct Codeword {
+    DBuf<F> cw;      // column-major [width][rows]
+    DBuf<F> nodes;   // heap [2 rows][8]
+    int width = 0;   // F per leaf
+    int log_rows = 0;
+    bool base = true;
+    int fold = 0;
+};
+
+void commit_codeword(wl_ctx *c, const P2Dev *pdev, Codeword &cwd, const F *cols, int n_cols,
+                     int n_nonzero, int log_m, int log_rows) {
+    size_t rows = (size_t)1 << log_rows;
+    cwd.cw = DBuf<F>(c->pool, (size_t)n_cols * rows);
+    cwd.nodes = DBuf<F>(c->pool, 2 * rows * 8);
+    cwd.width = n_cols;
+    cwd.log_rows = log_rows;
+    DBuf<F> tmp(c->pool, (size_t)std::max(1, n_nonzero) * rows);
+    wk::rs_encode(cols, n_cols, n_nonzero, log_m, log_rows, cwd.cw, tmp, c->twiddle(log_rows), c->stream);
+    wk::merkle_build(cwd.cw, n_cols, log_rows, pdev, nullptr, cwd.nodes, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+std::vector<uint32_t> sample_queries(Transcript &ts, int n, int log_rows) {
+    std::vector<uint32_t> q(n);
+    for (int i = 0; i < n; i++) q[i] = ts.sample_bits(log_rows);
+    std::sort(q.begin(), q.end());
+    q.erase(std::unique(q.begin(), q.end()), q.end());
+    return q;
+}
+
+// Open rows + paths as hints; returns the rows (Montgomery) for folding.
+std::vector<F> open_queries(wl_ctx *c, Transcript &ts, const Codeword &cwd, const std::vector<uint32_t> &q) {
+    int nq = (int)q.size();
+    std::vector<F> rows((size_t)nq * cwd.width), paths((size_t)nq * cwd.log_rows * 8);
+    if (nq == 0) return rows;
+    for (uint32_t x : q)
+        if ((size_t)x >= ((size_t)1 << cwd.log_rows)) throw std::runtime_error("query index outside the codeword");
+    DBuf<uint32_t> didx(c->pool, nq);
+    c->h2d(didx.p, q.data(), nq);
+    DBuf<F> drows(c->pool, rows.size()), dpaths(c->pool, paths.size() ? paths.size() : 1);
+    wk::merkle_open(cwd.cw, cwd.nodes, cwd.width, cwd.log_rows, didx, nq, drows, dpaths, c->stream);
+    c->d2h(rows.data(), drows.p, rows.size());
+    if (!paths.empty()) c->d2h(paths.data(), dpaths.p, paths.size());
+    for (int i = 0; i < nq; i++) {
+        ts.hint_base(&rows[(size_t)i * cwd.width], cwd.width);
+        ts.hint_base(&paths[(size_t)i * cwd.log_rows * 8], (size_t)cwd.log_rows * 8);
+    }
+    return rows;
+}
+
+EF fold_row(const F *row, bool base, int fold, const std::vector<EF> &eqr) {
+    EF acc = ezero();
+    int K = 1 << fold;
+    for (int b = 0; b < K; b++) {
+        if (base) acc = eadd(acc, emulf(eqr[b], row[b]));
+        else { EF v; memcpy(v.c, row + 8 * b, 32); acc = eadd(acc, emul(eqr[b], v)); }
+    }
+    return acc;
+}
+
+// A WHIR equality constraint on the current polynomial (m variables):
+// scal * eq(point, x); point kinds: univariate F (expand(y, m)), univariate EF
+// (expand(z, m)) or an explicit EF point.
+struct Constraint {
+    EF scal;
+    int kind;          // 0 explicit, 1 univariate EF, 2 univariate F
+    std::vector<EF> pt;
+    EF z;
+    F y;
+};
+
+std::vector<EF> cons_point(const Constraint &k, int m) {
+    if (k.kind == 0) return k.pt;
+    if (k.kind == 1) return wh::expand_univ(k.z, m);
+    return wh::expand_univ(efrom(k.y), m);
+}
+
+// After binding the top `r.size()` variables to r: scal *= eq(prefix, r), point -> suffix.
+void fold_constraint(Constraint &k, int m, const std::vector<EF> &r) {
+    int kk = (int)r.size();
+    std::vector<EF> pt = cons_point(k, m);
+    k.scal = emul(k.scal, wh::eq_outside(pt.data(), r.data(), kk));
+    if (k.kind == 0) k.pt.erase(k.pt.begin(), k.pt.begin() + kk);
+    // univariate kinds: expand(y, m)[kk..] = expand(y, m - kk): unchanged parameter
+}
+
+// W(x) += sum_k scal_k eq(point_k, x) over m variables (hi/lo tensor split).
+void add_constraints_w(wl_ctx *c, EF *W, int m, const std::vector<Constraint> &cs, bool zero_first) {
+    size_t Nw = (size_t)1 << m;
+    if (cs.empty()) {
+        if (zero_first) wk::fill_zero(W, Nw * sizeof(EF), c->stream);
+        return;
+    }
+    int lo = m / 2, hi = m - lo;
+    size_t LO = (size_t)1 << lo, HI = (size_t)1 << hi;
+    std::vector<EF> sF, sE;
+    std::vector<F> yl, yh;
+    std::vector<const Constraint *> efc;
+    for (const auto &k : cs) {
+        if (k.kind == 2) {
+            sF.push_back(k.scal);
+            yl.push_back(k.y);
+            F yy = k.y;
+            for (int i = 0; i < lo; i++) yy = fmul(yy, yy);
+            yh.push_back(yy);
+        } else {
+            sE.push_back(k.scal);
+            efc.push_back(&k);
+        }
+    }
+    int nF = (int)sF.size(), nE = (int)sE.size();
+    DBuf<F> hiF(c->pool, std::max<size_t>(1, nF * HI)), loF(c->pool, std::max<size_t>(1, nF * LO));
+    DBuf<EF> hiE(c->pool, std::max<size_t>(1, nE * HI)), loE(c->pool, std::max<size_t>(1, nE * LO));
+    DBuf<EF> dsF = upload_ef(c, sF), dsE = upload_ef(c, sE);
+    if (nF) {
+        DBuf<F> dyl(c->pool, nF), dyh(c->pool, nF);
+        c->h2d(dyl.p, yl.data(), nF);
+        c->h2d(dyh.p, yh.data(), nF);
+        wk::eq_univ_f_tables(dyh, nF, hi, hiF, c->stream);
+        wk::eq_univ_f_tables(dyl, nF, lo, loF, c->stream);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    std::vector<DBuf<EF>> keep;
+    for (int j = 0; j < nE; j++) {
+        std::vector<EF> pt = cons_point(*efc[j], m);
+        std::vector<EF> ph(pt.begin(), pt.begin() + hi), pl(pt.begin() + hi, pt.end());
+        std::vector<EF> up(ph);
+        up.insert(up.end(), pl.begin(), pl.end());
+        up.push_back(eone());
+        keep.push_back(upload_ef(c, up));
+        const EF *d = keep.back().p;
+        wk::eq_table(d, hi, d + m, hiE.p + (size_t)j * HI, c->stream);
+        wk::eq_table(d + hi, lo, d + m, loE.p + (size_t)j * LO, c->stream);
+    }
+    wk::whir_build_w(W, m, lo, nF, dsF, hiF, loF, nE, dsE, hiE, loE, !zero_first, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+struct SumcheckState {
+    DBuf<EF> f, W;
+    int m = 0;
+    EF sum;
+};
+
+// GPU sumcheck round pair (f, W), evaluations at 0,1,2 sent (whir-p3 convention).
+EF whir_sc_round(wl_ctx *c, Transcript &ts, const P2Dev *pdev, SumcheckState &s, int pow_bits) {
+    int nb = wk::dot_blocks((size_t)1 << (s.m - 1));
+    DBuf<EF> parts(c->pool, 2 * (size_t)nb);
+    wk::prod_round(s.f, s.W, s.m, parts, c->stream);
+    std::vector<EF> h02 = reduce_many(c, parts, 2, nb);
+    EF h[3] = {h02[0], esub(s.sum, h02[0]), h02[1]};
+    ts.add_ef(h, 3);
+    grind(c, ts, pdev, pow_bits);
+    EF r = ts.sample_ef();
+    s.sum = wh::interp3(h, r);
+    DBuf<EF> dr = upload_ef(c, std::vector<EF>(1, r));
+    wk::fold_ef(s.f, s.m, dr, c->stream);
+    wk::fold_ef(s.W, s.m, dr, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    s.m--;
+    return r;
+}
+
+// Host sumcheck round on small tables (final rounds).
+EF host_sc_round(Transcript &ts, wl_ctx *c, const P2Dev *pdev, std::vector<EF> &f, std::vector<EF> &W,
+                 EF &sum, int pow_bits) {
+    size_t h = f.size() / 2;
+    EF h0 = ezero(), h2 = ezero();
+    for (size_t i = 0; i < h; i++) {
+        h0 = eadd(h0, emul(f[i], W[i]));
+        h2 = eadd(h2, emul(esub(edbl(f[i + h]), f[i]), esub(edbl(W[i + h]), W[i])));
+    }
+    EF hv[3] = {h0, esub(sum, h0), h2};
+    ts.add_ef(hv, 3);
+    grind(c, ts, pdev, pow_bits);
+    EF r = ts.sample_ef();
+    sum = wh::interp3(hv, r);
+    for (size_t i = 0; i < h; i++) {
+        f[i] = eadd(f[i], emul(r, esub(f[i + h], f[i])));
+        W[i] = eadd(W[i], emul(r, esub(W[i + h], W[i])));
+    }
+    f.resize(h);
+    W.resize(h);
+    return r;
+}
+
+struct WhirWitness {
+    const F *evals;   // device, 2^v
+    Codeword cw0;
+    std::vector<EF> ood_points, ood_answers;
+};
+
+// evaluate a device F table (2^m) at expand(z, m) with one T-style pass
+EF eval_f_univ(wl_ctx *c, const F *f, int m, const EF &z) {
+    std::vector<EF> pt = wh::expand_univ(z, m);
+    int k = m / 2;
+    std::vector<EF> ph(pt.begin(), pt.begin() + k), pl(pt.begin() + k, pt.end());
+    DBuf<EF> El = eq_dev(c, pl, eone());
+    int nc = wk::whir_T_chunks(m, k);
+    DBuf<EF> parts(c->pool, ((size_t)1 << k) * nc);
+    wk::whir_T_f(f, m, k, El, 1, parts, c->stream);
+    DBuf<EF> T(c->pool, (size_t)1 << k);
+    wk::reduce_parts(parts, 1 << k, nc, T, c->stream);
+    DBuf<EF> Eh = eq_dev(c, ph, eone());
+    int nb = wk::dot_blocks((size_t)1 << k);
+    DBuf<EF> p2(c->pool, nb);
+    wk::seg_dot(T, Eh, (size_t)1 << k, 1, p2, c->stream);
+    return reduce_one(c, p2, nb);
+}
+
+EF eval_ef_univ(wl_ctx *c, const EF *f, int m, const EF &z) {
+    std::vector<EF> pt = wh::expand_univ(z, m);
+    DBuf<EF> E = eq_dev(c, pt, eone());
+    int nb = wk::dot_blocks((size_t)1 << m);
+    DBuf<EF> parts(c->pool, nb);
+    wk::seg_dot(f, E, (size_t)1 << m, 1, parts, c->stream);
+    return reduce_one(c, parts, nb);
+}
+
+// CommitmentWriter::commit: RS-encode the 2^f0 columns of the packed
+// polynomial, Merkle-commit, absorb the root, answer OOD samples.
+void whir_commit(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts, const F *evals,
+                 size_t nonzero_len, WhirWitness &w) {
+    int v = cfg.num_variables, f0 = cfg.f0;
+    int log_m = v - f0;
+    int n_cols = 1 << f0;
+    size_t colen = (size_t)1 << log_m;
+    int n_nonzero = (int)std::min<size_t>(n_cols, (nonzero_len + colen - 1) / colen);
+    w.evals = evals;
+    commit_codeword(c, pdev, w.cw0, evals, n_cols, n_nonzero, log_m, v + cfg.starting_log_inv_rate - f0);
+    w.cw0.base = true;
+    w.cw0.fold = f0;
+    F root[8];
+    c->d2h(root, w.cw0.nodes.p + 8, 8);
+    ts.add_base(root, 8);
+    for (int i = 0; i < cfg.commit_ood_samples; i++) w.ood_points.push_back(ts.sample_ef());
+    for (int i = 0; i < cfg.commit_ood_samples; i++) w.ood_answers.push_back(eval_f_univ(c, evals, v, w.ood_points[i]));
+    ts.add_ef(w.ood_answers.data(), w.ood_answers.size());
+}
+
+// Prover::prove (crates/air/src/prove.rs:216-225)
+void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts, WhirWitness &w,
+                const std::vector<EF> &point, const EF &value) {
+    int v = cfg.num_variables, f0 = cfg.f0;
+    std::vector<Constraint> cs;
+    EF g = ts.sample_ef(), gp = eone(), sum = ezero();
+    for (size_t i = 0; i < w.ood_points.size(); i++) {
+        Constraint k{gp, 1, {}, w.ood_points[i], fzero()};
+        cs.push_back(k);
+        sum = eadd(sum, emul(gp, w.ood_answers[i]));
+        gp = emul(gp, g);
+    }
+    {
+        Constraint k{gp, 0, point, ezero(), fzero()};
+        cs.push_back(k);
+        sum = eadd(sum, emul(gp, value));
+    }
+    // initial f0 rounds: T_j(u) = sum_x f(u, x) eq(pt_j[f0..], x) on the device,
+    // then the f0 round polynomials from the 2^f0-entry tables.
+    int X = v - f0, U = 1 << f0, nC = (int)cs.size();
+    DBuf<EF> E(c->pool, (size_t)nC << X);
+    std::vector<std::vector<EF>> A(nC);
+    std::vector<DBuf<EF>> keep;
+    for (int j = 0; j < nC; j++) {
+        std::vector<EF> pt = cons_point(cs[j], v);
+        std::vector<EF> pre(pt.begin(), pt.begin() + f0), suf(pt.begin() + f0, pt.end());
+        A[j] = wh::eq_host(pre);
+        std::vector<EF> up(suf);
+        up.push_back(eone());
+        keep.push_back(upload_ef(c, up));
+        wk::eq_table(keep.back().p, X, keep.back().p + X, E.p + ((size_t)j << X), c->stream);
+    }
+    int nch = wk::whir_T_chunks(v, f0);
+    DBuf<EF> parts(c->pool, (size_t)nC * U * nch);
+    wk::whir_T_f(w.evals, v, f0, E, nC, parts, c->stream);
+    std::vector<EF> T = reduce_many(c, parts, nC * U, nch);
+    keep.clear();
+    std::vector<std::vector<EF>> Tj(nC);
+    for (int j = 0; j < nC; j++) Tj[j].assign(T.begin() + (size_t)j * U, T.begin() + (size_t)(j + 1) * U);
+    std::vector<EF> rf;
+    for (int t = 0; t < f0; t++) {
+        size_t h = Tj[0].size() / 2;
+        EF h0 = ezero(), h1 = ezero(), h2 = ezero();
+        for (int j = 0; j < nC; j++) {
+            EF a0 = ezero(), a1 = ezero(), a2 = ezero();
+            for (size_t i = 0; i < h; i++) {
+                a0 = eadd(a0, emul(A[j][i], Tj[j][i]));
+                a1 = eadd(a1, emul(A[j][i + h], Tj[j][i + h]));
+                a2 = eadd(a2, emul(esub(edbl(A[j][i + h]), A[j][i]), esub(edbl(Tj[j][i + h]), Tj[j][i])));
+            }
+            h0 = eadd(h0, emul(cs[j].scal, a0));
+            h1 = eadd(h1, emul(cs[j].scal, a1));
+            h2 = eadd(h2, emul(cs[j].scal, a2));
+        }
+        EF hv[3] = {h0, h1, h2};
+        ts.add_ef(hv, 3);
+        grind(c, ts, pdev, cfg.starting_folding_pow_bits);
+        EF r = ts.sample_ef();
+        rf.push_back(r);
+        sum = wh::interp3(hv, r);
+        for (int j = 0; j < nC; j++) {
+            for (size_t i = 0; i < h; i++) {
+                A[j][i] = eadd(A[j][i], emul(r, esub(A[j][i + h], A[j][i])));
+                Tj[j][i] = eadd(Tj[j][i], emul(r, esub(Tj[j][i + h], Tj[j][i])));
+            }
+            A[j].resize(h);
+            Tj[j].resize(h);
+        }
+    }
+    for (auto &k : cs) fold_constraint(k, v, rf);
+    SumcheckState s;
+    s.m = X;
+    s.sum = sum;
+    s.f = DBuf<EF>(c->pool, (size_t)1 << X);
+    {
+        std::vector<EF> eqr = wh::eq_host(rf);
+        DBuf<EF> deq = upload_ef(c, eqr);
+        wk::whir_fold_f(w.evals, v, f0, deq, s.f, c->stream);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    s.W = DBuf<EF>(c->pool, (size_t)1 << X);
+    add_constraints_w(c, s.W, X, cs, true);
+
+    Codeword *prev = &w.cw0;
+    Codeword own[2];
+    int cur = 0;
+    for (int r = 0; r < cfg.n_rounds; r++) {
+        const wh::WhirRound &rp = cfg.rounds[r];
+        int m = s.m;
+        int rs = r == 0 ? cfg.rs_reduction0 : 1;
+        int new_logd = rp.log_domain_size - rs;
+        int kn = cfg.folding_at(r + 1);
+        Codeword &nc = own[cur];
+        {
+            DBuf<F> cols(c->pool, (size_t)8 << m);
+            wk::ef_to_cols(s.f, m, kn, cols, c->stream);
+            commit_codeword(c, pdev, nc, cols, 8 << kn, 8 << kn, m - kn, new_logd - kn);
+        }
+        nc.base = false;
+        nc.fold = kn;
+        F root[8];
+        c->d2h(root, nc.nodes.p + 8, 8);
+        ts.add_base(root, 8);
+        std::vector<EF> oodp, ooda;
+        for (int i = 0; i < rp.ood_samples; i++) oodp.push_back(ts.sample_ef());
+        for (int i = 0; i < rp.ood_samples; i++) ooda.push_back(eval_ef_univ(c, s.f, m, oodp[i]));
+        ts.add_ef(ooda.data(), ooda.size());
+        grind(c, ts, pdev, rp.pow_bits);
+        int log_rows = rp.log_domain_size - prev->fold;
+        std::vector<uint32_t> q = sample_queries(ts, rp.num_queries, log_rows);
+        std::vector<F> rows = open_queries(c, ts, *prev, q);
+        std::vector<EF> eqr = wh::eq_host(rf);
+        EF gg = ts.sample_ef(), gpp = eone();
+        std::vector<Constraint> fresh;
+        for (int i = 0; i < rp.ood_samples; i++) {
+            fresh.push_back(Constraint{gpp, 1, {}, oodp[i], fzero()});
+            s.sum = eadd(s.sum, emul(gpp, ooda[i]));
+            gpp = emul(gpp, gg);
+        }
+        F gen = fpow(fsmall(3), (uint64_t)(P - 1) >> log_rows);
+        for (size_t i = 0; i < q.size(); i++) {
+            EF val = fold_row(&rows[i * prev->width], prev->base, prev->fold, eqr);
+            fresh.push_back(Constraint{gpp, 2, {}, ezero(), fpow(gen, q[i])});
+            s.sum = eadd(s.sum, emul(gpp, val));
+            gpp = emul(gpp, gg);
+        }
+        add_constraints_w(c, s.W, m, fresh, false);
+        for (auto &k : fresh) cs.push_back(k);
+        rf.clear();
+        for (int k = 0; k < kn; k++) rf.push_back(whir_sc_round(c, ts, pdev, s, rp.folding_pow_bits));
+        for (auto &k : cs) fold_constraint(k, m, rf);
+        prev = &nc;
+        cur ^= 1;
+        own[cur] = Codeword();  // release the codeword two rounds back
+    }
+    // final round
+    size_t FN = (size_t)1 << s.m;
+    std::vector<EF> ff(FN), Wf(FN);
+    c->d2h(ff.data(), s.f.p, FN);
+    c->d2h(Wf.data(), s.W.p, FN);
+    ts.add_ef(ff.data(), FN);
+    grind(c, ts, pdev, cfg.final_pow_bits);
+    {
+        int log_rows = cfg.final_log_domain_size - prev->fold;
+        std::vector<uint32_t> q = sample_queries(ts, cfg.final_queries, log_rows);
+        open_queries(c, ts, *prev, q);
+    }
+    for (int k = 0; k < cfg.final_sumcheck_rounds; k++) host_sc_round(ts, c, pdev, ff, Wf, s.sum, cfg.final_folding_pow_bits);
+}
+
+// --------------------------------------------------------------- AIR prove --
+// trace: device, Montgomery, 165 columns x 2^n inside a zero-padded 2^(n+8) buffer
+std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2Dev &ph,
+                                const P2Dev *pdev, const F *packed) {
+    wl_ctx *c = t->ctx;
+    const int n = (int)t->log_length, skips = (int)t->univariate_skips, sec = (int)st->security_bits;
+    const size_t N = (size_t)1 << n;
+    const int lc = (int)wh::log2_up(AIR_NCOLS);
+    const int v = n + lc;
+    WhirCfg cfg = wh::whir_config(v, sec, 16, (int)st->folding_factor_first, (int)st->folding_factor,
+                                  (int)st->log_inv_rate, (int)st->rs_domain_initial_reduction);
+    Transcript ts(&ph.p16);
+    for (uint32_t x : wh::whir_domsep(cfg)) ts.observe(to_mont(x));
+
+    auto tp = std::chrono::steady_clock::now();
+    auto lap = [&](int i) {
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        auto now = std::chrono::steady_clock::now();
+        c->phase_ms[i] += std::chrono::duration<double, std::milli>(now - tp).count();
+        tp = now;
+    };
+
+    WhirWitness w;
+    whir_commit(c, cfg, pdev, ts, packed, (size_t)AIR_NCOLS * N, w);
+    lap(0);
+    grind(c, ts, pdev, pow_bits_air(sec, (int)wh::log2_up(AIR_NCONS)));
+    EF alpha = ts.sample_ef();
+    std::vector<EF> ap(AIR_NCONS);
+    ap[0] = eone();
+    for (int i = 1; i < AIR_NCONS; i++) ap[i] = emul(ap[i - 1], alpha);
+    DBuf<EF> dap = upload_ef(c, ap);
+    grind(c, ts, pdev, pow_bits_air(sec, n));
+    const int nz = n + 1 - skips;
+    std::vector<EF> zc(nz);
+    for (int i = 0; i < nz; i++) zc[i] = ts.sample_ef();
+
+    // ---- zerocheck round 0 (univariate skip) ----
+    const int K = 1 << skips;
+    const size_t M = N >> skips;
+    std::vector<EF> ch;
+    EF sum = ezero(), missing = eone();
+    DBuf<EF> U(c->pool, (size_t)AIR_NCOLS * M);
+    {
+        DBuf<EF> eq = eq_dev(c, std::vector<EF>(zc.begin() + 1, zc.end()), eone());
+        int zmax = 3 * (K - 1);
+        int nzp = zmax - K + 1;  // z = K .. zmax
+        std::vector<F> sel;
+        for (int z = K; z <= zmax; z++) {
+            std::vector<F> s = wh::selectors_f(skips, fsmall(z));
+            sel.insert(sel.end(), s.begin(), s.end());
+        }
+        DBuf<F> dsel(c->pool, sel.size());
+        c->h2d(dsel.p, sel.data(), sel.size());
+        size_t nbx = (M + 255) / 256;
+        DBuf<EF> parts(c->pool, (size_t)nzp * nbx);
+        wk::zc_round0(packed, n, skips, dsel, nzp, t->air_dev, dap, eq, parts, c->stream);
+        std::vector<EF> vals = reduce_many(c, parts, nzp, (int)nbx);
+        std::vector<F> xs;
+        std::vector<EF> ys;
+        for (int i = 0; i < K; i++) { xs.push_back(fsmall(i)); ys.push_back(ezero()); }
+        for (int i = 0; i < nzp; i++) { xs.push_back(fsmall(K + i)); ys.push_back(vals[i]); }
+        std::vector<EF> p = wh::lagrange(xs, ys);
+        std::vector<EF> sr = wh::selectors_ef(skips, zc[0]);
+        std::vector<F> qx;
+        for (int i = 0; i < K; i++) qx.push_back(fsmall(i));
+        p = wh::pmul(p, wh::lagrange(qx, sr));
+        ts.add_ef(p.data(), p.size());
+        EF c0 = ts.sample_ef();
+        ch.push_back(c0);
+        sum = wh::peval(p, c0);
+        grind(c, ts, pdev, pow_bits_sumcheck(sec, (3 + 1) * (K - 1)));
+        std::vector<EF> sc0 = wh::selectors_ef(skips, c0);
+        EF mm = ezero();
+        for (int i = 0; i < K; i++) mm = eadd(mm, emul(sr[i], sc0[i]));
+        missing = mm;
+        DBuf<EF> dS = upload_ef(c, sc0);
+        wk::zc_fold0(packed, n, skips, dS, U, c->stream);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    lap(1);
+    // ---- zerocheck rounds 1 .. n - skips ----
+    int l = n - skips;
+    for (int j = 1; j < nz; j++) {
+        EF r = zc[j];
+        DBuf<EF> eq = eq_dev(c, std::vector<EF>(zc.begin() + 1 + j, zc.end()), eone());
+        size_t nbx = (((size_t)1 << (l - 1)) + 255) / 256;
+        DBuf<EF> parts(c->pool, 3 * nbx);
+        wk::zc_round_strided(U, M, l, t->air_dev, dap, eq, parts, c->stream);
+        std::vector<EF> v3 = reduce_many(c, parts, 3, (int)nbx);
+        EF e0 = emul(v3[0], missing), e2 = emul(v3[1], missing), e3 = emul(v3[2], missing);
+        EF e1 = emul(esub(sum, emul(e0, esub(eone(), r))), wh::einv(r));
+        std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2), fsmall(3)};
+        std::vector<EF> p = wh::lagrange(xs, {e0, e1, e2, e3});
+        p = wh::pmul(p, {esub(eone(), r), esub(edbl(r), eone())});
+        ts.add_ef(p.data(), p.size());
+        EF cj = ts.sample_ef();
+        ch.push_back(cj);
+        sum = wh::peval(p, cj);
+        grind(c, ts, pdev, pow_bits_sumcheck(sec, 3 + 1));
+        missing = emul(eadd(emul(esub(eone(), r), esub(eone(), cj)), emul(r, cj)), missing);
+        DBuf<EF> dc = upload_ef(c, std::vector<EF>(1, cj));
+        wk::fold_cols_strided(U, AIR_NCOLS, M, l, dc, c->stream);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        l--;
+    }
+    lap(2);
+    // ---- inner sums ----
+    std::vector<EF> up(AIR_NCOLS), dn(AIR_NCOLS);
+    {
+        std::vector<EF> tmp((size_t)AIR_NCOLS);
+        uint32_t *st2 = c->stage((size_t)AIR_NCOLS * 8);
+        HIP_CHECK(hipMemcpy2DAsync(st2, sizeof(EF), U.p, M * sizeof(EF), sizeof(EF), AIR_NCOLS,
+                                   hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        memcpy(up.data(), st2, AIR_NCOLS * sizeof(EF));
+    }
+    U.reset();
+    std::vector<EF> chr(ch.begin() + 1, ch.end());
+    DBuf<EF> eqr = eq_dev(c, chr, eone());
+    {
+        DBuf<EF> parts(c->pool, (size_t)AIR_NCOLS * K);
+        wk::zc_down_parts(packed, n, skips, eqr, parts, c->stream);
+        DBuf<EF> dS = upload_ef(c, wh::selectors_ef(skips, ch[0]));
+        DBuf<EF> out(c->pool, AIR_NCOLS);
+        wk::zc_down_combine(parts, skips, dS, out, c->stream);
+        c->d2h(dn.data(), out.p, AIR_NCOLS);
+    }
+    ts.add_ef(up.data(), up.size());
+    ts.add_ef(dn.data(), dn.size());
+    grind(c, ts, pdev, pow_bits_air(sec, lc));
+    std::vector<EF> cbs(lc);
+    for (int i = 0; i < lc; i++) cbs[i] = ts.sample_ef();
+    DBuf<EF> eqc = eq_dev(c, cbs, eone());
+    DBuf<EF> B(c->pool, N);
+    wk::lincomb_cols(packed, n, AIR_NCOLS, eqc, B, c->stream);
+    EF alpha2 = ts.sample_ef();
+    DBuf<EF> da2 = upload_ef(c, std::vector<EF>(1, alpha2));
+    DBuf<EF> mixed(c->pool, N);
+    wk::mix_updown(B, n, da2, mixed, c->stream);
+    std::vector<EF> sub;
+    {
+        int nb = wk::dot_blocks(M);
+        DBuf<EF> parts(c->pool, (size_t)K * nb);
+        wk::seg_dot(mixed, eqr, M, K, parts, c->stream);
+        sub = reduce_many(c, parts, K, nb);
+    }
+    ts.add_ef(sub.data(), sub.size());
+    std::vector<EF> pt;
+    for (int i = 0; i < skips; i++) pt.push_back(ts.sample_ef());
+    for (size_t i = 1; i < ch.size(); i++) pt.push_back(ch[i]);
+    DBuf<EF> E = eq_dev(c, pt, eone());
+    DBuf<EF> Mt(c->pool, N);
+    wk::build_matrix(E, n, da2, Mt, c->stream);
+    EF inner_sum;
+    {
+        int nb = wk::dot_blocks(N);
+        DBuf<EF> parts(c->pool, nb);
+        wk::seg_dot(mixed, E, N, 1, parts, c->stream);
+        inner_sum = reduce_one(c, parts, nb);
+    }
+    mixed.reset();
+    E.reset();
+    lap(3);
+    // ---- inner sumcheck (degree 2, coefficients sent) ----
+    std::vector<EF> ich;
+    EF isum = inner_sum;
+    for (int j = 0; j < n; j++) {
+        int ll = n - j;
+        int nb = wk::dot_blocks((size_t)1 << (ll - 1));
+        DBuf<EF> parts(c->pool, 2 * (size_t)nb);
+        wk::prod_round(Mt, B, ll, parts, c->stream);
+        std::vector<EF> h02 = reduce_many(c, parts, 2, nb);
+        std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2)};
+        std::vector<EF> p = wh::lagrange(xs, {h02[0], esub(isum, h02[0]), h02[1]});
+        ts.add_ef(p.data(), p.size());
+        EF cj = ts.sample_ef();
+        ich.push_back(cj);
+        isum = wh::peval(p, cj);
+        grind(c, ts, pdev, pow_bits_sumcheck(sec, 2));
+        DBuf<EF> dc = upload_ef(c, std::vector<EF>(1, cj));
+        wk::fold_ef(Mt, ll, dc, c->stream);
+        wk::fold_ef(B, ll, dc, c->stream);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    EF packed_value;
+    c->d2h(&packed_value, B.p, 1);
+    Mt.reset();
+    B.reset();
+    lap(4);
+    std::vector<EF> fp(cbs);
+    fp.insert(fp.end(), ich.begin(), ich.end());
+    whir_prove(c, cfg, pdev, ts, w, fp, packed_value);
+    lap(5);
+    return ts.data;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI ==
+extern "C" {
+
+static thread_local std::string g_err;
+
+const char *wl_last_error(void) { return g_err.c_str(); }
+
+wl_ctx *wl_create(int device) {
+    try {
+        HIP_CHECK(hipSetDevice(device));
+        wl_ctx *c = new wl_ctx();
+        c->device = device;
+        HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreate(&c->ev0));
+        HIP_CHECK(hipEventCreate(&c->ev1));
+        return c;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void wl_destroy(wl_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->tw) (void)hipFree(kv.second);
+    c->pool.release();
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int wl_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+wl_table *wl_air_table_new(wl_ctx *c, const uint32_t *rc, uint32_t log_length, uint32_t univariate_skips) {
+    try {
+        if (!c || !rc) throw std::runtime_error("null argument");
+        if (univariate_skips >= log_length) throw std::runtime_error("univariate_skips must be < log_length");
+        wl_table *t = new wl_table();
+        t->ctx = c;
+        t->log_length = log_length;
+        t->univariate_skips = univariate_skips;
+        for (int i = 0; i < wk::AIR_NRC; i++) t->air_host.rc[i] = to_mont(rc[i] % P);
+        HIP_CHECK(hipMalloc(&t->air_dev, sizeof(wk::AirConsts)));
+        HIP_CHECK(hipMemcpy(t->air_dev, &t->air_host, sizeof(wk::AirConsts), hipMemcpyHostToDevice));
+        return t;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void wl_air_table_free(wl_table *t) {
+    if (!t) return;
+    (void)hipFree(t->air_dev);
+    delete t;
+}
+
+static int prove_common(wl_table *t, const wl_air_settings *st, const uint32_t *p16c,
+                        const uint32_t *p24c, F *packed, uint32_t *proof_out, size_t proof_cap,
+                        size_t *proof_len) {
+    wl_ctx *c = t->ctx;
+    P2Dev ph;
+    load_p2(p16c, p24c, ph);
+    DBuf<P2Dev> pd = upload_p2(c, ph);
+    std::vector<uint32_t> proof = air_prove(t, st, ph, pd.p, packed);
+    if (proof_len) *proof_len = proof.size();
+    if (proof_out) {
+        if (proof_cap < proof.size()) { g_err = "proof buffer too small"; return WL_ERR_BUFFER; }
+        memcpy(proof_out, proof.data(), proof.size() * 4);
+    }
+    return WL_OK;
+}
+
+static int check_settings(const wl_table *t, const wl_air_settings *st) {
+    if (!t || !st) { g_err = "null argument"; return WL_ERR_ARG; }
+    if (st->folding_factor_first < 1 || st->folding_factor < 1 ||
+        st->rs_domain_initial_reduction > st->folding_factor_first ||
+        t->log_length + 8 < st->folding_factor_first || t->log_length < 2 ||
+        t->univariate_skips < 1 || t->univariate_skips >= t->log_length || st->log_inv_rate < 1) {
+        g_err = "unsupported settings";
+        return WL_ERR_ARG;
+    }
+    return WL_OK;
+}
+
+int wl_air_prove(wl_table *t, const wl_air_settings *st, const uint32_t *p16c, const uint32_t *p24c,
+                 const uint32_t *witness, uint32_t *proof_out, size_t proof_cap, size_t *proof_len) {
+    int rc = check_settings(t, st);
+    if (rc) return rc;
+    try {
+        wl_ctx *c = t->ctx;
+        HIP_CHECK(hipSetDevice(c->device));
+        size_t N = (size_t)1 << t->log_length;
+        size_t v = t->log_length + 8;
+        DBuf<F> packed(c->pool, (size_t)1 << v);
+        wk::fill_zero(packed.p + AIR_NCOLS * N, (((size_t)1 << v) - AIR_NCOLS * N) * 4, c->stream);
+        HIP_CHECK(hipMemcpyAsync(packed.p, witness, AIR_NCOLS * N * 4, hipMemcpyHostToDevice, c->stream));
+        wk::convert_to_mont((uint32_t *)packed.p, AIR_NCOLS * N, c->stream);
+        return prove_common(t, st, p16c, p24c, packed, proof_out, proof_cap, proof_len);
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return WL_ERR_DEVICE;
+    }
+}
+
+
+wl_witness *wl_witness_upload(wl_ctx *c, const uint32_t *witness, uint32_t log_length) {
+    try {
+        HIP_CHECK(hipSetDevice(c->device));
+        wl_witness *w = new wl_witness();
+        w->ctx = c;
+        w->log_length = log_length;
+        size_t N = (size_t)1 << log_length, V = N << 8;
+        HIP_CHECK(hipMalloc(&w->packed, V * 4));
+        wk::fill_zero(w->packed + AIR_NCOLS * N, (V - AIR_NCOLS * N) * 4, c->stream);
+        HIP_CHECK(hipMemcpyAsync(w->packed, witness, AIR_NCOLS * N * 4, hipMemcpyHostToDevice, c->stream));
+        wk::convert_to_mont((uint32_t *)w->packed, AIR_NCOLS * N, c->stream);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        return w;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void wl_witness_free(wl_witness *w) {
+    if (!w) return;
+    (void)hipFree(w->packed);
+    delete w;
+}
+
+int wl_air_prove_resident(wl_table *t, const wl_air_settings *st, const uint32_t *p16c,
+                          const uint32_t *p24c, const wl_witness *w, uint32_t *proof_out,
+                          size_t proof_cap, size_t *proof_len) {
+    int rc = check_settings(t, st);
+    if (rc) return rc;
+    if (!w || w->log_length != t->log_length) { g_err = "witness size mismatch"; return WL_ERR_ARG; }
+    try {
+        HIP_CHECK(hipSetDevice(t->ctx->device));
+        return prove_common(t, st, p16c, p24c, w->packed, proof_out, proof_cap, proof_len);
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return WL_ERR_DEVICE;
+    }
+}
+
+void wl_phase_times(wl_ctx *c, double *out6, int reset) {
+    for (int i = 0; i < 6; i++) out6[i] = c->phase_ms[i];
+    if (reset) for (int i = 0; i < 8; i++) c->phase_ms[i] = 0;
+}
+
+}  // extern "C"
+
+void wl_set_error(const char *msg) { g_err = msg; }
