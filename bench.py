@@ -1,0 +1,143 @@
+#!/usr/bin/env python3
+"""Benchmark: Poseidon2 permutations proven per second on one MI355X.
+
+Workload: the Poseidon2-16 KoalaBear AIR with 2^17 permutations (one trace row
+each), inputs from StdRng::seed_from_u64(0) as in src/examples/poseidon2.rs,
+default AirSettings of src/main.rs (128-bit, CapacityBound, folding (7, 4),
+rate 1/2, 4 univariate skips, RS reduction 5).  One step = one full proof
+(AirTable::prove, crates/air/src/prove.rs:49-226) with the witness already
+resident in HBM; proof bytes land on the host at the end of every step.
+
+Prints one JSON line.  The CPU baseline is the oracle (a C restatement of the
+same prover, OpenMP) timed on this host on one proof of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Per-kernel-class roofline models: (name, bound, algorithmic work per unit, unit name).
+# VALU classes count field multiplications (F-mul) as in the per-unit figures of the
+# work model: Poseidon2-24 982, Poseidon2-16 616 F-mul per permutation; one EF
+# AIR evaluation point 31168 F-mul; one skip-round point 5700 F-mul.
+VALU_PEAK_TFMUL = 256 * 64 * 2.4e9 / 4 / 3 / 1e12   # quarter-rate 32-bit multiplies, 3 per F-mul
+HBM_PEAK_GBS = 8000.0
+MODELS = {
+    1: ("merkle_leaves (Poseidon2-24 sponge)", "valu", 982.0),
+    2: ("zerocheck_rounds (EF AIR eval, rounds>=1)", "valu", 31168.0),
+    3: ("rs_encode (Mobius + NTT)", "hbm", 1.0),          # work already in bytes
+    4: ("zerocheck_round0 (skip LDE + F AIR eval)", "valu", 5700.0),
+    5: ("grinding (Poseidon2-16)", "valu", 616.0),
+    6: ("merkle_nodes (Poseidon2-16 compress)", "valu", 616.0),
+}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log-n", type=int, default=17)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-log-n", type=int, default=17)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (multi-GPU sharding is a design "
+              "described in DESIGN.md, not a measured path); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import whirlaway_amd as wa
+    from whirlaway_amd.air import bench_inputs, generate_witness
+
+    n = args.log_n
+    ctx = wa.Context(0)
+    rc, inputs, p16, p24 = bench_inputs(args.seed, n)
+    table = wa.AirTable(ctx, rc, n)
+    wit = generate_witness(table, inputs)
+    settings = wa.AirSettings()
+    proof = None
+    for _ in range(args.warmup):
+        proof = table.prove(settings, p16, p24, wit)
+
+    ctx.kernel_timing(list(MODELS))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        proof = table.prove(settings, p16, p24, wit)
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = (1 << n) / (ms / 1e3)
+
+    per_class = {}
+    for k, (name, bound, per_unit) in MODELS.items():
+        launches, kms, work = ctx.kernel_timing_read(k)
+        per_class[k] = (launches, kms, work)
+    dom = max(per_class, key=lambda k: per_class[k][1])
+    name, bound, per_unit = MODELS[dom]
+    launches, kms, work = per_class[dom]
+    avg_ms = kms / max(launches, 1)
+    per_launch = work * per_unit / max(launches, 1)
+    if bound == "hbm":
+        achieved = per_launch / (avg_ms / 1e3) / 1e9
+        peak, unit = HBM_PEAK_GBS, "GB/s"
+    else:
+        achieved = per_launch / (avg_ms / 1e3) / 1e12
+        peak, unit = VALU_PEAK_TFMUL, "TFmul/s"
+    roofline = {
+        "bound": bound, "achieved": round(achieved, 4), "peak": round(peak, 4), "unit": unit,
+        "frac": round(achieved / peak, 4), "traffic": None, "kernel": name,
+        "avg_launch_ms": round(avg_ms, 5), "launches_per_step": launches / args.steps,
+        "kernel_ms_per_step": {MODELS[k][0].split(" ")[0]: round(v[1] / args.steps, 4)
+                               for k, v in per_class.items()},
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py  # the checker, timed here only as the reported CPU baseline
+        b = oracle_py.Bench(args.cpu_log_n, args.seed)
+        tc = time.perf_counter()
+        cpu_proof = b.prove()
+        tcpu = time.perf_counter() - tc
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": round((1 << args.cpu_log_n) / tcpu, 1), "unit": "perms/s", "cores": cores,
+               "kind": "port",
+               "sample": f"one oracle proof of 2^{args.cpu_log_n} permutations (same seed and "
+                         f"settings), {tcpu:.2f} s wall, OpenMP"}
+        if args.cpu_log_n == n and not (cpu_proof.size == proof.size and (cpu_proof == proof).all()):
+            cpu["note"] = "CPU and GPU proofs differ"
+        b.close()
+
+    phases = ctx.phase_times(reset=True)
+    line = {
+        "metric": "Poseidon2 perms proven/s (2^17 perms, 1 GPU)",
+        "value": round(value, 1),
+        "unit": "perms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (KoalaBear F_p, Montgomery) / EF8",
+        "data": "synthetic (StdRng seed 0 inputs, GPU-generated Poseidon2 trace)",
+        "config": {"workload": f"Poseidon2-16 KoalaBear AIR, 2^{n} permutations, one proof per step",
+                   "log_n_rows": n, "security_bits": 128, "folding": [7, 4], "log_inv_rate": 1,
+                   "univariate_skips": 4, "rs_domain_initial_reduction": 5,
+                   "proof_words": int(proof.size)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "phase_ms_per_step": [round(x / (args.steps + args.warmup), 4) for x in phases],
+    }
+    table.close()
+    ctx.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -100,6 +100,18 @@ int wl_witness_download(const wl_witness *w, uint32_t *out);
  * sumcheck, WHIR opening]. */
 void wl_phase_times(wl_ctx *ctx, double *out6, int reset);
 
+/* Live timing of kernel classes with HIP events on the context's stream (no
+ * counterpart in the reference; used by the benchmark's roofline line).
+ * Classes (bit k of class_mask): 1 Merkle leaf hashing (work = Poseidon2-24
+ * permutations), 2 zerocheck rounds >= 1 (work = evaluation points), 3 RS
+ * encoding (work = codeword elements), 4 zerocheck round 0 (points),
+ * 5 grinding (candidates), 6 Merkle compression levels (Poseidon2-16
+ * permutations).  wl_kernel_timing resets the counters; _read synchronizes and
+ * sums one class. */
+int wl_kernel_timing(wl_ctx *ctx, uint32_t class_mask);
+int wl_kernel_timing_read(wl_ctx *ctx, int kernel_class, long *launches, double *total_ms,
+                          double *work);
+
 /* ---------------------------------------------------------------- kernels --
  * crates/utils/src/multilinear.rs:90-123 fold_multilinear_in_large_field:
  * out[i] = sum_j scalars[j] * in[i + j * 2^(log_n - log_k)], in F (base) or EF,

@@ -60,6 +60,8 @@ def lib():
         L.wl_air_generate_witness.restype = vp
         L.wl_air_generate_witness.argtypes = [vp, u32p]
         L.wl_witness_download.argtypes = [vp, u32p]
+        L.wl_kernel_timing.argtypes = [vp, ctypes.c_uint32]
+        L.wl_kernel_timing_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), dp, dp]
         _lib = L
     return _lib
 

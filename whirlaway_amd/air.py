@@ -50,6 +50,24 @@ class Context:
         lib().wl_phase_times(self.ptr, out, int(reset))
         return list(out)
 
+    KERNEL_CLASSES = {1: "merkle_leaves", 2: "zerocheck_rounds", 3: "rs_encode_ntt",
+                      4: "zerocheck_round0", 5: "grinding", 6: "merkle_nodes"}
+
+    def kernel_timing(self, classes):
+        """Start live HIP-event timing of the given kernel classes (wl_kernel_timing)."""
+        mask = 0
+        for k in classes:
+            mask |= 1 << k
+        check(lib().wl_kernel_timing(self.ptr, mask))
+
+    def kernel_timing_read(self, kernel_class):
+        n = ctypes.c_long(0)
+        ms = ctypes.c_double(0)
+        work = ctypes.c_double(0)
+        check(lib().wl_kernel_timing_read(self.ptr, kernel_class, ctypes.byref(n), ctypes.byref(ms),
+                                          ctypes.byref(work)))
+        return n.value, ms.value, work.value
+
     def __enter__(self):
         return self
 

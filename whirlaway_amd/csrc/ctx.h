@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 #include <algorithm>
+#include <chrono>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -88,6 +89,11 @@ struct wl_ctx {
     std::map<int, kb::F *> tw;    // twiddle tables per log size
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double phase_ms[8] = {0};
+    // live timing of kernel classes (HIP events on this context's stream)
+    uint32_t kt_mask = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> kt_pairs[8], kt_free;
+    double kt_work[8] = {0};
+    long kt_launches[8] = {0};
 
     kb::F *twiddle(int L) {
         auto it = tw.find(L);
@@ -107,14 +113,10 @@ struct wl_ctx {
         }
         return pinned;
     }
-    // device -> host, blocking on the stream
+    // device -> host (waits for the data; see fetch)
     template <class T>
     void d2h(T *dst, const T *src, size_t count) {
-        if (!count) return;
-        uint32_t *st = stage((count * sizeof(T) + 3) / 4);
-        HIP_CHECK(hipMemcpyAsync(st, src, count * sizeof(T), hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
-        memcpy((void *)dst, st, count * sizeof(T));
+        fetch((void *)dst, (const void *)src, count * sizeof(T));
     }
     // host -> device through the pinned buffer (returns after the copy completed)
     template <class T>
@@ -125,6 +127,78 @@ struct wl_ctx {
         HIP_CHECK(hipMemcpyAsync(dst, st, count * sizeof(T), hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
     }
+    // Device -> host results of each protocol round: a kernel writes them into
+    // host-mapped pinned memory and then stores a sequence number; the host
+    // waits on that flag (far cheaper than a copy plus a stream synchronize).
+    uint32_t *mapped = nullptr;      // host view
+    uint32_t *mapped_dev = nullptr;  // device view
+    size_t mapped_words = 0;
+    uint32_t seq = 0;
+    void ensure_mapped() {
+        if (mapped) return;
+        mapped_words = (size_t)1 << 20;
+        HIP_CHECK(hipHostMalloc((void **)&mapped, (mapped_words + 64) * 4,
+                                hipHostMallocMapped | hipHostMallocCoherent));
+        HIP_CHECK(hipHostGetDevicePointer((void **)&mapped_dev, mapped, 0));
+        memset(mapped, 0, (mapped_words + 64) * 4);
+    }
+    volatile uint32_t *flag_dev() { return (volatile uint32_t *)(mapped_dev + mapped_words); }
+    void wait_flag(uint32_t s) {
+        volatile uint32_t *f = (volatile uint32_t *)(mapped + mapped_words);
+        long spins = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        while (*f != s) {
+            if (++spins % 4096 == 0) {
+                double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (el > 20.0) {
+                    HIP_CHECK(hipStreamSynchronize(stream));
+                    if (*f != s) throw std::runtime_error("device result was not published");
+                }
+            }
+        }
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    }
+    // copy `bytes` of device memory to dst through the mapped buffer (waits)
+    void fetch(void *dst, const void *src, size_t bytes) {
+        if (!bytes) return;
+        ensure_mapped();
+        size_t words = (bytes + 3) / 4;
+        if (words > mapped_words) {
+            uint32_t *st = stage(words);
+            HIP_CHECK(hipMemcpyAsync(st, src, bytes, hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            memcpy(dst, st, bytes);
+            return;
+        }
+        uint32_t s = ++seq;
+        wk::publish((const uint32_t *)src, words, mapped_dev, flag_dev(), s, stream);
+        wait_flag(s);
+        memcpy(dst, mapped, bytes);
+    }
+    // Asynchronous host -> device: the source is copied into a pinned ring that
+    // is never rewritten before the stream has drained (ring_reset), so the
+    // copy needs no synchronization.
+    uint32_t *ring = nullptr;
+    size_t ring_words = 0, ring_off = 0;
+    void ring_reset() {
+        HIP_CHECK(hipStreamSynchronize(stream));
+        ring_off = 0;
+    }
+    template <class T>
+    void h2d_async(T *dst, const T *src, size_t count) {
+        if (!count) return;
+        size_t words = (count * sizeof(T) + 255) / 256 * 64;
+        if (!ring) {
+            ring_words = (size_t)1 << 22;  // 16 MiB
+            HIP_CHECK(hipHostMalloc((void **)&ring, ring_words * 4, hipHostMallocDefault));
+        }
+        if (words > ring_words) { h2d(dst, src, count); return; }
+        if (ring_off + words > ring_words) ring_reset();
+        uint32_t *st = ring + ring_off;
+        ring_off += words;
+        memcpy(st, (const void *)src, count * sizeof(T));
+        HIP_CHECK(hipMemcpyAsync(dst, st, count * sizeof(T), hipMemcpyHostToDevice, stream));
+    }
 };
 
 // AirTable for the Poseidon2 AIR: trace length, skips and round constants.
@@ -134,6 +208,7 @@ struct wl_table {
     uint32_t univariate_skips;
     wk::AirConsts air_host;
     wk::AirConsts *air_dev = nullptr;
+    wk::ChainLin *chain_dev = nullptr;
 };
 
 // Trace resident in HBM as the commitment's packed polynomial (Montgomery form,
@@ -146,35 +221,79 @@ struct wl_witness {
 
 namespace wl {
 
-inline EF reduce_one(wl_ctx *c, const EF *parts, int n_parts) {
-    DBuf<EF> out(c->pool, 1);
-    wk::reduce_parts(parts, 1, n_parts, out, c->stream);
-    EF r;
-    c->d2h(&r, out.p, 1);
-    return r;
-}
+// Kernel classes that can be timed live (wl_kernel_timing).
+enum KernelClass { KT_NONE = 0, KT_MERKLE_LEAVES = 1, KT_ZC_ROUND = 2, KT_NTT = 3, KT_ZC_ROUND0 = 4,
+                   KT_GRIND = 5, KT_MERKLE_NODES = 6 };
+
+// Records a HIP event pair around the launches in its scope when class `id` is
+// being timed; `work` is the algorithmic work of those launches.
+struct KTimer {
+    wl_ctx *c;
+    bool on;
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    int id;
+    KTimer(wl_ctx *ctx, int cls, double work) : c(ctx), on((ctx->kt_mask >> cls) & 1), id(cls) {
+        if (!on) return;
+        if (!c->kt_free.empty()) {
+            ev = c->kt_free.back();
+            c->kt_free.pop_back();
+        } else {
+            HIP_CHECK(hipEventCreate(&ev.first));
+            HIP_CHECK(hipEventCreate(&ev.second));
+        }
+        HIP_CHECK(hipEventRecord(ev.first, c->stream));
+        c->kt_work[id] += work;
+        c->kt_launches[id]++;
+    }
+    ~KTimer() {
+        if (!on) return;
+        (void)hipEventRecord(ev.second, c->stream);
+        c->kt_pairs[id].push_back(ev);
+    }
+};
+
+// Sum block partials [item][part] and bring the item sums to the host.
 inline std::vector<EF> reduce_many(wl_ctx *c, const EF *parts, int n_items, int n_parts) {
+    std::vector<EF> r(n_items);
+    c->ensure_mapped();
+    if (n_items <= 4) {
+        uint32_t s = ++c->seq;
+        wk::reduce_publish(parts, n_items, n_parts, (EF *)c->mapped_dev, c->flag_dev(), s, c->stream);
+        c->wait_flag(s);
+        memcpy(r.data(), c->mapped, (size_t)n_items * sizeof(EF));
+        return r;
+    }
     DBuf<EF> out(c->pool, n_items);
     wk::reduce_parts(parts, n_items, n_parts, out, c->stream);
-    std::vector<EF> r(n_items);
-    c->d2h(r.data(), out.p, n_items);
+    c->fetch(r.data(), out.p, (size_t)n_items * sizeof(EF));
     return r;
 }
+inline EF reduce_one(wl_ctx *c, const EF *parts, int n_parts) { return reduce_many(c, parts, 1, n_parts)[0]; }
+// Stream-ordered upload (no synchronization; device buffers are recycled in
+// stream order, so reuse by later work is safe).
 inline DBuf<EF> upload_ef(wl_ctx *c, const std::vector<EF> &v) {
     DBuf<EF> d(c->pool, v.size() ? v.size() : 1);
-    if (!v.empty()) c->h2d(d.p, v.data(), v.size());
+    if (!v.empty()) c->h2d_async(d.p, v.data(), v.size());
     return d;
 }
-// eq(point, .) * scale on the device (2^n EF)
+inline wk::EqPoint eq_point(const std::vector<EF> &pt, const EF &scale) {
+    if (pt.size() > 32) throw std::runtime_error("eq point longer than 32 variables");
+    wk::EqPoint e;
+    memset(&e, 0, sizeof e);
+    for (size_t i = 0; i < pt.size(); i++) e.p[i] = pt[i];
+    e.scale = scale;
+    e.n = (int)pt.size();
+    return e;
+}
+// eq(point, .) * scale on the device (2^n EF), point passed by value
 inline DBuf<EF> eq_dev(wl_ctx *c, const std::vector<EF> &pt, const EF &scale) {
     size_t N = (size_t)1 << pt.size();
     DBuf<EF> out(c->pool, N);
-    std::vector<EF> tmp(pt);
-    tmp.push_back(scale);
-    DBuf<EF> d = upload_ef(c, tmp);
-    wk::eq_table(d.p, (int)pt.size(), d.p + pt.size(), out, c->stream);
-    HIP_CHECK(hipStreamSynchronize(c->stream));  // `d` is returned to the pool below
+    wk::eq_table_v(eq_point(pt, scale), out, c->stream);
     return out;
+}
+inline void eq_into(wl_ctx *c, const std::vector<EF> &pt, const EF &scale, EF *out) {
+    wk::eq_table_v(eq_point(pt, scale), out, c->stream);
 }
 
 inline void load_p2(const uint32_t *p16c, const uint32_t *p24c, P2Dev &host) {
@@ -208,21 +327,23 @@ inline int pow_bits_air(int sec, int x) { return std::max(0, sec - std::max(0, 2
 // after observe(w) (p3-challenger grind; the minimal witness is taken).
 inline void grind(wl_ctx *c, wh::Transcript &ts, const P2Dev *pdev, int bits) {
     if (bits <= 0) return;
-    F st[16];
-    for (int i = 0; i < 16; i++) st[i] = ts.state[i];
-    for (int i = 0; i < ts.n_in; i++) st[i] = ts.in_buf[i];
+    wk::State16 st;
+    for (int i = 0; i < 16; i++) st.s[i] = ts.state[i];
+    for (int i = 0; i < ts.n_in; i++) st.s[i] = ts.in_buf[i];
     int pos = ts.n_in;  // observe(w) places w here; the sample then duplexes
-    DBuf<F> dst(c->pool, 16);
     DBuf<uint32_t> found(c->pool, 1);
-    c->h2d(dst.p, st, 16);
     uint32_t mask = (uint32_t)((1ull << bits) - 1);
     uint32_t w = 0xffffffffu;
-    const uint32_t chunk = 1u << 22;
+    // 2^(bits+2) candidates per launch: the chance that a launch finds no witness is e^-4
+    const uint32_t chunk = 1u << std::min(22, std::max(12, bits + 2));
     for (uint64_t base = 0; base < P && w == 0xffffffffu; base += chunk) {
         uint32_t init = 0xffffffffu;
-        c->h2d(found.p, &init, 1);
+        c->h2d_async(found.p, &init, 1);
         uint32_t cnt = (uint32_t)std::min<uint64_t>(chunk, P - base);
-        wk::grind(pdev, dst, pos, mask, (uint32_t)base, cnt, found, c->stream);
+        {
+            KTimer kt(c, KT_GRIND, (double)cnt);
+            wk::grind_v(pdev, st, pos, mask, (uint32_t)base, cnt, found, c->stream);
+        }
         c->d2h(&w, found.p, 1);
     }
     if (w == 0xffffffffu) throw std::runtime_error("grinding found no witness");

@@ -17,29 +17,6 @@
 namespace wk {
 using namespace kb;
 
-// Lazy accumulator of EF x F products: 64-bit per coefficient, reduced every 4 terms.
-struct AccF {
-    uint64_t s[8];
-    EF r;
-    int n;
-    __device__ __forceinline__ AccF() : n(0) {
-#pragma unroll
-        for (int i = 0; i < 8; i++) s[i] = 0;
-        r = ezero();
-    }
-    __device__ __forceinline__ void flush() {
-#pragma unroll
-        for (int i = 0; i < 8; i++) { r.c[i] = fadd(r.c[i], F{redc64(s[i])}); s[i] = 0; }
-        n = 0;
-    }
-    __device__ __forceinline__ void add(const EF &a, F b) {
-#pragma unroll
-        for (int i = 0; i < 8; i++) s[i] += (uint64_t)a.c[i].v * b.v;
-        if (++n == 4) flush();
-    }
-    __device__ __forceinline__ EF get() { flush(); return r; }
-};
-
 // ----------------------------------------------------------------- basic --
 __global__ void k_to_mont(uint32_t *d, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,6 +60,43 @@ void reduce_parts(const EF *in, int n_items, int n_parts, EF *out, hipStream_t s
     k_reduce_parts<<<n_items, 256, 0, s>>>(in, n_parts, out);
 }
 
+__global__ void k_publish(const uint32_t *src, size_t n, uint32_t *dst, volatile uint32_t *flag, uint32_t seq) {
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        *flag = seq;
+    }
+}
+void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
+             hipStream_t s) {
+    k_publish<<<1, 1024, 0, s>>>(src, n, host_dst, flag, seq);
+}
+
+// one block: item i reduced by the threads of wave(s) in turn, written to host memory
+__global__ void k_reduce_publish(const EF *in, int n_items, int n_parts, EF *dst, volatile uint32_t *flag,
+                                 uint32_t seq) {
+    __shared__ EF lds[16];
+    for (int it = 0; it < n_items; it++) {
+        const EF *p = in + (size_t)it * n_parts;
+        EF acc = ezero();
+        for (int i = threadIdx.x; i < n_parts; i += blockDim.x) acc = eadd(acc, p[i]);
+        EF t = block_sum(acc, lds);
+        if (threadIdx.x == 0) dst[it] = t;
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        *flag = seq;
+    }
+}
+void reduce_publish(const EF *in, int n_items, int n_parts, EF *host_dst, volatile uint32_t *flag,
+                    uint32_t seq, hipStream_t s) {
+    k_reduce_publish<<<1, 256, 0, s>>>(in, n_items, n_parts, host_dst, flag, seq);
+}
+
 // --------------------------------------------------------------- eq table --
 // Builds T_k over (p_k..p_{n-1}) from T_{k+1}: out[S + j] = p_k T[j], out[j] = T[j] - out[S + j].
 __global__ void k_eq_small(const EF *pt, int n, const EF *scale, EF *out, int levels) {
@@ -119,6 +133,49 @@ void eq_table(const EF *pt, int n, const EF *scale, EF *out, hipStream_t s) {
         k_eq_level<<<nblk(S), 256, 0, s>>>(pt, n - 1 - t, S, out);
         S <<= 1;
     }
+}
+
+// Entry x of eq(p, .) directly: scale * prod_k (bit ? p_k : 1 - p_k); used for
+// the first levels (up to 2^10 entries, one thread each), then doublings.
+__global__ void k_eq_direct(EqPoint pt, int levels, EF *out) {
+    size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ((size_t)1 << levels)) return;
+    EF acc = pt.scale;
+    // the first `levels` doublings cover variables n-levels .. n-1 (big-endian: the
+    // last variable is the least significant bit)
+    for (int t = 0; t < levels; t++) {
+        const EF &p = pt.p[pt.n - levels + t];
+        bool bit = (x >> (levels - 1 - t)) & 1;
+        acc = emul(acc, bit ? p : esub(eone(), p));
+    }
+    out[x] = acc;
+}
+__global__ void k_eq_level_v(EF p, size_t S, EF *out) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= S) return;
+    EF v = out[j];
+    EF hi = emul(v, p);
+    out[S + j] = hi;
+    out[j] = esub(v, hi);
+}
+void eq_table_v(const EqPoint &pt, EF *out, hipStream_t s) {
+    int n = pt.n;
+    int first = n < 10 ? n : 10;
+    k_eq_direct<<<nblk((size_t)1 << first), 256, 0, s>>>(pt, first, out);
+    size_t S = (size_t)1 << first;
+    for (int t = first; t < n; t++) {
+        k_eq_level_v<<<nblk(S), 256, 0, s>>>(pt.p[n - 1 - t], S, out);
+        S <<= 1;
+    }
+}
+
+__global__ void k_eq_halve(EF *E, size_t h) {
+    size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < h) E[x] = eadd(E[x], E[x + h]);
+}
+void eq_halve(EF *E, int l, hipStream_t s) {
+    size_t h = (size_t)1 << (l - 1);
+    k_eq_halve<<<nblk(h), 256, 0, s>>>(E, h);
 }
 
 // --------------------------------------------------------- AIR evaluation --
@@ -160,7 +217,7 @@ __device__ __forceinline__ EF air_eval_f(Get get, const AirConsts *air, const EF
         k++;
         col++;
         s[0] = post;
-        internal_layer<16>(s, DDIAG.d16);
+        internal_layer_fast<16>(s);
     }
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
@@ -177,101 +234,6 @@ __device__ __forceinline__ EF air_eval_f(Get get, const AirConsts *air, const EF
         col += 16;
     }
     return acc.get();
-}
-
-__device__ __forceinline__ void mat4_ef(EF *x) {
-    EF t01 = eadd(x[0], x[1]), t23 = eadd(x[2], x[3]);
-    EF t0123 = eadd(t01, t23);
-    EF t01123 = eadd(t0123, x[1]);
-    EF t01233 = eadd(t0123, x[3]);
-    EF y3 = eadd(t01233, edbl(x[0]));
-    EF y1 = eadd(t01123, edbl(x[2]));
-    EF y0 = eadd(t01123, t01);
-    EF y2 = eadd(t01233, t23);
-    x[0] = y0; x[1] = y1; x[2] = y2; x[3] = y3;
-}
-__device__ __forceinline__ void mds_light_ef(EF *s) {
-#pragma unroll
-    for (int c = 0; c < 16; c += 4) mat4_ef(s + c);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        EF acc = eadd(eadd(s[k], s[k + 4]), eadd(s[k + 8], s[k + 12]));
-        s[k] = eadd(s[k], acc);
-        s[k + 4] = eadd(s[k + 4], acc);
-        s[k + 8] = eadd(s[k + 8], acc);
-        s[k + 12] = eadd(s[k + 12], acc);
-    }
-}
-__device__ __forceinline__ void internal_ef(EF *s) {
-    EF sum = s[0];
-#pragma unroll
-    for (int i = 1; i < 16; i++) sum = eadd(sum, s[i]);
-#pragma unroll
-    for (int i = 0; i < 16; i++) s[i] = eadd(sum, emulf(s[i], F{DDIAG.d16[i]}));
-}
-
-template <class Get>
-__device__ __forceinline__ EF air_eval_ef(Get get, const AirConsts *air, const EF *ap) {
-    EF acc;
-    int k = 0;
-    {
-        EF e = get(0);
-        acc = emul(ap[k++], emul(e, esub(e, eone())));
-    }
-    EF s[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) s[i] = get(1 + i);
-    mds_light_ef(s);
-    int col = 17;
-#pragma unroll 1
-    for (int r = 0; r < 4; r++) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            EF t = s[i];
-            t.c[0] = fadd(t.c[0], air->rc[r * 16 + i]);
-            s[i] = ecube(t);
-        }
-        mds_light_ef(s);
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            EF post = get(col + i);
-            acc = eadd(acc, emul(ap[k + i], esub(s[i], post)));
-            s[i] = post;
-        }
-        k += 16;
-        col += 16;
-    }
-#pragma unroll 1
-    for (int r = 0; r < 20; r++) {
-        EF t = s[0];
-        t.c[0] = fadd(t.c[0], air->rc[64 + r]);
-        t = ecube(t);
-        EF post = get(col);
-        acc = eadd(acc, emul(ap[k], esub(t, post)));
-        k++;
-        col++;
-        s[0] = post;
-        internal_ef(s);
-    }
-#pragma unroll 1
-    for (int r = 0; r < 4; r++) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            EF t = s[i];
-            t.c[0] = fadd(t.c[0], air->rc[84 + r * 16 + i]);
-            s[i] = ecube(t);
-        }
-        mds_light_ef(s);
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            EF post = get(col + i);
-            acc = eadd(acc, emul(ap[k + i], esub(s[i], post)));
-            s[i] = post;
-        }
-        k += 16;
-        col += 16;
-    }
-    return acc;
 }
 
 // -------------------------------------------------------- zerocheck round 0 --
@@ -336,40 +298,6 @@ void zc_fold0(const F *trace, int log_n, int skips, const EF *S, EF *U, hipStrea
     k_zc_fold0<<<dim3(nblk(M), AIR_NCOLS), 256, 0, s>>>(trace, log_n, skips, S, U);
 }
 
-// ------------------------------------------------------ zerocheck rounds >= 1 --
-__global__ void __launch_bounds__(256) k_zc_round(const EF *U, size_t stride, int l,
-                                                 const AirConsts *air, const EF *ap,
-                                                 const EF *eq, EF *parts) {
-    const size_t h = (size_t)1 << (l - 1);
-    const int zi = blockIdx.y;  // z = 0, 2, 3
-    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    EF val = ezero();
-    if (x < h) {
-        auto get = [&](int c) -> EF {
-            const EF *col = U + (size_t)c * stride;
-            EF a = col[x];
-            if (zi == 0) return a;
-            EF d = esub(col[x + h], a);
-            EF r = eadd(a, edbl(d));
-            if (zi == 2) r = eadd(r, d);
-            return r;
-        };
-        EF rr = air_eval_ef(get, air, ap);
-        val = emul(rr, eq[x]);
-    }
-    __shared__ EF lds[16];
-    EF t = block_sum(val, lds);
-    if (threadIdx.x == 0) parts[(size_t)zi * gridDim.x + blockIdx.x] = t;
-}
-
-int zc_round_strided(const EF *U, size_t stride, int l, const AirConsts *air, const EF *ap,
-                     const EF *eq, EF *parts, hipStream_t s) {
-    size_t h = (size_t)1 << (l - 1);
-    unsigned nb = nblk(h);
-    k_zc_round<<<dim3(nb, 3), 256, 0, s>>>(U, stride, l, air, ap, eq, parts);
-    return (int)nb;
-}
-
 __global__ void k_fold_cols(EF *U, size_t stride, int l, const EF *c) {
     const size_t h = (size_t)1 << (l - 1);
     const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -381,6 +309,27 @@ __global__ void k_fold_cols(EF *U, size_t stride, int l, const EF *c) {
 void fold_cols_strided(EF *U, int n_cols, size_t stride, int l, const EF *c, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
     k_fold_cols<<<dim3(nblk(h), n_cols), 256, 0, s>>>(U, stride, l, c);
+}
+
+// column c < n_cols: fold by (1 - ch, ch); the extra row blockIdx.y == n_cols
+// halves the eq table for the next round.
+__global__ void k_fold_cols_v(EF *U, size_t stride, int l, int n_cols, EF ch, EF *E) {
+    const size_t h = (size_t)1 << (l - 1);
+    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= h) return;
+    if ((int)blockIdx.y == n_cols) {
+        // eq table of length h -> h / 2
+        size_t q = h >> 1;
+        if (x < q) E[x] = eadd(E[x], E[x + q]);
+        return;
+    }
+    EF *col = U + (size_t)blockIdx.y * stride;
+    EF a = col[x];
+    col[x] = eadd(a, emul(ch, esub(col[x + h], a)));
+}
+void fold_cols_v(EF *U, int n_cols, size_t stride, int l, EF c, EF *E, hipStream_t s) {
+    size_t h = (size_t)1 << (l - 1);
+    k_fold_cols_v<<<dim3(nblk(h), n_cols + (E && h > 1 ? 1 : 0)), 256, 0, s>>>(U, stride, l, n_cols, c, E);
 }
 
 // down inner sums
@@ -518,6 +467,17 @@ __global__ void k_fold_ef(EF *A, size_t h, const EF *c) {
 void fold_ef(EF *A, int l, const EF *c, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
     k_fold_ef<<<nblk(h), 256, 0, s>>>(A, h, c);
+}
+__global__ void k_fold_ef2_v(EF *A, EF *B, size_t h, EF c) {
+    size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= h) return;
+    EF *T = blockIdx.y ? B : A;
+    EF a = T[x];
+    T[x] = eadd(a, emul(c, esub(T[x + h], a)));
+}
+void fold_ef2_v(EF *A, EF *B, int l, EF c, hipStream_t s) {
+    size_t h = (size_t)1 << (l - 1);
+    k_fold_ef2_v<<<dim3(nblk(h), B ? 2 : 1), 256, 0, s>>>(A, B, h, c);
 }
 
 // -------------------------------------------------------------------- micro --

@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_trace(const uint32_t *inputs, size_t N,
     for (int r = 0; r < 20; r++) {
         s[0] = fcube(fadd(s[0], air->rc[64 + r]));
         put(s[0]);
-        internal_layer<16>(s, DDIAG.d16);
+        internal_layer_fast<16>(s);
     }
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {

@@ -193,12 +193,10 @@ __global__ void __launch_bounds__(256) k_merkle_leaves(const F *cw, int width, s
     for (int i = 0; i < 24; i++) s[i] = F{0};
     for (int off = 0; off < width; off += 16) {
         int n = width - off < 16 ? width - off : 16;
-        if (n == 16) {
+        // compile-time state indices (a runtime index would place the state in scratch)
 #pragma unroll
-            for (int j = 0; j < 16; j++) s[j] = cw[(size_t)(off + j) * rows + r];
-        } else {
-            for (int j = 0; j < n; j++) s[j] = cw[(size_t)(off + j) * rows + r];
-        }
+        for (int j = 0; j < 16; j++)
+            if (j < n) s[j] = cw[(size_t)(off + j) * rows + r];
         p24_permute(s, pc->p24);
     }
     F *o = nodes + (rows + r) * 8;
@@ -223,27 +221,97 @@ __global__ void __launch_bounds__(256) k_merkle_level(F *nodes, size_t cnt, cons
     compress_node(nodes, cnt + i, pc);
 }
 
-__global__ void __launch_bounds__(256) k_merkle_top(F *nodes, size_t cnt, const P2DevT *pc) {
-    // levels with parents [cnt, 2cnt), [cnt/2, cnt), ... , [1, 2) in one block
+// Poseidon2-16 with the state spread over a lane quad: lane q holds elements
+// [4q, 4q + 4), one M4 block, so the external layer needs two xor-shuffle
+// steps for the cross-block sums and the internal layer one quad-wide sum.
+// Used where a tree level has too few nodes to fill the GPU: it cuts the
+// dependent instruction chain of one compression by about 2.5x.
+__device__ __forceinline__ F quad_sum(F x) {
+    x = fadd(x, F{(uint32_t)__shfl_xor((int)x.v, 1, 64)});
+    x = fadd(x, F{(uint32_t)__shfl_xor((int)x.v, 2, 64)});
+    return x;
+}
+__device__ __forceinline__ void mds_quad(F *v) {
+    mat4(v);
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = fadd(v[k], quad_sum(v[k]));
+}
+__device__ __forceinline__ void p16_quad(F *v, int q, const P2Consts16 &k) {
+    mds_quad(v);
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] = fcube(fadd(v[i], k.ext_init[r][4 * q + i]));
+        mds_quad(v);
+    }
+    F d[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) d[i] = F{DDIAG.d16[4 * q + i]};
+#pragma unroll 1
+    for (int r = 0; r < 20; r++) {
+        if (q == 0) v[0] = fcube(fadd(v[0], k.internal[r]));
+        F sum = quad_sum(fadd(fadd(v[0], v[1]), fadd(v[2], v[3])));
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] = fadd(sum, fmul(v[i], d[i]));
+    }
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] = fcube(fadd(v[i], k.ext_term[r][4 * q + i]));
+        mds_quad(v);
+    }
+}
+// compress node i = (children 2i, 2i+1) by the quad whose lane index is q
+__device__ __forceinline__ void compress_node_quad(F *nodes, size_t i, int q, const P2DevT *pc) {
+    const F *src = nodes + 2 * i * 8 + 4 * q;  // left child = elements 0..7, right = 8..15
+    F v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = src[j];
+    p16_quad(v, q, pc->p16);
+    if (q < 2) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) nodes[i * 8 + 4 * q + j] = v[j];
+    }
+}
+__global__ void __launch_bounds__(256) k_merkle_level_q(F *nodes, size_t cnt, const P2DevT *pc) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t i = t >> 2;
+    if (i >= cnt) return;  // whole quads exit together
+    compress_node_quad(nodes, cnt + i, (int)(t & 3), pc);
+}
+__global__ void __launch_bounds__(1024) k_merkle_top(F *nodes, size_t cnt, const P2DevT *pc) {
+    // levels with parents [cnt, 2cnt), [cnt/2, cnt), ... , [1, 2) in one block of quads
+    const int q = threadIdx.x & 3, quad = threadIdx.x >> 2, nq = blockDim.x >> 2;
     for (size_t c = cnt; c >= 1; c >>= 1) {
-        for (size_t i = threadIdx.x; i < c; i += blockDim.x) compress_node(nodes, c + i, pc);
+        for (size_t i = quad; i < c; i += nq) compress_node_quad(nodes, c + i, q, pc);
         __syncthreads();
-        __threadfence_block();
     }
 }
 
+void merkle_leaves(const F *cw, int width, int log_rows, const void *pc, F *nodes, hipStream_t s) {
+    size_t rows = (size_t)1 << log_rows;
+    k_merkle_leaves<<<nblk(rows), 256, 0, s>>>(cw, width, rows, (const P2DevT *)pc, nodes);
+}
 void merkle_build(const F *cw, int width, int log_rows, const void *pc, const void *unused, F *nodes,
                   hipStream_t s) {
     (void)unused;
+    merkle_leaves(cw, width, log_rows, pc, nodes, s);
+    merkle_nodes(log_rows, pc, nodes, s);
+}
+void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s) {
     const P2DevT *p = (const P2DevT *)pc;
     size_t rows = (size_t)1 << log_rows;
-    k_merkle_leaves<<<nblk(rows), 256, 0, s>>>(cw, width, rows, p, nodes);
     size_t cnt = rows >> 1;
-    while (cnt > 256) {
+    // wide levels: one thread per node; narrower ones: one lane quad per node
+    while (cnt >= ((size_t)1 << 15)) {
         k_merkle_level<<<nblk(cnt), 256, 0, s>>>(nodes, cnt, p);
         cnt >>= 1;
     }
-    if (cnt >= 1) k_merkle_top<<<1, 256, 0, s>>>(nodes, cnt, p);
+    while (cnt > 256) {
+        k_merkle_level_q<<<nblk(cnt * 4), 256, 0, s>>>(nodes, cnt, p);
+        cnt >>= 1;
+    }
+    if (cnt >= 1) k_merkle_top<<<1, 1024, 0, s>>>(nodes, cnt, p);
 }
 
 __global__ void k_merkle_open(const F *cw, const F *nodes, int width, int log_rows,
@@ -258,6 +326,29 @@ __global__ void k_merkle_open(const F *cw, const F *nodes, int width, int log_ro
         paths_out[((size_t)q * log_rows + l) * 8 + j] = nodes[node * 8 + j];
     }
 }
+__global__ void k_fold_rows(const F *rows, int fold, bool base, const EF *eqr, EF *out) {
+    __shared__ EF lds[16];
+    const int q = blockIdx.x, K = 1 << fold;
+    const int width = base ? K : 8 * K;
+    EF acc = ezero();
+    for (int b = threadIdx.x; b < K; b += blockDim.x) {
+        const F *row = rows + (size_t)q * width;
+        if (base) {
+            acc = eadd(acc, emulf(eqr[b], row[b]));
+        } else {
+            EF v;
+#pragma unroll
+            for (int c = 0; c < 8; c++) v.c[c] = row[8 * b + c];
+            acc = eadd(acc, emul(eqr[b], v));
+        }
+    }
+    EF t = block_sum(acc, lds);
+    if (threadIdx.x == 0) out[q] = t;
+}
+void fold_rows(const F *rows, int nq, int fold, bool base, const EF *eqr, EF *out, hipStream_t s) {
+    if (nq > 0) k_fold_rows<<<nq, 128, 0, s>>>(rows, fold, base, eqr, out);
+}
+
 void merkle_open(const F *cw, const F *nodes, int width, int log_rows, const uint32_t *idx, int nq,
                  F *rows_out, F *paths_out, hipStream_t s) {
     if (nq > 0) k_merkle_open<<<nq, 128, 0, s>>>(cw, nodes, width, log_rows, idx, rows_out, paths_out);
@@ -276,6 +367,26 @@ __global__ void __launch_bounds__(256) k_grind(const P2DevT *pc, const F *st, in
     s[pos] = to_mont(w);
     p16_permute(s, pc->p16);
     if ((from_mont(s[7]) & mask) == 0) atomicMin(found, w);
+}
+__global__ void __launch_bounds__(256) k_grind_v(const P2DevT *pc, State16 st, int pos, uint32_t mask,
+                                                uint32_t base, uint32_t count, uint32_t *found) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    uint32_t w = base + g;
+    if (w >= P) return;
+    F s[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) s[i] = st.s[i];
+    // pos is uniform; select without dynamic indexing of the state
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (i == pos) s[i] = to_mont(w);
+    p16_permute(s, pc->p16);
+    if ((from_mont(s[7]) & mask) == 0) atomicMin(found, w);
+}
+void grind_v(const void *pc, State16 st, int pos, uint32_t mask, uint32_t base, uint32_t count,
+             uint32_t *found, hipStream_t s) {
+    k_grind_v<<<nblk(count), 256, 0, s>>>((const P2DevT *)pc, st, pos, mask, base, count, found);
 }
 void grind(const void *pc, const F *state16, int pos, uint32_t mask, uint32_t base, uint32_t count,
            uint32_t *found, hipStream_t s) {
@@ -302,28 +413,7 @@ void p2_permute_batch(const void *pc, int width, F *states, int n, hipStream_t s
 }
 
 // ------------------------------------------------------- WHIR reductions ----
-// Lazy EF x F accumulator (64-bit per coefficient, reduced every 4 terms).
-struct AccF2 {
-    uint64_t s[8];
-    EF r;
-    int n;
-    __device__ __forceinline__ AccF2() : n(0) {
-#pragma unroll
-        for (int i = 0; i < 8; i++) s[i] = 0;
-        r = ezero();
-    }
-    __device__ __forceinline__ void flush() {
-#pragma unroll
-        for (int i = 0; i < 8; i++) { r.c[i] = fadd(r.c[i], F{redc64(s[i])}); s[i] = 0; }
-        n = 0;
-    }
-    __device__ __forceinline__ void add(const EF &a, F b) {
-#pragma unroll
-        for (int i = 0; i < 8; i++) s[i] += (uint64_t)a.c[i].v * b.v;
-        if (++n == 4) flush();
-    }
-    __device__ __forceinline__ EF get() { flush(); return r; }
-};
+using AccF2 = AccF;
 
 constexpr int T_JG = 2;
 __global__ void k_whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts) {

@@ -1,0 +1,188 @@
+// k_zerocheck.hip -- zerocheck rounds >= 1 (crates/sumcheck/src/prove.rs:138-188,
+// skips = 1, z in {0, 2, 3}) with 64 or 256 lanes per (point, z).
+//
+// The Poseidon2 AIR constraints (p3-poseidon2-air, order fixed by
+// crates/utils/src/constraints_folder.rs:51-56) split into independent tasks:
+//   - S-boxes of the 7 full rounds whose input state is a column vector
+//     (beginning rounds 0..3: ext(inputs) / previous post; ending rounds 1..3:
+//     previous post),
+//   - the 20 partial rounds and the input of ending round 0, whose states are
+//     fixed linear combinations of columns (ChainLin), evaluated as dot products,
+//   - the external MDS row and the alpha-weighted constraint of each of the
+//     128 full-round outputs.
+// The folded column values of each point and all constants (alpha powers,
+// round constants, chain coefficients) live in LDS.  Large rounds give each
+// point one wave (4 points per block); small rounds, which are latency-bound,
+// give each point the whole 256-lane block.
+#include "common.h"
+#include "kernels.h"
+
+namespace wk {
+using namespace kb;
+
+namespace {
+
+constexpr int NC = 165;
+
+__device__ __forceinline__ EF m4row(const EF *v, int r) {
+    // row r of [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]] applied to v[0..4)
+    EF s = eadd(eadd(v[0], v[1]), eadd(v[2], v[3]));
+    switch (r) {
+    case 0: return eadd(s, eadd(v[0], edbl(v[1])));
+    case 1: return eadd(s, eadd(v[1], edbl(v[2])));
+    case 2: return eadd(s, eadd(v[2], edbl(v[3])));
+    default: return eadd(s, eadd(edbl(v[0]), v[3]));
+    }
+}
+
+// row i of the external layer (mds_light): M4 per block, then each lane adds the
+// sum over blocks of its residue class.
+__device__ __forceinline__ EF mds_row(const EF *v, int i) {
+    int r = i & 3;
+    EF own = m4row(v + (i & ~3), r);
+    EF s = eadd(eadd(m4row(v, r), m4row(v + 4, r)), eadd(m4row(v + 8, r), m4row(v + 12, r)));
+    return eadd(s, own);
+}
+
+__device__ __forceinline__ EF add_rc(EF t, F rc) {
+    t.c[0] = fadd(t.c[0], rc);
+    return t;
+}
+
+struct SharedConsts {
+    EF ap[AIR_NCONS];
+    F rc[AIR_NRC];
+    ChainLin cl;
+};
+
+}  // namespace
+
+// PTS points per 256-thread block; LANES = 256 / PTS lanes per point.
+template <int PTS>
+__global__ void __launch_bounds__(256) k_zc_round_w(const EF *U, size_t stride, int l,
+                                                   const AirConsts *air, const ChainLin *cl,
+                                                   const EF *ap, const EF *eq, EF *parts) {
+    constexpr int LANES = 256 / PTS;
+    __shared__ SharedConsts K;
+    __shared__ EF cols[PTS][NC];
+    __shared__ EF sb[PTS][128];
+    __shared__ EF red[4];
+    // constants: alpha powers, round constants, chain coefficients
+    {
+        const uint32_t *srcs[3] = {(const uint32_t *)ap, (const uint32_t *)air->rc, (const uint32_t *)cl};
+        uint32_t *dsts[3] = {(uint32_t *)K.ap, (uint32_t *)K.rc, (uint32_t *)&K.cl};
+        const int words[3] = {AIR_NCONS * 8, AIR_NRC, (int)(sizeof(ChainLin) / 4)};
+        for (int a = 0; a < 3; a++)
+            for (int i = threadIdx.x; i < words[a]; i += 256) dsts[a][i] = srcs[a][i];
+    }
+    const int pt = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    const size_t h = (size_t)1 << (l - 1);
+    const int zi = blockIdx.y;
+    const int z = zi ? zi + 1 : 0;  // 0, 2, 3
+    const size_t x = (size_t)blockIdx.x * PTS + pt;
+    const bool act = x < h;
+    EF *cw = cols[pt];
+    if (act) {
+        for (int c = lane; c < NC; c += LANES) {
+            const EF *col = U + (size_t)c * stride;
+            EF a = col[x];
+            if (z) {
+                EF d = esub(col[x + h], a);
+                a = eadd(a, edbl(d));
+                if (z == 3) a = eadd(a, d);
+            }
+            cw[c] = a;
+        }
+    }
+    __syncthreads();
+    EF acc = ezero();
+    if (act) {
+        // phase 1: S-boxes (112 full-round tasks) and the linear chain (20 + 16 tasks)
+        for (int t = lane; t < 148; t += LANES) {
+            if (t < 112) {
+                int r8 = t >> 4, i = t & 15;  // 0..3 beginning rounds, 4..6 ending rounds 1..3
+                EF prev;
+                F rc;
+                int slot;
+                if (r8 < 4) {
+                    prev = r8 == 0 ? mds_row(cw + 1, i) : cw[17 + 16 * (r8 - 1) + i];
+                    rc = K.rc[r8 * 16 + i];
+                    slot = r8;
+                } else {
+                    int er = r8 - 3;  // 1..3
+                    prev = cw[101 + 16 * (er - 1) + i];
+                    rc = K.rc[84 + er * 16 + i];
+                    slot = 4 + er;
+                }
+                sb[pt][slot * 16 + i] = ecube(add_rc(prev, rc));
+            } else if (t < 132) {
+                int r = t - 112;
+                AccF d;
+#pragma unroll 4
+                for (int j = 0; j < 16; j++) d.add(cw[65 + j], K.cl.A[r][j]);
+                for (int q = 0; q < r; q++) d.add(cw[81 + q], K.cl.B[r][q]);
+                EF tt = ecube(add_rc(d.get(), K.rc[64 + r]));
+                acc = eadd(acc, emul(K.ap[65 + r], esub(tt, cw[81 + r])));
+            } else {
+                int i = t - 132;
+                AccF d;
+#pragma unroll 4
+                for (int j = 0; j < 16; j++) d.add(cw[65 + j], K.cl.Ei[i][j]);
+#pragma unroll 4
+                for (int q = 0; q < 20; q++) d.add(cw[81 + q], K.cl.Ep[i][q]);
+                sb[pt][64 + i] = ecube(add_rc(d.get(), K.rc[84 + i]));
+            }
+        }
+    }
+    __syncthreads();
+    if (act) {
+        // phase 2: external layer rows and the full-round constraints
+        for (int t = lane; t < 128; t += LANES) {
+            int r8 = t >> 4, i = t & 15;  // 0..3 beginning, 4..7 ending rounds 0..3
+            EF out = mds_row(&sb[pt][r8 * 16], i);
+            int post = r8 < 4 ? 17 + 16 * r8 + i : 101 + 16 * (r8 - 4) + i;
+            int k = r8 < 4 ? 1 + 16 * r8 + i : 85 + 16 * (r8 - 4) + i;
+            acc = eadd(acc, emul(K.ap[k], esub(out, cw[post])));
+        }
+        if (lane == 0) {
+            EF e = cw[0];
+            acc = eadd(acc, emul(K.ap[0], emul(e, esub(e, eone()))));
+        }
+    }
+    acc = wave_sum(acc);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[wv] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        EF s = ezero();
+        for (int p = 0; p < PTS; p++) {
+            size_t xp = (size_t)blockIdx.x * PTS + p;
+            if (xp >= h) continue;
+            EF v = ezero();
+            for (int w = p * LANES / 64; w < (p + 1) * LANES / 64; w++) v = eadd(v, red[w]);
+            s = eadd(s, emul(v, eq[xp]));
+        }
+        parts[(size_t)zi * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// points per block for a round with 2^(l-1) points per z
+static int pts_per_block(int l) { return (((size_t)1 << (l - 1)) >= 1024) ? 4 : 1; }
+
+int zc_round_wave_blocks(int l) {
+    size_t h = (size_t)1 << (l - 1);
+    int p = pts_per_block(l);
+    return (int)((h + p - 1) / p);
+}
+
+int zc_round_wave(const EF *U, size_t stride, int l, const AirConsts *air, const ChainLin *cl,
+                  const EF *ap, const EF *eq, EF *parts, hipStream_t s) {
+    int nb = zc_round_wave_blocks(l);
+    if (pts_per_block(l) == 4)
+        k_zc_round_w<4><<<dim3(nb, 3), 256, 0, s>>>(U, stride, l, air, cl, ap, eq, parts);
+    else
+        k_zc_round_w<1><<<dim3(nb, 3), 256, 0, s>>>(U, stride, l, air, cl, ap, eq, parts);
+    return nb;
+}
+
+}  // namespace wk

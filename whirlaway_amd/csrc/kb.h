@@ -26,27 +26,45 @@ constexpr uint32_t W_EXT = 3u;                // X^8 = W
 
 struct F { uint32_t v; };
 
+// Conditional corrections are written as unsigned minima: for s in [0, 2P),
+// min(s, s - P) (wrapping) is s mod P; for d in (-P, P) stored wrapped,
+// min(d, d + P) is d mod P.  One v_min_u32 replaces a compare and a select.
+KB_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+// lo * MU mod 2^32 with MU = 1 + 2^24 + 2^31: two full-rate shift-adds instead
+// of a quarter-rate v_mul_lo_u32 (the compiler would fold the shifts back into a
+// multiply, so the device path states the two instructions; register-only ALU).
+KB_HD uint32_t mul_mu(uint32_t lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t t, r;
+    asm("v_lshl_add_u32 %0, %1, 24, %1" : "=v"(t) : "v"(lo));
+    asm("v_lshl_add_u32 %0, %1, 31, %2" : "=v"(r) : "v"(lo), "v"(t));
+    return r;
+#else
+    return lo + (lo << 24) + (lo << 31);
+#endif
+}
+
 KB_HD uint32_t redc(uint64_t x) {  // x < 2^32 * P  ->  x / 2^32 mod P
     uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    uint32_t m = lo * MU;
+    uint32_t m = mul_mu(lo);
     uint32_t mh = (uint32_t)(((uint64_t)m * P) >> 32);
-    uint32_t r = hi - mh;
-    return hi < mh ? r + P : r;
+    uint32_t r = hi - mh;  // in (-P, P)
+    return umin(r, r + P);
 }
 
 KB_HD uint32_t redc64(uint64_t x) {  // any x < 2^64
     uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    uint32_t m = lo * MU;
+    uint32_t m = mul_mu(lo);
     uint32_t mh = (uint32_t)(((uint64_t)m * P) >> 32);
     uint32_t r = hi - mh;
-    if (hi < mh) return r + P;
-    if (r >= P) r -= P;
-    if (r >= P) r -= P;
-    return r;
+    uint32_t rb = r + P;           // result when hi < mh
+    r = umin(r, r - P);            // otherwise r < 2^32 < 3P: two reductions
+    r = umin(r, r - P);
+    return hi < mh ? rb : r;
 }
 
-KB_HD F fadd(F a, F b) { uint32_t s = a.v + b.v; return F{s >= P ? s - P : s}; }
-KB_HD F fsub(F a, F b) { uint32_t d = a.v - b.v; return F{a.v < b.v ? d + P : d}; }
+KB_HD F fadd(F a, F b) { uint32_t s = a.v + b.v; return F{umin(s, s - P)}; }
+KB_HD F fsub(F a, F b) { uint32_t d = a.v - b.v; return F{umin(d, d + P)}; }
 KB_HD F fneg(F a) { return F{a.v ? P - a.v : 0u}; }
 KB_HD F fdbl(F a) { return fadd(a, a); }
 KB_HD F fmul(F a, F b) { return F{redc((uint64_t)a.v * b.v)}; }
@@ -142,6 +160,69 @@ KB_HD void mds_light(F *s) {
     for (int i = 0; i < W; i++) s[i] = fadd(s[i], sums[i & 3]);
 }
 
+// x * 2^-k (1 <= k <= 24) without a multiplication: with x = hi 2^k + lo and
+// 2^-k = -127 * 2^(24-k) mod P (since 127 * 2^24 = P - 1),
+// x 2^-k = hi + P - lo * 127 * 2^(24-k), a value in [1, 2P).
+// Works on Montgomery residues because scaling by a constant commutes with R.
+KB_HD F fdiv2k(F x, int k) {
+    uint32_t lo = x.v & ((1u << k) - 1), hi = x.v >> k;
+    uint32_t t = ((lo << 7) - lo) << (24 - k);
+    uint32_t r = hi + (P - t);
+    return F{umin(r, r - P)};
+}
+KB_HD F fhalf(F x) { return F{(x.v & 1) ? (x.v + P) >> 1 : x.v >> 1}; }
+
+// Internal diagonal entries as (sign, kind, value): kind 0 = small integer
+// multiplier, kind 1 = 2^-value.  16: [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4,
+// 1/2^8, 1/8, 1/2^24, -1/2^8, -1/8, -1/16, -1/2^24]; 24: [-2, 1, 2, 1/2, 3, 4,
+// -1/2, -3, -4, 1/2^8, 1/4, 1/8, 1/16, 1/32, 1/64, 1/2^24, -1/2^8, -1/8,
+// -1/16, -1/32, -1/64, -1/2^7, -1/2^9, -1/2^24].
+struct DiagEntry { int neg, kind, val; };
+constexpr DiagEntry DIAG16_SPEC[16] = {{1, 0, 2}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 0, 3}, {0, 0, 4},
+                                       {1, 1, 1}, {1, 0, 3}, {1, 0, 4}, {0, 1, 8}, {0, 1, 3}, {0, 1, 24},
+                                       {1, 1, 8}, {1, 1, 3}, {1, 1, 4}, {1, 1, 24}};
+constexpr DiagEntry DIAG24_SPEC[24] = {{1, 0, 2}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1}, {0, 0, 3}, {0, 0, 4},
+                                       {1, 1, 1}, {1, 0, 3}, {1, 0, 4}, {0, 1, 8}, {0, 1, 2}, {0, 1, 3},
+                                       {0, 1, 4}, {0, 1, 5}, {0, 1, 6}, {0, 1, 24}, {1, 1, 8}, {1, 1, 3},
+                                       {1, 1, 4}, {1, 1, 5}, {1, 1, 6}, {1, 1, 7}, {1, 1, 9}, {1, 1, 24}};
+
+template <int W, int I>
+KB_HD F diag_scale(F x) {  // |d_I| * x
+    constexpr DiagEntry e = W == 16 ? DIAG16_SPEC[I] : DIAG24_SPEC[I];
+    if constexpr (e.kind == 0) {
+        if constexpr (e.val == 1) return x;
+        else if constexpr (e.val == 2) return fdbl(x);
+        else if constexpr (e.val == 3) return fadd(fdbl(x), x);
+        else return fdbl(fdbl(x));
+    } else if constexpr (e.val == 1) {
+        return fhalf(x);
+    } else {
+        return fdiv2k(x, e.val);
+    }
+}
+template <int W, int I>
+KB_HD F diag_apply(F sum, F x) {  // sum + d_I * x
+    constexpr DiagEntry e = W == 16 ? DIAG16_SPEC[I] : DIAG24_SPEC[I];
+    F t = diag_scale<W, I>(x);
+    if constexpr (e.neg) return fsub(sum, t);
+    else return fadd(sum, t);
+}
+template <int W, int I = 0>
+KB_HD void internal_apply_all(F *s, F sum) {
+    if constexpr (I < W) {
+        s[I] = diag_apply<W, I>(sum, s[I]);
+        internal_apply_all<W, I + 1>(s, sum);
+    }
+}
+// s = (J + diag(d)) s with shifts and adds only
+template <int W>
+KB_HD void internal_layer_fast(F *s) {
+    F sum = s[0];
+#pragma unroll
+    for (int i = 1; i < W; i++) sum = fadd(sum, s[i]);
+    internal_apply_all<W>(s, sum);
+}
+
 template <int W>
 KB_HD void internal_layer(F *s, const uint32_t *diag) {
     F sum = s[0];
@@ -151,21 +232,25 @@ KB_HD void internal_layer(F *s, const uint32_t *diag) {
     for (int i = 0; i < W; i++) s[i] = fadd(sum, fmul(s[i], F{diag[i]}));
 }
 
+// Round loops are kept rolled so that each round loads only its own constants
+// (uniform scalar loads); unrolling them hoists every constant of the
+// permutation into scalar registers and spills.
 template <int W, int RP, typename C>
 KB_HD void p2_permute(F *s, const C &k, const uint32_t *diag) {
     mds_light<W>(s);
-#pragma unroll
+#pragma unroll 1
     for (int r = 0; r < 4; r++) {
 #pragma unroll
         for (int i = 0; i < W; i++) s[i] = fcube(fadd(s[i], k.ext_init[r][i]));
         mds_light<W>(s);
     }
-#pragma unroll 4
+    (void)diag;
+#pragma unroll 1
     for (int r = 0; r < RP; r++) {
         s[0] = fcube(fadd(s[0], k.internal[r]));
-        internal_layer<W>(s, diag);
+        internal_layer_fast<W>(s);
     }
-#pragma unroll
+#pragma unroll 1
     for (int r = 0; r < 4; r++) {
 #pragma unroll
         for (int i = 0; i < W; i++) s[i] = fcube(fadd(s[i], k.ext_term[r][i]));

@@ -16,15 +16,36 @@ constexpr int AIR_NRC = 4 * 16 + 20 + 4 * 16;  // beginning, partial, ending
 
 struct AirConsts { F rc[AIR_NRC]; };  // Montgomery form, beginning[4][16], partial[20], ending[4][16]
 
+// The 20 partial rounds overwrite state[0] with the post_sbox column and then
+// apply the internal layer, so every state they produce is a fixed linear
+// combination of the 16 columns entering the chain (post of beginning round 3)
+// and the post_sbox columns: s0 before partial round r = A[r] . init + B[r] . ps,
+// state after the chain = Ei . init + Ep . ps (Montgomery form constants).
+struct ChainLin { F A[20][16]; F B[20][20]; F Ei[16][16]; F Ep[16][20]; };
+
 // -------------------------------------------------------------- basic --
 void convert_to_mont(uint32_t *d, size_t n, hipStream_t s);
 void convert_from_mont(uint32_t *d, size_t n, hipStream_t s);
 void fill_zero(void *d, size_t bytes, hipStream_t s);
 // out[i] = sum_p in[i * n_parts + p]  (n_items outputs)
 void reduce_parts(const EF *in, int n_items, int n_parts, EF *out, hipStream_t s);
+// Copy n words to host-mapped memory, then (after a system-scope fence) store
+// seq into *flag; the host waits on the flag instead of synchronizing the stream.
+void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
+             hipStream_t s);
+// reduce_parts whose outputs go straight to host-mapped memory, then the flag.
+void reduce_publish(const EF *in, int n_items, int n_parts, EF *host_dst, volatile uint32_t *flag,
+                    uint32_t seq, hipStream_t s);
 
 // eq(point, .) over n variables, big-endian, times scale; point on device.
 void eq_table(const EF *d_point, int n, const EF *d_scale, EF *out, hipStream_t s);
+// Same with the point passed by value (n <= 32).
+struct EqPoint { EF p[32]; EF scale; int n; };
+void eq_table_v(const EqPoint &pt, EF *out, hipStream_t s);
+// E[x] += E[x + 2^(l-1)] for x < 2^(l-1): eq(r[1..], .) from eq(r, .) since the
+// two halves of eq(r, .) sum to eq(r[1..], .).
+void eq_halve(EF *E, int l, hipStream_t s);
+struct State16 { F s[16]; };
 
 // ------------------------------------------------------------ zerocheck --
 // Round 0 (univariate skip, 2^skips nodes): for each of nz evaluation points z
@@ -38,11 +59,15 @@ void zc_fold0(const F *trace, int log_n, int skips, const EF *S, EF *U, hipStrea
 // Rounds >= 1: U holds 165 EF columns (column c at U + c * stride) of current
 // length 2^l.  For z in {0,2,3}: parts[zi][block] partial sums of
 // eq[x] * AIR(U_z)(x), x < 2^(l-1).
-int zc_round_strided(const EF *U, size_t stride, int l, const AirConsts *air,
-                     const EF *alpha_pows, const EF *eq, EF *parts, hipStream_t s);
+// One 64-lane wave per (point, z); returns the block count (parts[zi][block]).
+int zc_round_wave(const EF *U, size_t stride, int l, const AirConsts *air, const ChainLin *cl,
+                  const EF *alpha_pows, const EF *eq, EF *parts, hipStream_t s);
+int zc_round_wave_blocks(int l);
 // In-place fold of n_cols EF columns by (1-c, c): col[x] += c (col[x+h] - col[x]);
 // the column stride is unchanged.
 void fold_cols_strided(EF *U, int n_cols, size_t stride, int l, const EF *c, hipStream_t s);
+// by-value challenge; also halves the eq table E (length 2^(l-1)) when E != nullptr
+void fold_cols_v(EF *U, int n_cols, size_t stride, int l, EF c, EF *E, hipStream_t s);
 // down inner sums: parts[c][j] = sum_{x'} eqr[x'] * down_c(j*M + x'), M = 2^(log_n-skips)
 void zc_down_parts(const F *trace, int log_n, int skips, const EF *eqr, EF *parts, hipStream_t s);
 // out[c] = sum_j S[j] * parts[c][j]
@@ -69,6 +94,8 @@ int whir_T_chunks(int m, int k);
 int prod_round(const EF *A, const EF *B, int l, EF *parts, hipStream_t s);
 // in-place fold of one EF table by (1-c, c)
 void fold_ef(EF *A, int l, const EF *c, hipStream_t s);
+// fold one or two EF tables by a by-value challenge (B may be nullptr)
+void fold_ef2_v(EF *A, EF *B, int l, EF c, hipStream_t s);
 
 // -------------------------------------------------------------- micro --
 // fold an F table (2^l) by 2^k EF scalars: out[i] = sum_j s_j in[i + j 2^(l-k)]
@@ -107,11 +134,19 @@ void twiddles(F *out, int L, hipStream_t s);
 // Merkle tree over rows of a column-major codeword (width F per leaf), heap layout nodes[2*rows][8]
 void merkle_build(const F *cw, int width, int log_rows, const void *p16, const void *p24, F *nodes,
                   hipStream_t s);
+// the two halves of merkle_build: leaf hashes, then the compression levels
+void merkle_leaves(const F *cw, int width, int log_rows, const void *pc, F *nodes, hipStream_t s);
+void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s);
 // gather rows (width F each) and sibling paths for sorted query indices
 void merkle_open(const F *cw, const F *nodes, int width, int log_rows, const uint32_t *idx, int nq,
                  F *rows_out, F *paths_out, hipStream_t s);
+// folded query values: out[q] = sum_b eqr[b] * row_q[b], rows of 2^fold F (base)
+// or 2^fold EF (8 F each, coefficient-wise) values
+void fold_rows(const F *rows, int nq, int fold, bool base, const EF *eqr, EF *out, hipStream_t s);
 // minimal w in [base, base + count) with P16(state with w at position pos)[7] & mask == 0,
 // result via atomicMin into *found (init 0xffffffff)
+void grind_v(const void *p16, State16 st, int pos, uint32_t mask, uint32_t base, uint32_t count,
+             uint32_t *found, hipStream_t s);
 void grind(const void *p16, const F *state16, int pos, uint32_t mask, uint32_t base, uint32_t count,
            uint32_t *found, hipStream_t s);
 // device Poseidon2 (tests): permute n states of width W in place

@@ -47,9 +47,21 @@ void commit_codeword(wl_ctx *c, const P2Dev *pdev, Codeword &cwd, const F *cols,
     cwd.width = n_cols;
     cwd.log_rows = log_rows;
     DBuf<F> tmp(c->pool, (size_t)std::max(1, n_nonzero) * rows);
-    wk::rs_encode(cols, n_cols, n_nonzero, log_m, log_rows, cwd.cw, tmp, c->twiddle(log_rows), c->stream);
-    wk::merkle_build(cwd.cw, n_cols, log_rows, pdev, nullptr, cwd.nodes, c->stream);
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    F *tw = c->twiddle(log_rows);
+    {
+        // algorithmic HBM bytes: read the evaluations, write the codeword columns
+        KTimer kt(c, KT_NTT, 4.0 * (double)n_nonzero * (double)(rows + ((size_t)1 << log_m)));
+        wk::rs_encode(cols, n_cols, n_nonzero, log_m, log_rows, cwd.cw, tmp, tw, c->stream);
+    }
+    {
+        // one Poseidon2-24 permutation per 16 leaf elements (PaddingFreeSponge rate 16)
+        KTimer kt(c, KT_MERKLE_LEAVES, (double)rows * ((n_cols + 15) / 16));
+        wk::merkle_leaves(cwd.cw, n_cols, log_rows, pdev, cwd.nodes, c->stream);
+    }
+    {
+        KTimer kt(c, KT_MERKLE_NODES, (double)(rows - 1));
+        wk::merkle_nodes(log_rows, pdev, cwd.nodes, c->stream);
+    }
 }
 
 std::vector<uint32_t> sample_queries(Transcript &ts, int n, int log_rows) {
@@ -60,34 +72,38 @@ std::vector<uint32_t> sample_queries(Transcript &ts, int n, int log_rows) {
     return q;
 }
 
-// Open rows + paths as hints; returns the rows (Montgomery) for folding.
-std::vector<F> open_queries(wl_ctx *c, Transcript &ts, const Codeword &cwd, const std::vector<uint32_t> &q) {
+// Open rows + sibling paths of the sorted query indices as hints; when eqr is
+// given, also return the rows folded by it (computed on the device).
+std::vector<EF> open_queries(wl_ctx *c, Transcript &ts, const Codeword &cwd, const std::vector<uint32_t> &q,
+                             const std::vector<EF> *eqr) {
     int nq = (int)q.size();
-    std::vector<F> rows((size_t)nq * cwd.width), paths((size_t)nq * cwd.log_rows * 8);
-    if (nq == 0) return rows;
+    std::vector<EF> vals;
+    if (nq == 0) return vals;
     for (uint32_t x : q)
         if ((size_t)x >= ((size_t)1 << cwd.log_rows)) throw std::runtime_error("query index outside the codeword");
+    size_t nrow = (size_t)nq * cwd.width, npath = (size_t)nq * cwd.log_rows * 8;
+    size_t nval = eqr ? (size_t)nq * 8 : 0;
+    // one device block [rows | paths | folded values] so that one fetch brings it all
+    DBuf<F> buf(c->pool, nrow + npath + nval);
     DBuf<uint32_t> didx(c->pool, nq);
-    c->h2d(didx.p, q.data(), nq);
-    DBuf<F> drows(c->pool, rows.size()), dpaths(c->pool, paths.size() ? paths.size() : 1);
-    wk::merkle_open(cwd.cw, cwd.nodes, cwd.width, cwd.log_rows, didx, nq, drows, dpaths, c->stream);
-    c->d2h(rows.data(), drows.p, rows.size());
-    if (!paths.empty()) c->d2h(paths.data(), dpaths.p, paths.size());
+    c->h2d_async(didx.p, q.data(), nq);
+    wk::merkle_open(cwd.cw, cwd.nodes, cwd.width, cwd.log_rows, didx, nq, buf.p, buf.p + nrow, c->stream);
+    DBuf<EF> deq;
+    if (eqr) {
+        deq = upload_ef(c, *eqr);
+        wk::fold_rows(buf.p, nq, cwd.fold, cwd.base, deq, (EF *)(buf.p + nrow + npath), c->stream);
+    }
+    std::vector<F> host(nrow + npath + nval);
+    c->d2h(host.data(), buf.p, host.size());
     for (int i = 0; i < nq; i++) {
-        ts.hint_base(&rows[(size_t)i * cwd.width], cwd.width);
-        ts.hint_base(&paths[(size_t)i * cwd.log_rows * 8], (size_t)cwd.log_rows * 8);
+        ts.hint_base(&host[(size_t)i * cwd.width], cwd.width);
+        ts.hint_base(&host[nrow + (size_t)i * cwd.log_rows * 8], (size_t)cwd.log_rows * 8);
     }
-    return rows;
-}
-
-EF fold_row(const F *row, bool base, int fold, const std::vector<EF> &eqr) {
-    EF acc = ezero();
-    int K = 1 << fold;
-    for (int b = 0; b < K; b++) {
-        if (base) acc = eadd(acc, emulf(eqr[b], row[b]));
-        else { EF v; memcpy(v.c, row + 8 * b, 32); acc = eadd(acc, emul(eqr[b], v)); }
+    if (eqr) {
+        vals.resize(nq);
+        memcpy(vals.data(), &host[nrow + npath], nval * sizeof(F));
     }
-    return acc;
+    return vals;
 }
 
 // A WHIR equality constraint on the current polynomial (m variables):
@@ -110,10 +126,25 @@ std::vector<EF> cons_point(const Constraint &k, int m) {
 // After binding the top `r.size()` variables to r: scal *= eq(prefix, r), point -> suffix.
 void fold_constraint(Constraint &k, int m, const std::vector<EF> &r) {
     int kk = (int)r.size();
+    if (k.kind == 2) {
+        // prefix of expand(y, m): y^(2^(m-1)), ..., y^(2^(m-kk)), computed in F;
+        // eq factor per coordinate: (1 - p) + r_t (2p - 1)
+        std::vector<F> pw(m);
+        F cur = k.y;
+        for (int i = 0; i < m; i++) { pw[m - 1 - i] = cur; cur = fmul(cur, cur); }
+        EF acc = eone();
+        for (int t = 0; t < kk; t++) {
+            F p = pw[t];
+            EF f = emulf(r[t], fsub(fdbl(p), fone()));
+            f.c[0] = fadd(f.c[0], fsub(fone(), p));
+            acc = emul(acc, f);
+        }
+        k.scal = emul(k.scal, acc);
+        return;  // expand(y, m)[kk..] = expand(y, m - kk): the parameter is unchanged
+    }
     std::vector<EF> pt = cons_point(k, m);
     k.scal = emul(k.scal, wh::eq_outside(pt.data(), r.data(), kk));
     if (k.kind == 0) k.pt.erase(k.pt.begin(), k.pt.begin() + kk);
-    // univariate kinds: expand(y, m)[kk..] = expand(y, m - kk): unchanged parameter
 }
 
 // W(x) += sum_k scal_k eq(point_k, x) over m variables (hi/lo tensor split).
@@ -146,26 +177,18 @@ void add_constraints_w(wl_ctx *c, EF *W, int m, const std::vector<Constraint> &c
     DBuf<EF> dsF = upload_ef(c, sF), dsE = upload_ef(c, sE);
     if (nF) {
         DBuf<F> dyl(c->pool, nF), dyh(c->pool, nF);
-        c->h2d(dyl.p, yl.data(), nF);
-        c->h2d(dyh.p, yh.data(), nF);
+        c->h2d_async(dyl.p, yl.data(), nF);
+        c->h2d_async(dyh.p, yh.data(), nF);
         wk::eq_univ_f_tables(dyh, nF, hi, hiF, c->stream);
         wk::eq_univ_f_tables(dyl, nF, lo, loF, c->stream);
-        HIP_CHECK(hipStreamSynchronize(c->stream));
     }
-    std::vector<DBuf<EF>> keep;
     for (int j = 0; j < nE; j++) {
         std::vector<EF> pt = cons_point(*efc[j], m);
         std::vector<EF> ph(pt.begin(), pt.begin() + hi), pl(pt.begin() + hi, pt.end());
-        std::vector<EF> up(ph);
-        up.insert(up.end(), pl.begin(), pl.end());
-        up.push_back(eone());
-        keep.push_back(upload_ef(c, up));
-        const EF *d = keep.back().p;
-        wk::eq_table(d, hi, d + m, hiE.p + (size_t)j * HI, c->stream);
-        wk::eq_table(d + hi, lo, d + m, loE.p + (size_t)j * LO, c->stream);
+        eq_into(c, ph, eone(), hiE.p + (size_t)j * HI);
+        eq_into(c, pl, eone(), loE.p + (size_t)j * LO);
     }
     wk::whir_build_w(W, m, lo, nF, dsF, hiF, loF, nE, dsE, hiE, loE, !zero_first, c->stream);
-    HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
 struct SumcheckState {
@@ -185,10 +208,7 @@ EF whir_sc_round(wl_ctx *c, Transcript &ts, const P2Dev *pdev, SumcheckState &s,
     grind(c, ts, pdev, pow_bits);
     EF r = ts.sample_ef();
     s.sum = wh::interp3(h, r);
-    DBuf<EF> dr = upload_ef(c, std::vector<EF>(1, r));
-    wk::fold_ef(s.f, s.m, dr, c->stream);
-    wk::fold_ef(s.W, s.m, dr, c->stream);
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    wk::fold_ef2_v(s.f, s.W, s.m, r, c->stream);
     s.m--;
     return r;
 }
@@ -292,21 +312,16 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
     int X = v - f0, U = 1 << f0, nC = (int)cs.size();
     DBuf<EF> E(c->pool, (size_t)nC << X);
     std::vector<std::vector<EF>> A(nC);
-    std::vector<DBuf<EF>> keep;
     for (int j = 0; j < nC; j++) {
         std::vector<EF> pt = cons_point(cs[j], v);
         std::vector<EF> pre(pt.begin(), pt.begin() + f0), suf(pt.begin() + f0, pt.end());
         A[j] = wh::eq_host(pre);
-        std::vector<EF> up(suf);
-        up.push_back(eone());
-        keep.push_back(upload_ef(c, up));
-        wk::eq_table(keep.back().p, X, keep.back().p + X, E.p + ((size_t)j << X), c->stream);
+        eq_into(c, suf, eone(), E.p + ((size_t)j << X));
     }
     int nch = wk::whir_T_chunks(v, f0);
     DBuf<EF> parts(c->pool, (size_t)nC * U * nch);
     wk::whir_T_f(w.evals, v, f0, E, nC, parts, c->stream);
     std::vector<EF> T = reduce_many(c, parts, nC * U, nch);
-    keep.clear();
     std::vector<std::vector<EF>> Tj(nC);
     for (int j = 0; j < nC; j++) Tj[j].assign(T.begin() + (size_t)j * U, T.begin() + (size_t)(j + 1) * U);
     std::vector<EF> rf;
@@ -348,7 +363,6 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
         std::vector<EF> eqr = wh::eq_host(rf);
         DBuf<EF> deq = upload_ef(c, eqr);
         wk::whir_fold_f(w.evals, v, f0, deq, s.f, c->stream);
-        HIP_CHECK(hipStreamSynchronize(c->stream));
     }
     s.W = DBuf<EF>(c->pool, (size_t)1 << X);
     add_constraints_w(c, s.W, X, cs, true);
@@ -380,8 +394,8 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
         grind(c, ts, pdev, rp.pow_bits);
         int log_rows = rp.log_domain_size - prev->fold;
         std::vector<uint32_t> q = sample_queries(ts, rp.num_queries, log_rows);
-        std::vector<F> rows = open_queries(c, ts, *prev, q);
         std::vector<EF> eqr = wh::eq_host(rf);
+        std::vector<EF> qvals = open_queries(c, ts, *prev, q, &eqr);
         EF gg = ts.sample_ef(), gpp = eone();
         std::vector<Constraint> fresh;
         for (int i = 0; i < rp.ood_samples; i++) {
@@ -391,7 +405,7 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
         }
         F gen = fpow(fsmall(3), (uint64_t)(P - 1) >> log_rows);
         for (size_t i = 0; i < q.size(); i++) {
-            EF val = fold_row(&rows[i * prev->width], prev->base, prev->fold, eqr);
+            EF val = qvals[i];
             fresh.push_back(Constraint{gpp, 2, {}, ezero(), fpow(gen, q[i])});
             s.sum = eadd(s.sum, emul(gpp, val));
             gpp = emul(gpp, gg);
@@ -415,7 +429,7 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
     {
         int log_rows = cfg.final_log_domain_size - prev->fold;
         std::vector<uint32_t> q = sample_queries(ts, cfg.final_queries, log_rows);
-        open_queries(c, ts, *prev, q);
+        open_queries(c, ts, *prev, q, nullptr);
     }
     for (int k = 0; k < cfg.final_sumcheck_rounds; k++) host_sc_round(ts, c, pdev, ff, Wf, s.sum, cfg.final_folding_pow_bits);
 }
@@ -433,6 +447,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
                                   (int)st->log_inv_rate, (int)st->rs_domain_initial_reduction);
     Transcript ts(&ph.p16);
     for (uint32_t x : wh::whir_domsep(cfg)) ts.observe(to_mont(x));
+    c->ring_reset();
 
     auto tp = std::chrono::steady_clock::now();
     auto lap = [&](int i) {
@@ -472,10 +487,13 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
             sel.insert(sel.end(), s.begin(), s.end());
         }
         DBuf<F> dsel(c->pool, sel.size());
-        c->h2d(dsel.p, sel.data(), sel.size());
+        c->h2d_async(dsel.p, sel.data(), sel.size());
         size_t nbx = (M + 255) / 256;
         DBuf<EF> parts(c->pool, (size_t)nzp * nbx);
-        wk::zc_round0(packed, n, skips, dsel, nzp, t->air_dev, dap, eq, parts, c->stream);
+        {
+            KTimer kt(c, KT_ZC_ROUND0, (double)nzp * M);
+            wk::zc_round0(packed, n, skips, dsel, nzp, t->air_dev, dap, eq, parts, c->stream);
+        }
         std::vector<EF> vals = reduce_many(c, parts, nzp, (int)nbx);
         std::vector<F> xs;
         std::vector<EF> ys;
@@ -497,17 +515,20 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
         missing = mm;
         DBuf<EF> dS = upload_ef(c, sc0);
         wk::zc_fold0(packed, n, skips, dS, U, c->stream);
-        HIP_CHECK(hipStreamSynchronize(c->stream));
     }
     lap(1);
     // ---- zerocheck rounds 1 .. n - skips ----
     int l = n - skips;
+    // eq(zc[1 + j ..], .) for round j; each fold launch halves it for the next round
+    DBuf<EF> eq = eq_dev(c, std::vector<EF>(zc.begin() + 2, zc.end()), eone());
     for (int j = 1; j < nz; j++) {
         EF r = zc[j];
-        DBuf<EF> eq = eq_dev(c, std::vector<EF>(zc.begin() + 1 + j, zc.end()), eone());
-        size_t nbx = (((size_t)1 << (l - 1)) + 255) / 256;
+        size_t nbx = (size_t)wk::zc_round_wave_blocks(l);
         DBuf<EF> parts(c->pool, 3 * nbx);
-        wk::zc_round_strided(U, M, l, t->air_dev, dap, eq, parts, c->stream);
+        {
+            KTimer kt(c, KT_ZC_ROUND, 3.0 * (double)((size_t)1 << (l - 1)));
+            wk::zc_round_wave(U, M, l, t->air_dev, t->chain_dev, dap, eq, parts, c->stream);
+        }
         std::vector<EF> v3 = reduce_many(c, parts, 3, (int)nbx);
         EF e0 = emul(v3[0], missing), e2 = emul(v3[1], missing), e3 = emul(v3[2], missing);
         EF e1 = emul(esub(sum, emul(e0, esub(eone(), r))), wh::einv(r));
@@ -520,11 +541,10 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
         sum = wh::peval(p, cj);
         grind(c, ts, pdev, pow_bits_sumcheck(sec, 3 + 1));
         missing = emul(eadd(emul(esub(eone(), r), esub(eone(), cj)), emul(r, cj)), missing);
-        DBuf<EF> dc = upload_ef(c, std::vector<EF>(1, cj));
-        wk::fold_cols_strided(U, AIR_NCOLS, M, l, dc, c->stream);
-        HIP_CHECK(hipStreamSynchronize(c->stream));
+        wk::fold_cols_v(U, AIR_NCOLS, M, l, cj, eq, c->stream);
         l--;
     }
+    eq.reset();
     lap(2);
     // ---- inner sums ----
     std::vector<EF> up(AIR_NCOLS), dn(AIR_NCOLS);
@@ -599,10 +619,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
         ich.push_back(cj);
         isum = wh::peval(p, cj);
         grind(c, ts, pdev, pow_bits_sumcheck(sec, 2));
-        DBuf<EF> dc = upload_ef(c, std::vector<EF>(1, cj));
-        wk::fold_ef(Mt, ll, dc, c->stream);
-        wk::fold_ef(B, ll, dc, c->stream);
-        HIP_CHECK(hipStreamSynchronize(c->stream));
+        wk::fold_ef2_v(Mt, B, ll, cj, c->stream);
     }
     EF packed_value;
     c->d2h(&packed_value, B.p, 1);
@@ -617,6 +634,30 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
 }
 
 }  // namespace
+
+// Coefficients of the partial-round chain as linear maps of (init, post_sbox):
+// follow the state symbolically through the 20 rounds of
+// state[0] := post_sbox_r; state := (J + diag) state.
+static void chain_linear_maps(wk::ChainLin &cl) {
+    const int NV = 36;  // 16 entering columns + 20 post_sbox columns
+    std::vector<std::vector<F>> st(16, std::vector<F>(NV, fzero()));
+    for (int i = 0; i < 16; i++) st[i][i] = fone();
+    for (int r = 0; r < 20; r++) {
+        for (int j = 0; j < 16; j++) cl.A[r][j] = st[0][j];
+        for (int q = 0; q < 20; q++) cl.B[r][q] = st[0][16 + q];
+        std::fill(st[0].begin(), st[0].end(), fzero());
+        st[0][16 + r] = fone();
+        std::vector<F> sum(NV, fzero());
+        for (int i = 0; i < 16; i++)
+            for (int k = 0; k < NV; k++) sum[k] = fadd(sum[k], st[i][k]);
+        for (int i = 0; i < 16; i++)
+            for (int k = 0; k < NV; k++) st[i][k] = fadd(sum[k], fmul(F{DIAG.d16[i]}, st[i][k]));
+    }
+    for (int i = 0; i < 16; i++) {
+        for (int j = 0; j < 16; j++) cl.Ei[i][j] = st[i][j];
+        for (int q = 0; q < 20; q++) cl.Ep[i][q] = st[i][16 + q];
+    }
+}
 
 // ===================================================================== C ABI ==
 extern "C" {
@@ -647,6 +688,11 @@ void wl_destroy(wl_ctx *c) {
     for (auto &kv : c->tw) (void)hipFree(kv.second);
     c->pool.release();
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->ring) (void)hipHostFree(c->ring);
+    if (c->mapped) (void)hipHostFree(c->mapped);
+    for (auto &e : c->kt_free) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto &v : c->kt_pairs)
+        for (auto &e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);
@@ -670,6 +716,10 @@ wl_table *wl_air_table_new(wl_ctx *c, const uint32_t *rc, uint32_t log_length, u
         for (int i = 0; i < wk::AIR_NRC; i++) t->air_host.rc[i] = to_mont(rc[i] % P);
         HIP_CHECK(hipMalloc(&t->air_dev, sizeof(wk::AirConsts)));
         HIP_CHECK(hipMemcpy(t->air_dev, &t->air_host, sizeof(wk::AirConsts), hipMemcpyHostToDevice));
+        wk::ChainLin cl;
+        chain_linear_maps(cl);
+        HIP_CHECK(hipMalloc(&t->chain_dev, sizeof(wk::ChainLin)));
+        HIP_CHECK(hipMemcpy(t->chain_dev, &cl, sizeof(wk::ChainLin), hipMemcpyHostToDevice));
         return t;
     } catch (const std::exception &e) {
         g_err = e.what();
@@ -680,6 +730,7 @@ wl_table *wl_air_table_new(wl_ctx *c, const uint32_t *rc, uint32_t log_length, u
 void wl_air_table_free(wl_table *t) {
     if (!t) return;
     (void)hipFree(t->air_dev);
+    (void)hipFree(t->chain_dev);
     delete t;
 }
 
@@ -766,6 +817,39 @@ int wl_air_prove_resident(wl_table *t, const wl_air_settings *st, const uint32_t
     try {
         HIP_CHECK(hipSetDevice(t->ctx->device));
         return prove_common(t, st, p16c, p24c, w->packed, proof_out, proof_cap, proof_len);
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return WL_ERR_DEVICE;
+    }
+}
+
+int wl_kernel_timing(wl_ctx *c, uint32_t class_mask) {
+    if (!c) { g_err = "null context"; return WL_ERR_ARG; }
+    (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 8; k++) {
+        for (auto &e : c->kt_pairs[k]) c->kt_free.push_back(e);
+        c->kt_pairs[k].clear();
+        c->kt_work[k] = 0;
+        c->kt_launches[k] = 0;
+    }
+    c->kt_mask = class_mask;
+    return WL_OK;
+}
+
+int wl_kernel_timing_read(wl_ctx *c, int kernel_class, long *launches, double *total_ms, double *work) {
+    if (!c || kernel_class < 0 || kernel_class >= 8) { g_err = "bad arguments"; return WL_ERR_ARG; }
+    try {
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        double ms = 0;
+        for (auto &e : c->kt_pairs[kernel_class]) {
+            float t = 0;
+            HIP_CHECK(hipEventElapsedTime(&t, e.first, e.second));
+            ms += t;
+        }
+        if (launches) *launches = c->kt_launches[kernel_class];
+        if (total_ms) *total_ms = ms;
+        if (work) *work = c->kt_work[kernel_class];
+        return WL_OK;
     } catch (const std::exception &e) {
         g_err = e.what();
         return WL_ERR_DEVICE;
