@@ -26,6 +26,14 @@ sys.path.insert(0, ROOT)
 # AIR evaluation point 31168 F-mul; one skip-round point 5700 F-mul.
 VALU_PEAK_TFMUL = 256 * 64 * 2.4e9 / 4 / 3 / 1e12   # quarter-rate 32-bit multiplies, 3 per F-mul
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of each kernel class at 2^17 permutations, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE passes (profiles/prove_n17_hbm_traffic.md), fetch scaled x2.0
+# and write x1.0 by the same counters on a 256 MiB device copy.  (read MiB, write MiB)
+MEASURED_TRAFFIC_N17 = {
+    1: (92.24, 5.75),                    # k_merkle_leaves
+    3: (96.09 + 43.97, 69.50 + 69.50),   # k_ntt_a + k_ntt_b (one rs_encode)
+}
+PHASES = ["commit", "zerocheck_round0", "zerocheck_rounds", "batching", "inner_sumcheck", "whir_open"]
 MODELS = {
     1: ("merkle_leaves (Poseidon2-24 sponge)", "valu", 982.0),
     2: ("zerocheck_rounds (EF AIR eval, rounds>=1)", "valu", 31168.0),
@@ -45,6 +53,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-log-n", type=int, default=17)
+    ap.add_argument("--pcie-steps", type=int, default=5,
+                    help="extra proofs timed from a host-memory witness (PCIe-inclusive rate)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (multi-GPU sharding is a design "
@@ -52,9 +64,19 @@ def main():
         sys.exit(2)
 
     import whirlaway_amd as wa
-    from whirlaway_amd.air import bench_inputs, generate_witness
+    from whirlaway_amd.air import bench_inputs, download_witness, generate_witness
 
     n = args.log_n
+    if args.copy_probe_mib:
+        # a plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # give the scale factor for the kernels' HBM traffic
+        import torch
+        a = torch.ones(args.copy_probe_mib << 18, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     ctx = wa.Context(0)
     rc, inputs, p16, p24 = bench_inputs(args.seed, n)
     table = wa.AirTable(ctx, rc, n)
@@ -65,17 +87,33 @@ def main():
         proof = table.prove(settings, p16, p24, wit)
 
     ctx.kernel_timing(list(MODELS))
+    ctx.phase_times(reset=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         proof = table.prove(settings, p16, p24, wit)
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     value = (1 << n) / (ms / 1e3)
-
     per_class = {}
     for k, (name, bound, per_unit) in MODELS.items():
         launches, kms, work = ctx.kernel_timing_read(k)
         per_class[k] = (launches, kms, work)
+    ctx.kernel_timing([])
+    phases = ctx.phase_times(reset=True)
+
+    pcie = None
+    if args.pcie_steps > 0:
+        # same proof with the witness handed over in host memory (wl_air_prove): adds the
+        # host-to-device copy of the 165 x 2^n trace to every proof
+        host_w = download_witness(wit)
+        tp = time.perf_counter()
+        for _ in range(args.pcie_steps):
+            proof_h = table.prove(settings, p16, p24, host_w)
+        tp = (time.perf_counter() - tp) / args.pcie_steps
+        pcie = {"value": round((1 << n) / tp, 1), "ms_per_step": round(tp * 1e3, 4),
+                "steps": args.pcie_steps, "same_proof": bool((proof_h == proof).all())}
+        del host_w
+
     dom = max(per_class, key=lambda k: per_class[k][1])
     name, bound, per_unit = MODELS[dom]
     launches, kms, work = per_class[dom]
@@ -87,9 +125,13 @@ def main():
     else:
         achieved = per_launch / (avg_ms / 1e3) / 1e12
         peak, unit = VALU_PEAK_TFMUL, "TFmul/s"
+    traffic = None
+    if n == 17 and dom in MEASURED_TRAFFIC_N17:
+        rd, wr = MEASURED_TRAFFIC_N17[dom]
+        traffic = round((rd + wr) * 2**20)
     roofline = {
         "bound": bound, "achieved": round(achieved, 4), "peak": round(peak, 4), "unit": unit,
-        "frac": round(achieved / peak, 4), "traffic": None, "kernel": name,
+        "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": name,
         "avg_launch_ms": round(avg_ms, 5), "launches_per_step": launches / args.steps,
         "kernel_ms_per_step": {MODELS[k][0].split(" ")[0]: round(v[1] / args.steps, 4)
                                for k, v in per_class.items()},
@@ -112,7 +154,6 @@ def main():
             cpu["note"] = "CPU and GPU proofs differ"
         b.close()
 
-    phases = ctx.phase_times(reset=True)
     line = {
         "metric": "Poseidon2 perms proven/s (2^17 perms, 1 GPU)",
         "value": round(value, 1),
@@ -132,8 +173,10 @@ def main():
                    "proof_words": int(proof.size)},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "phase_ms_per_step": [round(x / (args.steps + args.warmup), 4) for x in phases],
+        "phase_ms_per_step": dict(zip(PHASES, [round(x / args.steps, 4) for x in phases])),
+        "pcie_inclusive": pcie,
     }
+    wit.close()
     table.close()
     ctx.close()
     print(json.dumps(line))
