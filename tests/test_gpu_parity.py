@@ -84,7 +84,8 @@ def test_sumcheck_a2b(gpu_ctx, oracle, n):
     assert np.array_equal(got["final_sum"], want["final_sum"])
 
 
-@pytest.mark.parametrize("n_cols,log_m,log_rows", [(4, 5, 6), (8, 10, 12), (3, 10, 13), (2, 13, 16), (16, 6, 15)])
+@pytest.mark.parametrize("n_cols,log_m,log_rows", [(4, 5, 6), (8, 10, 12), (3, 10, 13), (3, 13, 14), (2, 13, 16),
+                                                   (16, 6, 15), (2, 18, 19), (1, 19, 20)])
 def test_rs_commit(gpu_ctx, oracle, n_cols, log_m, log_rows):
     from whirlaway_amd import utils
     _say(f"rs_commit {n_cols}x2^{log_m} -> 2^{log_rows}")
@@ -100,6 +101,8 @@ def test_rs_commit(gpu_ctx, oracle, n_cols, log_m, log_rows):
         L.or_mobius(oracle.ptr(col), log_m)
         L.or_ntt(oracle.ptr(col), log_rows)
         assert np.array_equal(cw[c], col), c
+    if log_rows > 14:
+        return  # the Python tree walk below is too slow; full proofs cover the tree at these sizes
     # Merkle root from the oracle's sponge/compression
     rows = cw.T.copy()
     level = []

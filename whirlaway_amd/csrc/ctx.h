@@ -87,6 +87,7 @@ struct wl_ctx {
     uint32_t *pinned = nullptr;   // host staging
     size_t pinned_words = 0;
     std::map<int, kb::F *> tw;    // twiddle tables per log size
+    std::map<int, kb::F *> ntt_tab;  // compact two-pass NTT tables per log size
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double phase_ms[8] = {0};
     // live timing of kernel classes (HIP events on this context's stream)
@@ -102,6 +103,18 @@ struct wl_ctx {
         HIP_CHECK(hipMalloc(&t, ((size_t)1 << L) * sizeof(kb::F)));
         wk::twiddles(t, L, stream);
         tw[L] = t;
+        return t;
+    }
+    // compact twiddle tables of the two-pass RS encoding (nullptr when unused)
+    kb::F *ntt_tabs(int L) {
+        int la, lb;
+        if (!wk::ntt_split(L, &la, &lb)) return nullptr;
+        auto it = ntt_tab.find(L);
+        if (it != ntt_tab.end()) return it->second;
+        kb::F *t;
+        HIP_CHECK(hipMalloc(&t, wk::ntt_tables_words(L) * sizeof(kb::F)));
+        wk::ntt_tables(twiddle(L), L, t, stream);
+        ntt_tab[L] = t;
         return t;
     }
     uint32_t *stage(size_t words) {

@@ -158,15 +158,261 @@ __global__ void k_ntt_b(const F *Y, int log_m, int L1, int L2, int T, const F *t
     }
 }
 
-void rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int L, F *cw, F *tmp,
-               const F *tw_full, hipStream_t s) {
+// ------------------------------------------------------- two-pass RS encoding --
+// Columns of length N = 2^L, L = LA + LB, n = NB*nA + nB, k = kA + NA*kB:
+//   pass A: for each nB, Mobius over the evaluation bits of nA, DFT of length NA
+//           along nA -> Y[kA*NB + nB];
+//   pass B: for each kA, Mobius over nB, twiddle w_N^(nB*kA), DFT of length NB
+//           along nB -> X[kA + NA*kB], stored at position kA*NB + kB of the column.
+// Each thread keeps 2^R elements of one row in registers and runs R radix-2
+// stages on them (decimation in time on bit-reversed storage positions); rows
+// move between phases through LDS.  Mobius along natural bits of a later phase
+// commutes with the DIT stages of earlier phases (those act on higher natural
+// bits with coefficients independent of the lower ones), so each phase first
+// applies the Mobius steps of its own bits.
+constexpr int NTT_TILE_A = 16384;  // words of LDS per pass-A block (64 KiB)
+constexpr int NTT_TILE_B = 8192;   // words of LDS per pass-B block
+
+__device__ __forceinline__ uint32_t brevc(uint32_t x, int bits) { return bits ? __brev(x) >> (32 - bits) : 0; }
+
+// Radix-2 DIT stages P0+1 .. P0+R of a length-2^LG transform on a[s] = storage
+// position base + (s << P0); base_lo = base mod 2^P0.  tw[i] = w_{2^LG}^i.
+template <int R, int LG, int P0>
+__device__ __forceinline__ void reg_dit(F *a, uint32_t base_lo, const F *tw) {
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        const int st = P0 + q + 1;
+        const int half = 1 << q;
+#pragma unroll
+        for (int s = 0; s < (1 << R); s++) {
+            if (s & half) continue;
+            const uint32_t klo = (uint32_t)(s & (half - 1)) << P0;
+            F v = a[s + half];
+            if (P0 != 0 || klo != 0) v = fmul(v, tw[(base_lo | klo) << (LG - st)]);
+            const F u = a[s];
+            a[s] = fadd(u, v);
+            a[s + half] = fsub(u, v);
+        }
+    }
+}
+// hi -= lo along every register bit q with bit q of mask set
+template <int R>
+__device__ __forceinline__ void reg_mobius(F *a, uint32_t mask) {
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        if (!((mask >> q) & 1)) continue;
+#pragma unroll
+        for (int s = 0; s < (1 << R); s++)
+            if (s & (1 << q)) a[s] = fsub(a[s], a[s ^ (1 << q)]);
+    }
+}
+
+// Pass A.  Block: T = NTT_TILE_A / NA consecutive nB of one column; grid (NB / T, cols).
+template <int LA>
+__global__ void __launch_bounds__(512) k_ntt2_a(const F *cols, int log_m, int LB, const F *twA, F *Y) {
+    constexpr int NA = 1 << LA, T = NTT_TILE_A / NA;
+    constexpr int R1 = (LA + 1) / 2, R2 = LA - R1;
+    __shared__ F tile[NTT_TILE_A];
+    const size_t NB = (size_t)1 << LB;
+    const size_t m = (size_t)1 << log_m;
+    const F *col = cols + ((size_t)blockIdx.y << log_m);
+    F *y = Y + ((size_t)blockIdx.y << (LA + LB));
+    const size_t nB0 = (size_t)blockIdx.x * T;
+    // phase 1: natural high R1 bits of nA in registers (register s <-> j = brev(s))
+    uint32_t mask1 = 0;
+#pragma unroll
+    for (int b = 0; b < R1; b++)
+        if (LB + R2 + b < log_m) mask1 |= 1u << (R1 - 1 - b);
+    for (int gid = threadIdx.x; gid < (T << R2); gid += blockDim.x) {
+        const int t = gid % T, g = gid / T;
+        F a[1 << R1];
+#pragma unroll
+        for (int s = 0; s < (1 << R1); s++) {
+            const size_t nA = ((size_t)brevc(s, R1) << R2) | g;
+            const size_t n = NB * nA + nB0 + t;
+            a[s] = n < m ? col[n] : F{0};
+        }
+        reg_mobius<R1>(a, mask1);
+        reg_dit<R1, LA, 0>(a, 0, twA);
+        const uint32_t base = brevc(g, R2) << R1;
+        F *row = tile + t * NA;
+#pragma unroll
+        for (int s = 0; s < (1 << R1); s++) row[(base + s) ^ (t & (NA - 1))] = a[s];
+    }
+    __syncthreads();
+    // phase 2: storage bits [R1, LA) in registers, storage low bits c fixed
+    uint32_t mask2 = 0;
+#pragma unroll
+    for (int q = 0; q < R2; q++)
+        if (LB + R2 - 1 - q < log_m) mask2 |= 1u << q;
+    for (int gid = threadIdx.x; gid < (T << R1); gid += blockDim.x) {
+        const int t = gid % T, c = gid / T;
+        const F *row = tile + t * NA;
+        F a[1 << R2];
+#pragma unroll
+        for (int s = 0; s < (1 << R2); s++) a[s] = row[((s << R1) | c) ^ (t & (NA - 1))];
+        reg_mobius<R2>(a, mask2);
+        reg_dit<R2, LA, R1>(a, (uint32_t)c, twA);
+#pragma unroll
+        for (int s = 0; s < (1 << R2); s++) y[(size_t)((s << R1) | c) * NB + nB0 + t] = a[s];
+    }
+}
+
+// Pass B.  Block: TB = NTT_TILE_B / NB rows kA of one column; grid (NA / TB, cols).
+// tl[i] = w_N^i (i < 1024), th[i] = w_N^(1024 i) (i < N / 1024), twB[i] = w_NB^i.
+template <int LB>
+__global__ void __launch_bounds__(256) k_ntt2_b(const F *Y, int log_m, int LA, const F *tl, const F *th,
+                                               const F *twB, F *cw) {
+    constexpr int NB = 1 << LB, TB = NTT_TILE_B / NB;
+    constexpr int RB1 = (LB + 1) / 2, RB2 = LB - RB1;  // high / low natural bits of nB
+    constexpr int LOW = (1 << RB2) - 1;
+    __shared__ F tile[NTT_TILE_B];
+    __shared__ F tws[NB / 2];
+    const int L = LA + LB;
+    const uint32_t Nmask = (1u << L) - 1;
+    const size_t colo = (size_t)blockIdx.y << L;
+    const F *y = Y + colo;
+    F *out = cw + colo;
+    const uint32_t kA0 = blockIdx.x * TB;
+    for (int i = threadIdx.x; i < NB / 2; i += blockDim.x) tws[i] = twB[i];
+    // LDS position of natural index nB in a row (xor swizzle against bank conflicts)
+    auto pos = [](uint32_t nB) { return nB ^ ((nB >> RB2) & LOW); };
+    // B1: natural low RB2 bits in registers, contiguous in global memory
+    uint32_t maskl = 0;
+#pragma unroll
+    for (int q = 0; q < RB2; q++)
+        if (q < log_m) maskl |= 1u << q;
+    for (int gid = threadIdx.x; gid < (TB << RB1); gid += blockDim.x) {
+        const int r = gid >> RB1, h = gid & ((1 << RB1) - 1);
+        const F *src = y + (size_t)(kA0 + r) * NB + ((size_t)h << RB2);
+        F a[1 << RB2];
+#pragma unroll
+        for (int j = 0; j < (1 << RB2); j++) a[j] = src[j];
+        reg_mobius<RB2>(a, maskl);
+        F *row = tile + r * NB;
+#pragma unroll
+        for (int j = 0; j < (1 << RB2); j++) row[pos(((uint32_t)h << RB2) | j)] = a[j];
+    }
+    __syncthreads();
+    // B2: natural high RB1 bits in registers; Mobius, twiddle, DIT stages 1..RB1
+    uint32_t maskh = 0;
+#pragma unroll
+    for (int b = 0; b < RB1; b++)
+        if (RB2 + b < log_m) maskh |= 1u << (RB1 - 1 - b);
+    for (int gid = threadIdx.x; gid < (TB << RB2); gid += blockDim.x) {
+        const int r = gid >> RB2, l = gid & LOW;
+        const uint32_t kA = kA0 + r;
+        F *row = tile + r * NB;
+        F a[1 << RB1];
+#pragma unroll
+        for (int s = 0; s < (1 << RB1); s++) a[s] = row[pos((brevc(s, RB1) << RB2) | l)];
+        reg_mobius<RB1>(a, maskh);
+        // w_N^(kA * nB) for nB = j * 2^RB2 + l: base w^(kA l), step w^(kA 2^RB2)
+        const uint32_t e0 = (kA * (uint32_t)l) & Nmask, e1 = (kA << RB2) & Nmask;
+        const F step = fmul(tl[e1 & 1023], th[e1 >> 10]);
+        F cur = fmul(tl[e0 & 1023], th[e0 >> 10]);
+#pragma unroll
+        for (int j = 0; j < (1 << RB1); j++) {
+            const int s = brevc(j, RB1);
+            a[s] = fmul(a[s], cur);
+            cur = fmul(cur, step);
+        }
+        reg_dit<RB1, LB, 0>(a, 0, tws);
+#pragma unroll
+        for (int s = 0; s < (1 << RB1); s++) row[pos((brevc(s, RB1) << RB2) | l)] = a[s];
+    }
+    __syncthreads();
+    // B3: storage bits [RB1, LB) in registers; DIT stages RB1+1..LB; output k = storage position
+    for (int gid = threadIdx.x; gid < (TB << RB1); gid += blockDim.x) {
+        const int r = gid >> RB1, c = gid & ((1 << RB1) - 1);
+        const F *row = tile + r * NB;
+        const uint32_t hi = brevc(c, RB1) << RB2;
+        F a[1 << RB2];
+#pragma unroll
+        for (int s = 0; s < (1 << RB2); s++) a[s] = row[pos(hi | brevc(s, RB2))];
+        reg_dit<RB2, LB, RB1>(a, (uint32_t)c, tws);
+        F *o = out + (size_t)(kA0 + r) * NB;
+#pragma unroll
+        for (int s = 0; s < (1 << RB2); s++) o[(s << RB1) | c] = a[s];
+    }
+}
+
+// Compact twiddle tables for the two-pass encoding of length 2^L:
+// [twA: w_NA^i, NA/2 | twB: w_NB^i, NB/2 | tl: w_N^i, 1024 | th: w_N^(1024 i), N/1024]
+__global__ void k_ntt_tables(const F *tw_full, int L, int LA, F *out) {
+    const int LB = L - LA;
+    const int na = 1 << (LA - 1), nb = 1 << (LB - 1), nh = 1 << (L - 10);
+    const size_t N = (size_t)1 << L;
+    auto w = [&](size_t e) -> F { return tw_full[e & (N - 1)]; };
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na + nb + 1024 + nh; i += gridDim.x * blockDim.x) {
+        F v;
+        if (i < na) v = w((size_t)i << LB);
+        else if (i < na + nb) v = w((size_t)(i - na) << LA);
+        else if (i < na + nb + 1024) v = w((size_t)(i - na - nb));
+        else v = w((size_t)(i - na - nb - 1024) << 10);
+        out[i] = v;
+    }
+}
+size_t ntt_tables_words(int L) {
+    int LA, LB;
+    ntt_split(L, &LA, &LB);
+    return ((size_t)1 << (LA - 1)) + ((size_t)1 << (LB - 1)) + 1024 + ((size_t)1 << (L - 10));
+}
+void ntt_tables(const F *tw_full, int L, F *out, hipStream_t s) {
+    int LA, LB;
+    ntt_split(L, &LA, &LB);
+    k_ntt_tables<<<8, 256, 0, s>>>(tw_full, L, LA, out);
+}
+bool ntt_split(int L, int *LA, int *LB) {
+    if (L < NTT2_MIN_L || L > NTT2_MAX_L) { *LA = 0; *LB = L; return false; }
+    *LB = L - 4 < 10 ? L - 4 : 10;
+    *LA = L - *LB;
+    return true;
+}
+
+template <int LA>
+static void launch_a(const F *cols, int n, int log_m, int LB, const F *twA, F *Y, hipStream_t s) {
+    constexpr int T = NTT_TILE_A / (1 << LA);
+    k_ntt2_a<LA><<<dim3((unsigned)(((size_t)1 << LB) / T), n), 512, 0, s>>>(cols, log_m, LB, twA, Y);
+}
+template <int LB>
+static void launch_b(const F *Y, int n, int log_m, int LA, const F *tabs, F *cw, hipStream_t s) {
+    constexpr int TB = NTT_TILE_B / (1 << LB);
+    const int L = LA + LB;
+    const F *twB = tabs + ((size_t)1 << (LA - 1));
+    const F *tl = twB + ((size_t)1 << (LB - 1));
+    const F *th = tl + 1024;
+    k_ntt2_b<LB><<<dim3((unsigned)(((size_t)1 << LA) / TB), n), 256, 0, s>>>(Y, log_m, LA, tl, th, twB, cw);
+    (void)L;
+}
+
+int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int L, F *cw, F *tmp,
+              const F *tw_full, const F *tabs, hipStream_t s) {
     size_t rows = (size_t)1 << L;
     if (n_nonzero < n_cols) fill_zero(cw + (size_t)n_nonzero * rows, (size_t)(n_cols - n_nonzero) * rows * 4, s);
-    if (n_nonzero <= 0) return;
+    int LA, LB;
+    const bool two = ntt_split(L, &LA, &LB);
+    if (n_nonzero <= 0) return two ? LA : 0;
     if (L <= 12) {
+        // whole column in LDS: 2^L + 2^(L-1) words (dynamic LDS stays below 48 KiB)
         size_t sh = ((size_t)1 << L) * 4 + ((size_t)1 << L) / 2 * 4 + 16;
         k_ntt_small<<<n_nonzero, 256, sh, s>>>(cols, log_m, L, tw_full, cw);
-        return;
+        return 0;
+    }
+    if (two) {
+        const F *twA = tabs;
+        switch (LA) {
+            case 4: launch_a<4>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
+            case 5: launch_a<5>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
+            case 6: launch_a<6>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
+            case 7: launch_a<7>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
+            case 8: launch_a<8>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
+            case 9: launch_a<9>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
+            default: launch_a<10>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
+        }
+        if (LB == 9) launch_b<9>(tmp, n_nonzero, log_m, LA, tabs, cw, s);
+        else launch_b<10>(tmp, n_nonzero, log_m, LA, tabs, cw, s);
+        return LA;
     }
     int L1 = L / 2, L2 = L - L1;
     // tiles of T transforms: T * (len + 1) + len / 2 words must fit 64 KiB of LDS
@@ -181,13 +427,17 @@ void rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int L, F *cw
     size_t shb = ((size_t)Tb * ((1 << L2) + 1) + (1 << L2) / 2) * 4;
     k_ntt_a<<<dim3((1 << L2) / Ta, n_nonzero), 256, sha, s>>>(cols, log_m, L1, L2, Ta, tw_full, tmp);
     k_ntt_b<<<dim3((1 << L1) / Tb, n_nonzero), 256, shb, s>>>(tmp, log_m, L1, L2, Tb, tw_full, cw);
+    return 0;  // natural order
 }
 
 // ---------------------------------------------------------------- Merkle ---
-__global__ void __launch_bounds__(256) k_merkle_leaves(const F *cw, int width, size_t rows,
-                                                      const P2DevT *pc, F *nodes) {
-    size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
+// Thread = storage position q of the codeword columns (coalesced reads); the row it
+// holds is r = (q >> lb) | ((q mod 2^lb) << la) (natural order when la = 0).
+__global__ void __launch_bounds__(256) k_merkle_leaves(const F *cw, int width, size_t rows, int la,
+                                                      int lb, const P2DevT *pc, F *nodes) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= rows) return;
+    const size_t r = (q >> lb) | ((q & (((size_t)1 << lb) - 1)) << la);
     F s[24];
 #pragma unroll
     for (int i = 0; i < 24; i++) s[i] = F{0};
@@ -196,7 +446,7 @@ __global__ void __launch_bounds__(256) k_merkle_leaves(const F *cw, int width, s
         // compile-time state indices (a runtime index would place the state in scratch)
 #pragma unroll
         for (int j = 0; j < 16; j++)
-            if (j < n) s[j] = cw[(size_t)(off + j) * rows + r];
+            if (j < n) s[j] = cw[(size_t)(off + j) * rows + q];
         p24_permute(s, pc->p24);
     }
     F *o = nodes + (rows + r) * 8;
@@ -288,14 +538,12 @@ __global__ void __launch_bounds__(1024) k_merkle_top(F *nodes, size_t cnt, const
     }
 }
 
-void merkle_leaves(const F *cw, int width, int log_rows, const void *pc, F *nodes, hipStream_t s) {
+void merkle_leaves(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s) {
     size_t rows = (size_t)1 << log_rows;
-    k_merkle_leaves<<<nblk(rows), 256, 0, s>>>(cw, width, rows, (const P2DevT *)pc, nodes);
+    k_merkle_leaves<<<nblk(rows), 256, 0, s>>>(cw, width, rows, la, log_rows - la, (const P2DevT *)pc, nodes);
 }
-void merkle_build(const F *cw, int width, int log_rows, const void *pc, const void *unused, F *nodes,
-                  hipStream_t s) {
-    (void)unused;
-    merkle_leaves(cw, width, log_rows, pc, nodes, s);
+void merkle_build(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s) {
+    merkle_leaves(cw, width, log_rows, la, pc, nodes, s);
     merkle_nodes(log_rows, pc, nodes, s);
 }
 void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s) {
@@ -314,12 +562,14 @@ void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s) {
     if (cnt >= 1) k_merkle_top<<<1, 1024, 0, s>>>(nodes, cnt, p);
 }
 
-__global__ void k_merkle_open(const F *cw, const F *nodes, int width, int log_rows,
+__global__ void k_merkle_open(const F *cw, const F *nodes, int width, int log_rows, int la,
                               const uint32_t *idx, F *rows_out, F *paths_out) {
     int q = blockIdx.x;
     size_t rows = (size_t)1 << log_rows;
     uint32_t r = idx[q];
-    for (int c = threadIdx.x; c < width; c += blockDim.x) rows_out[(size_t)q * width + c] = cw[(size_t)c * rows + r];
+    const int lb = log_rows - la;
+    const size_t at = ((size_t)(r & ((1u << la) - 1)) << lb) | (r >> la);  // storage position of row r
+    for (int c = threadIdx.x; c < width; c += blockDim.x) rows_out[(size_t)q * width + c] = cw[(size_t)c * rows + at];
     for (int t = threadIdx.x; t < log_rows * 8; t += blockDim.x) {
         int l = t / 8, j = t % 8;
         size_t node = ((rows + r) >> l) ^ 1;
@@ -349,9 +599,9 @@ void fold_rows(const F *rows, int nq, int fold, bool base, const EF *eqr, EF *ou
     if (nq > 0) k_fold_rows<<<nq, 128, 0, s>>>(rows, fold, base, eqr, out);
 }
 
-void merkle_open(const F *cw, const F *nodes, int width, int log_rows, const uint32_t *idx, int nq,
+void merkle_open(const F *cw, const F *nodes, int width, int log_rows, int la, const uint32_t *idx, int nq,
                  F *rows_out, F *paths_out, hipStream_t s) {
-    if (nq > 0) k_merkle_open<<<nq, 128, 0, s>>>(cw, nodes, width, log_rows, idx, rows_out, paths_out);
+    if (nq > 0) k_merkle_open<<<nq, 128, 0, s>>>(cw, nodes, width, log_rows, la, idx, rows_out, paths_out);
 }
 
 // ----------------------------------------------------------------- grinding --

@@ -126,19 +126,28 @@ void eq_univ_f_tables(const F *ys, int nY, int n, F *out, hipStream_t s);
 // AoS EF (2^m) -> SoA F columns: out[(b*8 + c)*2^(m-k) + i] = in[b*2^(m-k) + i].c[c]
 void ef_to_cols(const EF *in, int m, int k, F *out, hipStream_t s);
 // RS encode n_cols columns of 2^log_m evaluations (contiguous) into columns of
-// 2^log_rows natural-order NTT values of the univariate embedding.
-void rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int log_rows, F *cw,
-               F *tmp, const F *tw_full, hipStream_t s);
+// 2^log_rows NTT values of the univariate embedding (codeword row r = value at
+// w^r).  Returns la: row r is stored at position ((r mod 2^la) << (log_rows - la))
+// | (r >> la) of every column (la = 0: natural order).  tmp: n_nonzero x
+// 2^log_rows words; tabs: ntt_tables(log_rows) (used when ntt_split holds).
+int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int log_rows, F *cw,
+              F *tmp, const F *tw_full, const F *tabs, hipStream_t s);
+// lengths up to 2^12 are encoded in one block per column, 2^14 .. 2^20 by the
+// two-pass register kernels with L = LA + LB, LB = 10, others by LDS four-step kernels
+constexpr int NTT2_MIN_L = 14, NTT2_MAX_L = 20;
+bool ntt_split(int L, int *LA, int *LB);
+size_t ntt_tables_words(int L);
+void ntt_tables(const F *tw_full, int L, F *out, hipStream_t s);
 // twiddle table w_{2^L}^i for i < 2^L
 void twiddles(F *out, int L, hipStream_t s);
 // Merkle tree over rows of a column-major codeword (width F per leaf), heap layout nodes[2*rows][8]
-void merkle_build(const F *cw, int width, int log_rows, const void *p16, const void *p24, F *nodes,
-                  hipStream_t s);
+// (la: codeword storage order as returned by rs_encode)
+void merkle_build(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s);
 // the two halves of merkle_build: leaf hashes, then the compression levels
-void merkle_leaves(const F *cw, int width, int log_rows, const void *pc, F *nodes, hipStream_t s);
+void merkle_leaves(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s);
 void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s);
 // gather rows (width F each) and sibling paths for sorted query indices
-void merkle_open(const F *cw, const F *nodes, int width, int log_rows, const uint32_t *idx, int nq,
+void merkle_open(const F *cw, const F *nodes, int width, int log_rows, int la, const uint32_t *idx, int nq,
                  F *rows_out, F *paths_out, hipStream_t s);
 // folded query values: out[q] = sum_b eqr[b] * row_q[b], rows of 2^fold F (base)
 // or 2^fold EF (8 F each, coefficient-wise) values

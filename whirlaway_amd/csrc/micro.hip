@@ -209,11 +209,22 @@ int wl_rs_commit(wl_ctx *c, const uint32_t *p16, const uint32_t *p24, const uint
         DBuf<uint32_t> dcols(c->pool, n_cols * m);
         upload_canon(c, dcols, cols, n_cols * m);
         DBuf<F> cw(c->pool, n_cols * rows), tmp(c->pool, n_cols * rows), nodes(c->pool, 2 * rows * 8);
-        wk::rs_encode((const F *)dcols.p, (int)n_cols, (int)n_cols, (int)log_m, (int)log_rows, cw, tmp,
-                      c->twiddle((int)log_rows), c->stream);
-        wk::merkle_build(cw, (int)n_cols, (int)log_rows, pd.p, nullptr, nodes, c->stream);
+        int la = wk::rs_encode((const F *)dcols.p, (int)n_cols, (int)n_cols, (int)log_m, (int)log_rows, cw, tmp,
+                               c->twiddle((int)log_rows), c->ntt_tabs((int)log_rows), c->stream);
+        HIP_CHECK(hipGetLastError());
+        wk::merkle_build(cw, (int)n_cols, (int)log_rows, la, pd.p, nodes, c->stream);
         download_canon(c, root_out, (uint32_t *)nodes.p + 8, 8);
-        if (codeword_out) download_canon(c, codeword_out, (uint32_t *)cw.p, n_cols * rows);
+        if (codeword_out) {
+            // storage order -> natural row order
+            std::vector<uint32_t> st(n_cols * rows);
+            download_canon(c, st.data(), (uint32_t *)cw.p, n_cols * rows);
+            const int lb = (int)log_rows - la;
+            for (size_t col = 0; col < n_cols; col++)
+                for (size_t r = 0; r < rows; r++) {
+                    size_t at = ((r & (((size_t)1 << la) - 1)) << lb) | (r >> la);
+                    codeword_out[col * rows + r] = st[col * rows + at];
+                }
+        }
         return WL_OK;
     });
 }

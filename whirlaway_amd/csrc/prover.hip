@@ -35,6 +35,7 @@ struct Codeword {
     DBuf<F> nodes;   // heap [2 rows][8]
     int width = 0;   // F per leaf
     int log_rows = 0;
+    int la = 0;      // storage order of the rows (rs_encode)
     bool base = true;
     int fold = 0;
 };
@@ -51,12 +52,14 @@ void commit_codeword(wl_ctx *c, const P2Dev *pdev, Codeword &cwd, const F *cols,
     {
         // algorithmic HBM bytes: read the evaluations, write the codeword columns
         KTimer kt(c, KT_NTT, 4.0 * (double)n_nonzero * (double)(rows + ((size_t)1 << log_m)));
-        wk::rs_encode(cols, n_cols, n_nonzero, log_m, log_rows, cwd.cw, tmp, tw, c->stream);
+        cwd.la = wk::rs_encode(cols, n_cols, n_nonzero, log_m, log_rows, cwd.cw, tmp, tw, c->ntt_tabs(log_rows),
+                               c->stream);
+        HIP_CHECK(hipGetLastError());
     }
     {
         // one Poseidon2-24 permutation per 16 leaf elements (PaddingFreeSponge rate 16)
         KTimer kt(c, KT_MERKLE_LEAVES, (double)rows * ((n_cols + 15) / 16));
-        wk::merkle_leaves(cwd.cw, n_cols, log_rows, pdev, cwd.nodes, c->stream);
+        wk::merkle_leaves(cwd.cw, n_cols, log_rows, cwd.la, pdev, cwd.nodes, c->stream);
     }
     {
         KTimer kt(c, KT_MERKLE_NODES, (double)(rows - 1));
@@ -87,7 +90,7 @@ std::vector<EF> open_queries(wl_ctx *c, Transcript &ts, const Codeword &cwd, con
     DBuf<F> buf(c->pool, nrow + npath + nval);
     DBuf<uint32_t> didx(c->pool, nq);
     c->h2d_async(didx.p, q.data(), nq);
-    wk::merkle_open(cwd.cw, cwd.nodes, cwd.width, cwd.log_rows, didx, nq, buf.p, buf.p + nrow, c->stream);
+    wk::merkle_open(cwd.cw, cwd.nodes, cwd.width, cwd.log_rows, cwd.la, didx, nq, buf.p, buf.p + nrow, c->stream);
     DBuf<EF> deq;
     if (eqr) {
         deq = upload_ef(c, *eqr);
@@ -686,6 +689,7 @@ void wl_destroy(wl_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (auto &kv : c->tw) (void)hipFree(kv.second);
+    for (auto &kv : c->ntt_tab) (void)hipFree(kv.second);
     c->pool.release();
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->ring) (void)hipHostFree(c->ring);
