@@ -44,6 +44,110 @@ MODELS = {
 }
 
 
+def _requests(seed, count, lo, hi):
+    """count proof requests with log_n uniform in {lo..hi} and inputs drawn per request
+    (src/examples/poseidon2.rs:104-151 with seed = request seed)."""
+    import numpy as np
+    from whirlaway_amd.air import bench_inputs
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(lo, hi + 1, count)
+    return [(int(n), bench_inputs(10_000 + seed * 1000 + i, int(n))) for i, n in enumerate(sizes)]
+
+
+def _prove_alone(ctx, n, job):
+    import whirlaway_amd as wa
+    from whirlaway_amd.air import generate_witness
+    rc, inputs, p16, p24 = job
+    table = wa.AirTable(ctx, rc, n)
+    w = generate_witness(table, inputs)
+    proof = table.prove(wa.AirSettings(), p16, p24, w)
+    w.close()
+    table.close()
+    return proof
+
+
+def run_batched(args):
+    """64 independent proofs of uneven sizes (log_n ~ U{10..16}): sequential plain
+    single-proof path on one context vs. the engine with several slots.  Both include
+    table setup, input upload, GPU trace generation and the proof."""
+    import numpy as np
+    import whirlaway_amd as wa
+    from whirlaway_amd.engine import Engine
+    reqs = _requests(args.seed, args.batch_jobs, 10, 16)
+    ctx = wa.Context(0)
+    for n, job in reqs:  # warm-up: pools and twiddle tables of every size
+        _prove_alone(ctx, n, job)
+    t0 = time.perf_counter()
+    seq = [_prove_alone(ctx, n, job) for n, job in reqs]
+    t_seq = time.perf_counter() - t0
+    ctx.close()
+    with Engine(0, n_slots=args.slots) as eng:
+        for rnd in range(2):  # round 0 warms the slots' pools
+            t0 = time.perf_counter()
+            tickets = [eng.submit(n, *job) for n, job in reqs]
+            got = [eng.wait(t)[0] for t in tickets]
+            t_eng = time.perf_counter() - t0
+        st = eng.stats()
+    same = all(np.array_equal(a, b) for a, b in zip(seq, got))
+    return {"jobs": len(reqs), "log_n": "U{10..16}", "slots": args.slots,
+            "proofs_per_s": round(len(reqs) / t_eng, 1),
+            "plain_sequential_proofs_per_s": round(len(reqs) / t_seq, 1),
+            "speedup": round(t_seq / t_eng, 3), "peak_concurrent": int(st["peak_running"]),
+            "identical_to_plain": bool(same)}
+
+
+def _replay(eng, reqs, arrivals):
+    """Open-loop replay: submit request i at arrivals[i] seconds after the start."""
+    t0 = time.perf_counter()
+    tickets = []
+    for (n, job), at in zip(reqs, arrivals):
+        while True:
+            dt = t0 + at - time.perf_counter()
+            if dt <= 0:
+                break
+            time.sleep(min(dt, 0.0005) if dt > 0.001 else 0)
+        tickets.append(eng.submit(n, *job))
+    res = [eng.wait(t) for t in tickets]
+    wall = time.perf_counter() - t0
+    return res, wall
+
+
+def run_serving(args):
+    """Poisson arrivals of proof requests (log_n ~ U{10..18}) replayed open-loop against
+    the engine (several slots, continuous admission) and against the plain path (one
+    slot: the single-proof path behind a FIFO).  Latency = submit to proof ready."""
+    import numpy as np
+    from whirlaway_amd.engine import Engine
+    reqs = _requests(args.seed + 1, args.serve_requests, 10, 18)
+    rng = np.random.default_rng(args.seed + 2)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, len(reqs)))
+    arrivals -= arrivals[0]
+    out = {}
+    proofs = {}
+    for name, slots in (("plain", 1), ("engine", args.slots)):
+        with Engine(0, n_slots=slots) as eng:
+            # warm-up: every slot proves each size once (pools, twiddle tables)
+            by_n = {}
+            for n, job in reqs:
+                by_n.setdefault(n, job)
+            for _ in range(slots):
+                ts = [eng.submit(n, *job) for n, job in sorted(by_n.items())]
+                for t in ts:
+                    eng.wait(t)
+            res, wall = _replay(eng, reqs, arrivals)
+        lat = np.array([r[1]["latency_ms"] for r in res])
+        proofs[name] = [r[0] for r in res]
+        out[name] = {"proofs_per_s": round(len(reqs) / wall, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                     "mean_queue_ms": round(float(np.mean([r[1]["queued_ms"] for r in res])), 3)}
+    same = all(np.array_equal(a, b) for a, b in zip(proofs["plain"], proofs["engine"]))
+    e = out["engine"]
+    return {"requests": len(reqs), "rate_per_s": args.serve_rate, "log_n": "U{10..18}",
+            "slots": args.slots, "proofs_per_s": e["proofs_per_s"], "p50_ms": e["p50_ms"],
+            "p99_ms": e["p99_ms"], "mean_queue_ms": e["mean_queue_ms"], "plain": out["plain"],
+            "identical_to_plain": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,6 +159,10 @@ def main():
     ap.add_argument("--cpu-log-n", type=int, default=17)
     ap.add_argument("--pcie-steps", type=int, default=5,
                     help="extra proofs timed from a host-memory witness (PCIe-inclusive rate)")
+    ap.add_argument("--batch-jobs", type=int, default=64, help="batched leg: proofs (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=256, help="serving leg: requests (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=120.0, help="serving leg: Poisson arrivals per second")
+    ap.add_argument("--slots", type=int, default=4, help="engine slots (concurrent proofs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -154,8 +262,11 @@ def main():
             cpu["note"] = "CPU and GPU proofs differ"
         b.close()
 
+    batched = run_batched(args) if args.batch_jobs > 0 else None
+    serving = run_serving(args) if args.serve_requests > 0 else None
+
     line = {
-        "metric": "Poseidon2 perms proven/s (2^17 perms, 1 GPU)",
+        "metric": "Poseidon2 perms proven/s (2^17 perms, 1 GPU); batched proofs/s + p50/p99 latency",
         "value": round(value, 1),
         "unit": "perms/s",
         "n_gpus": 1,
@@ -175,6 +286,8 @@ def main():
         "cpu_baseline": cpu,
         "phase_ms_per_step": dict(zip(PHASES, [round(x / args.steps, 4) for x in phases])),
         "pcie_inclusive": pcie,
+        "batched": batched,
+        "serving": serving,
     }
     wit.close()
     table.close()

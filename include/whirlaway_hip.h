@@ -112,6 +112,45 @@ int wl_kernel_timing(wl_ctx *ctx, uint32_t class_mask);
 int wl_kernel_timing_read(wl_ctx *ctx, int kernel_class, long *launches, double *total_ms,
                           double *work);
 
+/* Device memory of a context's pool: bytes held (live + cached) and the peak of
+ * live bytes since the last reset (no counterpart in the reference). */
+void wl_mem_stats(wl_ctx *ctx, uint64_t *held, uint64_t *peak_live, int reset_peak);
+
+/* ----------------------------------------------------- batched and served --
+ * The reference proves one table at a time (src/examples/poseidon2.rs:153-174);
+ * these entry points have no counterpart there.  An engine runs independent
+ * proof jobs concurrently on one GPU: n_slots worker threads, each with its own
+ * context (stream + pool).  Jobs are admitted in FIFO order when a slot is free
+ * and wl_job_footprint(log_n) fits the memory budget beside the running jobs'
+ * footprints (budget 0: 80% of the device's free memory).  Each job runs the
+ * single-proof path (wl_air_table_new, wl_air_generate_witness,
+ * wl_air_prove_resident), so its proof equals the one proven alone.  Pools keep
+ * freed blocks for reuse, up to about one footprint per slot, outside the budget. */
+typedef struct wl_engine wl_engine;
+typedef struct {
+    uint32_t log_n;            /* 2^log_n permutations (trace rows) */
+    const uint32_t *air_rc;    /* 148 AIR round constants (as wl_air_table_new) */
+    const uint32_t *inputs;    /* 16 x 2^log_n, row-major (as wl_air_generate_witness) */
+    const uint32_t *p16;       /* Poseidon2-16 constants (as wl_air_prove) */
+    const uint32_t *p24;       /* Poseidon2-24 constants */
+    wl_air_settings settings;
+} wl_job;                      /* arrays must stay valid until the job has completed */
+
+uint64_t wl_job_footprint(uint32_t log_n);
+wl_engine *wl_engine_create(int device, int n_slots, uint64_t mem_budget_bytes);
+/* Drops queued jobs, waits for running ones, frees the slots' contexts. */
+void wl_engine_destroy(wl_engine *e);
+int wl_engine_submit(wl_engine *e, const wl_job *job, uint64_t *ticket);
+/* 1 done, 0 pending, -1 unknown ticket */
+int wl_engine_poll(wl_engine *e, uint64_t ticket);
+/* Blocks until the job is done; proof as wl_air_prove; times_ms (may be NULL):
+ * [queued, running, submit-to-done latency].  The ticket is retired unless the
+ * buffer was too small (WL_ERR_BUFFER, *proof_len set). */
+int wl_engine_wait(wl_engine *e, uint64_t ticket, uint32_t *proof_out, size_t proof_cap, size_t *proof_len,
+                   double *times_ms);
+/* [completed jobs, peak concurrent jobs, peak reserved bytes, budget bytes] */
+void wl_engine_stats(wl_engine *e, uint64_t *out4);
+
 /* ---------------------------------------------------------------- kernels --
  * crates/utils/src/multilinear.rs:90-123 fold_multilinear_in_large_field:
  * out[i] = sum_j scalars[j] * in[i + j * 2^(log_n - log_k)], in F (base) or EF,

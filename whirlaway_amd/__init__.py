@@ -60,6 +60,18 @@ def lib():
         L.wl_air_generate_witness.restype = vp
         L.wl_air_generate_witness.argtypes = [vp, u32p]
         L.wl_witness_download.argtypes = [vp, u32p]
+        u64 = ctypes.c_uint64
+        u64p = ctypes.POINTER(u64)
+        L.wl_mem_stats.argtypes = [vp, u64p, u64p, ctypes.c_int]
+        L.wl_job_footprint.restype = u64
+        L.wl_job_footprint.argtypes = [ctypes.c_uint32]
+        L.wl_engine_create.restype = vp
+        L.wl_engine_create.argtypes = [ctypes.c_int, ctypes.c_int, u64]
+        L.wl_engine_destroy.argtypes = [vp]
+        L.wl_engine_submit.argtypes = [vp, vp, u64p]
+        L.wl_engine_poll.argtypes = [vp, u64]
+        L.wl_engine_wait.argtypes = [vp, u64, u32p, ctypes.c_size_t, szp, dp]
+        L.wl_engine_stats.argtypes = [vp, u64p]
         L.wl_kernel_timing.argtypes = [vp, ctypes.c_uint32]
         L.wl_kernel_timing_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), dp, dp]
         _lib = L
@@ -80,4 +92,4 @@ def check(rc):
 
 
 from .air import AirSettings, AirTable, Context  # noqa: E402
-from . import sumcheck, utils  # noqa: E402,F401
+from . import engine, sumcheck, utils  # noqa: E402,F401

@@ -50,6 +50,12 @@ class Context:
         lib().wl_phase_times(self.ptr, out, int(reset))
         return list(out)
 
+    def mem_stats(self, reset_peak=False):
+        """(bytes held by the pool, peak live bytes since the last reset)."""
+        held, peak = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        lib().wl_mem_stats(self.ptr, ctypes.byref(held), ctypes.byref(peak), int(reset_peak))
+        return held.value, peak.value
+
     KERNEL_CLASSES = {1: "merkle_leaves", 2: "zerocheck_rounds", 3: "rs_encode_ntt",
                       4: "zerocheck_round0", 5: "grinding", 6: "merkle_nodes"}
 

@@ -27,6 +27,7 @@ struct Pool {
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> live_;
     size_t bytes_held = 0;
+    size_t live_bytes = 0, peak_live = 0;
     void *get(size_t bytes) {
         bytes = (bytes + 255) & ~(size_t)255;
         if (bytes == 0) bytes = 256;
@@ -40,6 +41,8 @@ struct Pool {
             bytes_held += bytes;
         }
         live_[p] = bytes;
+        live_bytes += bytes;
+        if (live_bytes > peak_live) peak_live = live_bytes;
         return p;
     }
     void put(void *p) {
@@ -47,6 +50,7 @@ struct Pool {
         auto it = live_.find(p);
         if (it == live_.end()) return;
         free_.emplace(it->second, p);
+        live_bytes -= it->second;
         live_.erase(it);
     }
     void release() {
@@ -55,6 +59,7 @@ struct Pool {
         free_.clear();
         live_.clear();
         bytes_held = 0;
+        live_bytes = 0;
     }
 };
 
@@ -220,7 +225,8 @@ struct wl_table {
     uint32_t log_length;
     uint32_t univariate_skips;
     wk::AirConsts air_host;
-    wk::AirConsts *air_dev = nullptr;
+    wk::ChainLin chain_host;
+    wk::AirConsts *air_dev = nullptr;   // from the context's pool
     wk::ChainLin *chain_dev = nullptr;
 };
 
@@ -229,7 +235,7 @@ struct wl_table {
 struct wl_witness {
     wl_ctx *ctx;
     uint32_t log_length;
-    kb::F *packed;
+    kb::F *packed;   // from the context's pool
 };
 
 namespace wl {

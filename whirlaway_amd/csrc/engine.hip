@@ -1,0 +1,259 @@
+// engine.hip -- batched and served proving on one GPU.
+//
+// The reference proves one AirTable at a time on a global rayon pool
+// (src/examples/poseidon2.rs:153-174).  Here an engine owns n_slots worker
+// threads, each with its own context (HIP stream, memory pool, twiddle cache),
+// and a FIFO of jobs.  A job is admitted when a slot is free and its device
+// footprint fits the memory budget beside the footprints of the jobs already
+// running; admitted jobs overlap on the device (separate streams, so one job's
+// host round trips are hidden by the others' kernels).  Every job runs the
+// same code as the single-proof path (wl_air_table_new ->
+// wl_air_generate_witness -> wl_air_prove_resident), so its proof does not
+// depend on what else runs.
+#include <hip/hip_runtime.h>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+#include "../../include/whirlaway_hip.h"
+
+void wl_set_error(const char *msg);
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Job {
+    uint64_t id = 0;
+    wl_job spec{};
+    uint64_t footprint = 0;
+    clk::time_point t_submit, t_start, t_done;
+    bool done = false;
+    int status = WL_OK;
+    std::string err;
+    std::vector<uint32_t> proof;
+};
+
+double ms_between(clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+}  // namespace
+
+struct wl_engine {
+    int device = 0;
+    int n_slots = 0;
+    uint64_t budget = 0;
+    std::vector<wl_ctx *> ctxs;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Job>> queue;
+    std::map<uint64_t, std::shared_ptr<Job>> jobs;
+    uint64_t reserved = 0, peak_reserved = 0;
+    int running = 0, peak_running = 0;
+    uint64_t completed = 0, next_id = 1;
+    bool stop = false;
+
+    void worker(int slot);
+};
+
+namespace {
+
+// Run one job on a context; fills job->proof or job->err.
+void run_job(wl_ctx *ctx, Job &j, std::vector<uint32_t> &buf) {
+    const wl_job &s = j.spec;
+    wl_table *t = wl_air_table_new(ctx, s.air_rc, s.log_n, s.settings.univariate_skips);
+    if (!t) { j.status = WL_ERR_DEVICE; j.err = wl_last_error(); return; }
+    wl_witness *w = wl_air_generate_witness(t, s.inputs);
+    if (!w) {
+        j.status = WL_ERR_DEVICE;
+        j.err = wl_last_error();
+        wl_air_table_free(t);
+        return;
+    }
+    size_t len = 0;
+    int rc = wl_air_prove_resident(t, &s.settings, s.p16, s.p24, w, buf.data(), buf.size(), &len);
+    if (rc == WL_ERR_BUFFER) {
+        buf.resize(len);
+        rc = wl_air_prove_resident(t, &s.settings, s.p16, s.p24, w, buf.data(), buf.size(), &len);
+    }
+    if (rc == WL_OK) j.proof.assign(buf.begin(), buf.begin() + len);
+    else { j.status = rc; j.err = wl_last_error(); }
+    wl_witness_free(w);
+    wl_air_table_free(t);
+}
+
+}  // namespace
+
+void wl_engine::worker(int slot) {
+    (void)hipSetDevice(device);
+    wl_ctx *ctx = ctxs[slot];
+    std::vector<uint32_t> buf((size_t)1 << 18);
+    for (;;) {
+        std::shared_ptr<Job> j;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            // FIFO: the head of the queue runs as soon as its footprint fits
+            cv_work.wait(lk, [&] { return stop || (!queue.empty() && reserved + queue.front()->footprint <= budget); });
+            if (queue.empty() && stop) return;
+            if (stop && !(reserved + queue.front()->footprint <= budget)) return;
+            j = queue.front();
+            queue.pop_front();
+            reserved += j->footprint;
+            if (reserved > peak_reserved) peak_reserved = reserved;
+            running++;
+            if (running > peak_running) peak_running = running;
+            j->t_start = clk::now();
+        }
+        cv_work.notify_all();  // the next head may fit too
+        run_job(ctx, *j, buf);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            j->t_done = clk::now();
+            j->done = true;
+            reserved -= j->footprint;
+            running--;
+            completed++;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+}
+
+extern "C" {
+
+uint64_t wl_job_footprint(uint32_t log_n) {
+    // Device bytes a proof of 2^log_n rows holds at its peak, from the buffers of
+    // the prover (packed witness 2^(n+10) B, first codeword + encoding scratch +
+    // Merkle tree ~2^(n+11.6) B, zerocheck columns 2^(n+8.4) B, WHIR rounds), with
+    // margin; checked against measured pool peaks by the test suite.
+    return ((uint64_t)6144 << log_n) + ((uint64_t)64 << 20);
+}
+
+wl_engine *wl_engine_create(int device, int n_slots, uint64_t mem_budget_bytes) {
+    if (n_slots < 1 || n_slots > 16) { wl_set_error("n_slots must be in [1, 16]"); return nullptr; }
+    auto *e = new wl_engine();
+    e->device = device;
+    e->n_slots = n_slots;
+    if (mem_budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            wl_set_error("cannot query device memory");
+            delete e;
+            return nullptr;
+        }
+        mem_budget_bytes = (uint64_t)(fr * 0.8);
+    }
+    e->budget = mem_budget_bytes;
+    for (int i = 0; i < n_slots; i++) {
+        wl_ctx *c = wl_create(device);
+        if (!c) {
+            for (wl_ctx *x : e->ctxs) wl_destroy(x);
+            delete e;
+            return nullptr;
+        }
+        e->ctxs.push_back(c);
+    }
+    for (int i = 0; i < n_slots; i++) e->workers.emplace_back([e, i] { e->worker(i); });
+    return e;
+}
+
+void wl_engine_destroy(wl_engine *e) {
+    if (!e) return;
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        e->stop = true;
+        e->queue.clear();  // jobs not yet admitted are dropped
+    }
+    e->cv_work.notify_all();
+    for (auto &t : e->workers) t.join();
+    for (wl_ctx *c : e->ctxs) wl_destroy(c);
+    delete e;
+}
+
+int wl_engine_submit(wl_engine *e, const wl_job *job, uint64_t *ticket) {
+    if (!e || !job || !ticket || !job->air_rc || !job->inputs || !job->p16 || !job->p24) {
+        wl_set_error("null argument");
+        return WL_ERR_ARG;
+    }
+    if (job->log_n < 2 || job->log_n > 24 || job->settings.univariate_skips >= job->log_n) {
+        wl_set_error("unsupported log_n");
+        return WL_ERR_ARG;
+    }
+    auto j = std::make_shared<Job>();
+    j->spec = *job;
+    j->footprint = wl_job_footprint(job->log_n);
+    if (j->footprint > e->budget) {
+        wl_set_error("job footprint exceeds the engine's memory budget");
+        return WL_ERR_ARG;
+    }
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        if (e->stop) { wl_set_error("engine stopped"); return WL_ERR_ARG; }
+        j->id = e->next_id++;
+        j->t_submit = clk::now();
+        e->jobs[j->id] = j;
+        e->queue.push_back(j);
+    }
+    e->cv_work.notify_all();
+    *ticket = j->id;
+    return WL_OK;
+}
+
+int wl_engine_poll(wl_engine *e, uint64_t ticket) {
+    if (!e) return -1;
+    std::lock_guard<std::mutex> lk(e->mu);
+    auto it = e->jobs.find(ticket);
+    if (it == e->jobs.end()) return -1;
+    return it->second->done ? 1 : 0;
+}
+
+int wl_engine_wait(wl_engine *e, uint64_t ticket, uint32_t *proof_out, size_t proof_cap, size_t *proof_len,
+                   double *times_ms) {
+    if (!e) { wl_set_error("null engine"); return WL_ERR_ARG; }
+    std::shared_ptr<Job> j;
+    {
+        std::unique_lock<std::mutex> lk(e->mu);
+        auto it = e->jobs.find(ticket);
+        if (it == e->jobs.end()) { wl_set_error("unknown ticket"); return WL_ERR_ARG; }
+        j = it->second;
+        e->cv_done.wait(lk, [&] { return j->done; });
+    }
+    if (proof_len) *proof_len = j->proof.size();
+    if (times_ms) {
+        times_ms[0] = ms_between(j->t_submit, j->t_start);  // queued
+        times_ms[1] = ms_between(j->t_start, j->t_done);    // running
+        times_ms[2] = ms_between(j->t_submit, j->t_done);   // latency
+    }
+    if (j->status != WL_OK) {
+        wl_set_error(j->err.c_str());
+        std::lock_guard<std::mutex> lk(e->mu);
+        e->jobs.erase(ticket);
+        return j->status;
+    }
+    if (proof_out) {
+        if (proof_cap < j->proof.size()) { wl_set_error("proof buffer too small"); return WL_ERR_BUFFER; }
+        memcpy(proof_out, j->proof.data(), j->proof.size() * 4);
+    }
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->jobs.erase(ticket);
+    return WL_OK;
+}
+
+void wl_engine_stats(wl_engine *e, uint64_t *out4) {
+    if (!e || !out4) return;
+    std::lock_guard<std::mutex> lk(e->mu);
+    out4[0] = e->completed;
+    out4[1] = (uint64_t)e->peak_running;
+    out4[2] = e->peak_reserved;
+    out4[3] = e->budget;
+}
+
+}  // extern "C"

@@ -132,7 +132,7 @@ wl_witness *wl_air_generate_witness(wl_table *t, const uint32_t *inputs) {
         wl_witness *w = new wl_witness();
         w->ctx = c;
         w->log_length = t->log_length;
-        HIP_CHECK(hipMalloc(&w->packed, V * 4));
+        w->packed = (kb::F *)c->pool.get(V * 4);
         wk::fill_zero(w->packed + 165 * N, (V - 165 * N) * 4, c->stream);
         wl::DBuf<uint32_t> din(c->pool, N * 16);
         HIP_CHECK(hipMemcpyAsync(din.p, inputs, N * 16 * 4, hipMemcpyHostToDevice, c->stream));

@@ -717,13 +717,17 @@ wl_table *wl_air_table_new(wl_ctx *c, const uint32_t *rc, uint32_t log_length, u
         t->ctx = c;
         t->log_length = log_length;
         t->univariate_skips = univariate_skips;
+        HIP_CHECK(hipSetDevice(c->device));
         for (int i = 0; i < wk::AIR_NRC; i++) t->air_host.rc[i] = to_mont(rc[i] % P);
-        HIP_CHECK(hipMalloc(&t->air_dev, sizeof(wk::AirConsts)));
-        HIP_CHECK(hipMemcpy(t->air_dev, &t->air_host, sizeof(wk::AirConsts), hipMemcpyHostToDevice));
-        wk::ChainLin cl;
-        chain_linear_maps(cl);
-        HIP_CHECK(hipMalloc(&t->chain_dev, sizeof(wk::ChainLin)));
-        HIP_CHECK(hipMemcpy(t->chain_dev, &cl, sizeof(wk::ChainLin), hipMemcpyHostToDevice));
+        chain_linear_maps(t->chain_host);
+        // pool memory and stream-ordered copies: no device-wide synchronization, so
+        // contexts on other host threads keep running
+        t->air_dev = (wk::AirConsts *)c->pool.get(sizeof(wk::AirConsts));
+        t->chain_dev = (wk::ChainLin *)c->pool.get(sizeof(wk::ChainLin));
+        HIP_CHECK(hipMemcpyAsync(t->air_dev, &t->air_host, sizeof(wk::AirConsts), hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemcpyAsync(t->chain_dev, &t->chain_host, sizeof(wk::ChainLin), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
         return t;
     } catch (const std::exception &e) {
         g_err = e.what();
@@ -733,8 +737,9 @@ wl_table *wl_air_table_new(wl_ctx *c, const uint32_t *rc, uint32_t log_length, u
 
 void wl_air_table_free(wl_table *t) {
     if (!t) return;
-    (void)hipFree(t->air_dev);
-    (void)hipFree(t->chain_dev);
+    (void)hipStreamSynchronize(t->ctx->stream);
+    t->ctx->pool.put(t->air_dev);
+    t->ctx->pool.put(t->chain_dev);
     delete t;
 }
 
@@ -794,7 +799,7 @@ wl_witness *wl_witness_upload(wl_ctx *c, const uint32_t *witness, uint32_t log_l
         w->ctx = c;
         w->log_length = log_length;
         size_t N = (size_t)1 << log_length, V = N << 8;
-        HIP_CHECK(hipMalloc(&w->packed, V * 4));
+        w->packed = (F *)c->pool.get(V * 4);
         wk::fill_zero(w->packed + AIR_NCOLS * N, (V - AIR_NCOLS * N) * 4, c->stream);
         HIP_CHECK(hipMemcpyAsync(w->packed, witness, AIR_NCOLS * N * 4, hipMemcpyHostToDevice, c->stream));
         wk::convert_to_mont((uint32_t *)w->packed, AIR_NCOLS * N, c->stream);
@@ -808,8 +813,16 @@ wl_witness *wl_witness_upload(wl_ctx *c, const uint32_t *witness, uint32_t log_l
 
 void wl_witness_free(wl_witness *w) {
     if (!w) return;
-    (void)hipFree(w->packed);
+    (void)hipStreamSynchronize(w->ctx->stream);
+    w->ctx->pool.put(w->packed);
     delete w;
+}
+
+void wl_mem_stats(wl_ctx *c, uint64_t *held, uint64_t *peak_live, int reset_peak) {
+    if (!c) return;
+    if (held) *held = c->pool.bytes_held;
+    if (peak_live) *peak_live = c->pool.peak_live;
+    if (reset_peak) c->pool.peak_live = c->pool.live_bytes;
 }
 
 int wl_air_prove_resident(wl_table *t, const wl_air_settings *st, const uint32_t *p16c,
