@@ -132,9 +132,10 @@ extern "C" {
 uint64_t wl_job_footprint(uint32_t log_n) {
     // Device bytes a proof of 2^log_n rows holds at its peak, from the buffers of
     // the prover (packed witness 2^(n+10) B, first codeword + encoding scratch +
-    // Merkle tree ~2^(n+11.6) B, zerocheck columns 2^(n+8.4) B, WHIR rounds), with
-    // margin; checked against measured pool peaks by the test suite.
-    return ((uint64_t)6144 << log_n) + ((uint64_t)64 << 20);
+    // Merkle tree ~2^(n+11.6) B, zerocheck round-0 rows 2^(n+10.3) B and columns
+    // 2^(n+8.4) B, WHIR rounds), with margin; checked against measured pool peaks
+    // by the test suite.
+    return ((uint64_t)8192 << log_n) + ((uint64_t)64 << 20);
 }
 
 wl_engine *wl_engine_create(int device, int n_slots, uint64_t mem_budget_bytes) {

@@ -98,75 +98,49 @@ void reduce_publish(const EF *in, int n_items, int n_parts, EF *host_dst, volati
 }
 
 // --------------------------------------------------------------- eq table --
-// Builds T_k over (p_k..p_{n-1}) from T_{k+1}: out[S + j] = p_k T[j], out[j] = T[j] - out[S + j].
-__global__ void k_eq_small(const EF *pt, int n, const EF *scale, EF *out, int levels) {
-    // one block: first `levels` doublings (variables n-1 .. n-levels)
-    if (threadIdx.x == 0) out[0] = *scale;
+// eq(point, .) * scale in one launch.  Block b covers entries b * 2^lo + t
+// (lo = min(n, 8)): the table over the lo low variables is built by doublings in
+// LDS while a product tree over 32 slots forms the block's factor from the high
+// variables (bits of b); entry = factor * low[t].  Latency ~max(lo, 5) EF products.
+__global__ void __launch_bounds__(256) k_eq_one(EqPoint pt, int lo, EF *out) {
+    __shared__ EF ta[256], tb[256], ha[32], hb[32];
+    const int t = threadIdx.x, hi = pt.n - lo;
+    const uint32_t b = blockIdx.x;
+    if (t < 32) {
+        EF f = eone();
+        if (t < hi) {
+            const EF p = pt.p[t];
+            f = ((b >> (hi - 1 - t)) & 1) ? p : esub(eone(), p);
+        }
+        if (t == 0) f = emul(f, pt.scale);
+        ha[t] = f;
+    }
+    if (t == 0) ta[0] = eone();
     __syncthreads();
-    size_t S = 1;
-    for (int t = 0; t < levels; t++) {
-        EF p = pt[n - 1 - t];
-        for (size_t j = threadIdx.x; j < S; j += blockDim.x) {
-            EF v = out[j];
-            EF hi = emul(v, p);
-            out[S + j] = hi;
-            out[j] = esub(v, hi);
+    EF *src = ta, *dst = tb, *hs = ha, *hd = hb;
+    const int levels = lo > 5 ? lo : 5;
+    for (int l = 0; l < levels; l++) {
+        if (l < lo && t < (1 << l)) {
+            const EF p = pt.p[hi + l];
+            const EF v = src[t];
+            const EF h = emul(v, p);
+            dst[2 * t] = esub(v, h);
+            dst[2 * t + 1] = h;
+        }
+        if (l < 5 && t >= 128 && t < 128 + (16 >> l)) {
+            const int i = t - 128;
+            hd[i] = emul(hs[2 * i], hs[2 * i + 1]);
         }
         __syncthreads();
-        S <<= 1;
+        if (l < lo) { EF *x = src; src = dst; dst = x; }
+        if (l < 5) { EF *x = hs; hs = hd; hd = x; }
     }
-}
-__global__ void k_eq_level(const EF *pt, int k, size_t S, EF *out) {
-    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= S) return;
-    EF p = pt[k];
-    EF v = out[j];
-    EF hi = emul(v, p);
-    out[S + j] = hi;
-    out[j] = esub(v, hi);
-}
-void eq_table(const EF *pt, int n, const EF *scale, EF *out, hipStream_t s) {
-    int first = n < 10 ? n : 10;
-    k_eq_small<<<1, 512, 0, s>>>(pt, n, scale, out, first);
-    size_t S = (size_t)1 << first;
-    for (int t = first; t < n; t++) {
-        k_eq_level<<<nblk(S), 256, 0, s>>>(pt, n - 1 - t, S, out);
-        S <<= 1;
-    }
-}
-
-// Entry x of eq(p, .) directly: scale * prod_k (bit ? p_k : 1 - p_k); used for
-// the first levels (up to 2^10 entries, one thread each), then doublings.
-__global__ void k_eq_direct(EqPoint pt, int levels, EF *out) {
-    size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= ((size_t)1 << levels)) return;
-    EF acc = pt.scale;
-    // the first `levels` doublings cover variables n-levels .. n-1 (big-endian: the
-    // last variable is the least significant bit)
-    for (int t = 0; t < levels; t++) {
-        const EF &p = pt.p[pt.n - levels + t];
-        bool bit = (x >> (levels - 1 - t)) & 1;
-        acc = emul(acc, bit ? p : esub(eone(), p));
-    }
-    out[x] = acc;
-}
-__global__ void k_eq_level_v(EF p, size_t S, EF *out) {
-    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= S) return;
-    EF v = out[j];
-    EF hi = emul(v, p);
-    out[S + j] = hi;
-    out[j] = esub(v, hi);
+    if (t < (1 << lo)) out[((size_t)b << lo) + t] = emul(hs[0], src[t]);
 }
 void eq_table_v(const EqPoint &pt, EF *out, hipStream_t s) {
-    int n = pt.n;
-    int first = n < 10 ? n : 10;
-    k_eq_direct<<<nblk((size_t)1 << first), 256, 0, s>>>(pt, first, out);
-    size_t S = (size_t)1 << first;
-    for (int t = first; t < n; t++) {
-        k_eq_level_v<<<nblk(S), 256, 0, s>>>(pt.p[n - 1 - t], S, out);
-        S <<= 1;
-    }
+    const int n = pt.n;
+    const int lo = n < 8 ? n : 8;
+    k_eq_one<<<1u << (n - lo), 256, 0, s>>>(pt, lo, out);
 }
 
 __global__ void k_eq_halve(EF *E, size_t h) {
@@ -274,6 +248,77 @@ int zc_round0(const F *trace, int log_n, int skips, const F *sel, int nz, const 
     size_t M = (size_t)1 << (log_n - skips);
     unsigned nb = nblk(M);
     k_zc_round0<<<dim3(nb, nz), 256, 0, s>>>(trace, log_n, skips, sel, air, ap, eq, parts);
+    return (int)nb;
+}
+
+// Round 0 in two passes.  Pass 1: for each column c and x, the 2^skips values
+// up_c[j*M + x] are those of a polynomial of degree < K = 2^skips in the skipped
+// variable at j = 0..K-1; its values at z = K, K+1, ... follow from the
+// backward-difference table at j = K-1 (a_k = nabla^k f(K-1); one step:
+// a_k += a_{k+1} for k = K-2..0, then a_0 = f(next)) -- additions only.
+// U0[zi][c][x] = f_c,x(K + zi), zi < nz.
+template <int K>
+__global__ void __launch_bounds__(256) k_zc_lde(const F *trace, int log_n, int nz, F *U0) {
+    const size_t N = (size_t)1 << log_n;
+    const size_t M = N / K;
+    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (x >= M) return;
+    const F *col = trace + (size_t)c * N;
+    F v[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        size_t idx = (size_t)j * M + x;
+        if (idx == N - 1) idx = N - 2;  // column_up: the last row repeats the one before
+        v[j] = col[idx];
+    }
+    // forward differences: after pass k, v[i] = Delta^k f(i) for i <= K-1-k, and
+    // v[K-1-k] = Delta^k f(K-1-k) = nabla^k f(K-1) stays untouched afterwards
+#pragma unroll
+    for (int k = 1; k < K; k++)
+#pragma unroll
+        for (int i = 0; i < K - k; i++) v[i] = fsub(v[i + 1], v[i]);
+    F a[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) a[k] = v[K - 1 - k];
+    F *out = U0 + (size_t)c * M + x;
+    const size_t zstride = (size_t)AIR_NCOLS * M;
+    for (int zi = 0; zi < nz; zi++) {
+#pragma unroll
+        for (int k = K - 2; k >= 0; k--) a[k] = fadd(a[k], a[k + 1]);
+        out[(size_t)zi * zstride] = a[0];
+    }
+}
+
+// Pass 2: one lane per (x, z): AIR on the extended row, times eq[x], block sums.
+__global__ void __launch_bounds__(256) k_zc_air0(const F *U0, size_t M, const AirConsts *air,
+                                                const EF *ap, const EF *eq, EF *parts) {
+    const int zi = blockIdx.y;
+    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    EF val = ezero();
+    if (x < M) {
+        const F *row = U0 + (size_t)zi * AIR_NCOLS * M + x;
+        auto get = [&](int c) -> F { return row[(size_t)c * M]; };
+        val = emul(air_eval_f(get, air, ap), eq[x]);
+    }
+    __shared__ EF lds[16];
+    EF t = block_sum(val, lds);
+    if (threadIdx.x == 0) parts[(size_t)zi * gridDim.x + blockIdx.x] = t;
+}
+
+int zc_round0_lde(const F *trace, int log_n, int skips, int nz, const AirConsts *air, const EF *ap,
+                  const EF *eq, F *U0, EF *parts, hipStream_t s) {
+    const size_t M = (size_t)1 << (log_n - skips);
+    const dim3 g1(nblk(M), AIR_NCOLS);
+    switch (skips) {
+        case 1: k_zc_lde<2><<<g1, 256, 0, s>>>(trace, log_n, nz, U0); break;
+        case 2: k_zc_lde<4><<<g1, 256, 0, s>>>(trace, log_n, nz, U0); break;
+        case 3: k_zc_lde<8><<<g1, 256, 0, s>>>(trace, log_n, nz, U0); break;
+        case 4: k_zc_lde<16><<<g1, 256, 0, s>>>(trace, log_n, nz, U0); break;
+        default: return -1;
+    }
+    const unsigned nb = nblk(M);
+    k_zc_air0<<<dim3(nb, nz), 256, 0, s>>>(U0, M, air, ap, eq, parts);
     return (int)nb;
 }
 

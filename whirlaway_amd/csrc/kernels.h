@@ -37,9 +37,7 @@ void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_
 void reduce_publish(const EF *in, int n_items, int n_parts, EF *host_dst, volatile uint32_t *flag,
                     uint32_t seq, hipStream_t s);
 
-// eq(point, .) over n variables, big-endian, times scale; point on device.
-void eq_table(const EF *d_point, int n, const EF *d_scale, EF *out, hipStream_t s);
-// Same with the point passed by value (n <= 32).
+// eq(point, .) over n <= 32 variables, big-endian, times scale (point by value).
 struct EqPoint { EF p[32]; EF scale; int n; };
 void eq_table_v(const EqPoint &pt, EF *out, hipStream_t s);
 // E[x] += E[x + 2^(l-1)] for x < 2^(l-1): eq(r[1..], .) from eq(r, .) since the
@@ -54,6 +52,11 @@ struct State16 { F s[16]; };
 // parts: [nz][n_blocks] EF; returns n_blocks.
 int zc_round0(const F *trace, int log_n, int skips, const F *sel, int nz, const AirConsts *air,
               const EF *alpha_pows, const EF *eq, EF *parts, hipStream_t s);
+// The same sums for z = 2^skips .. 2^skips + nz - 1 in two passes (skips <= 4):
+// the extended rows U0 (nz x 165 x M F scratch) by finite differences, then the
+// AIR per (x, z).  Returns n_blocks, or -1 when skips is not supported.
+int zc_round0_lde(const F *trace, int log_n, int skips, int nz, const AirConsts *air, const EF *alpha_pows,
+                  const EF *eq, F *U0, EF *parts, hipStream_t s);
 // U[c][x] = sum_j S_j * up_c[j*M + x]  (S: 2^skips EF on device)
 void zc_fold0(const F *trace, int log_n, int skips, const EF *S, EF *U, hipStream_t s);
 // Rounds >= 1: U holds 165 EF columns (column c at U + c * stride) of current

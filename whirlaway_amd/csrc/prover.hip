@@ -495,7 +495,13 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
         DBuf<EF> parts(c->pool, (size_t)nzp * nbx);
         {
             KTimer kt(c, KT_ZC_ROUND0, (double)nzp * M);
-            wk::zc_round0(packed, n, skips, dsel, nzp, t->air_dev, dap, eq, parts, c->stream);
+            if (skips <= 4) {
+                // consecutive z = K .. zmax: rows extended by finite differences
+                DBuf<F> U0(c->pool, (size_t)nzp * AIR_NCOLS * M);
+                wk::zc_round0_lde(packed, n, skips, nzp, t->air_dev, dap, eq, U0, parts, c->stream);
+            } else {
+                wk::zc_round0(packed, n, skips, dsel, nzp, t->air_dev, dap, eq, parts, c->stream);
+            }
         }
         std::vector<EF> vals = reduce_many(c, parts, nzp, (int)nbx);
         std::vector<F> xs;
