@@ -464,7 +464,9 @@ __global__ void k_seg_dot(const void *Av, const EF *E, size_t len, EF *parts) {
     if (threadIdx.x == 0) parts[(size_t)j * gridDim.x + blockIdx.x] = t;
 }
 int dot_blocks(size_t len) {
-    unsigned nb = nblk(len, 256 * 4);
+    // one element per thread up to 1024 blocks (EF products are latency-bound per
+    // thread; wider grids hide it), grid-stride beyond
+    unsigned nb = nblk(len, 256);
     if (nb > 1024) nb = 1024;
     if (nb == 0) nb = 1;
     return (int)nb;
@@ -501,6 +503,42 @@ int prod_round(const EF *A, const EF *B, int l, EF *parts, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
     unsigned nb = dot_blocks(h);
     k_prod_round<<<nb, 256, 0, s>>>(A, B, h, parts);
+    return (int)nb;
+}
+// One round of the product sumcheck on (A, B) of length 2h, fused with the fold
+// by the previous challenge c when `fold` is set (the tables then hold 4h
+// entries and are folded in place to 2h: thread x owns x, x+h, x+2h, x+3h), so
+// each table is read once per round.  parts as k_prod_round.
+__global__ void __launch_bounds__(256) k_prod_round_f(EF *A, EF *B, size_t h, int fold, EF c, EF *parts) {
+    EF a0 = ezero(), a2 = ezero();
+    const size_t st = (size_t)gridDim.x * blockDim.x;
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < h; x += st) {
+        EF al = A[x], ah = A[x + h], bl = B[x], bh = B[x + h];
+        if (fold) {
+            al = eadd(al, emul(c, esub(A[x + 2 * h], al)));
+            ah = eadd(ah, emul(c, esub(A[x + 3 * h], ah)));
+            bl = eadd(bl, emul(c, esub(B[x + 2 * h], bl)));
+            bh = eadd(bh, emul(c, esub(B[x + 3 * h], bh)));
+            A[x] = al;
+            A[x + h] = ah;
+            B[x] = bl;
+            B[x + h] = bh;
+        }
+        a0 = eadd(a0, emul(al, bl));
+        a2 = eadd(a2, emul(esub(edbl(ah), al), esub(edbl(bh), bl)));
+    }
+    __shared__ EF lds[16];
+    EF t0 = block_sum(a0, lds);
+    EF t2 = block_sum(a2, lds);
+    if (threadIdx.x == 0) {
+        parts[blockIdx.x] = t0;
+        parts[gridDim.x + blockIdx.x] = t2;
+    }
+}
+int prod_round_fused(EF *A, EF *B, int l, bool fold, EF c, EF *parts, hipStream_t s) {
+    size_t h = (size_t)1 << (l - 1);
+    unsigned nb = dot_blocks(h);
+    k_prod_round_f<<<nb, 256, 0, s>>>(A, B, h, fold ? 1 : 0, c, parts);
     return (int)nb;
 }
 __global__ void k_fold_ef(EF *A, size_t h, const EF *c) {

@@ -95,6 +95,9 @@ int whir_T_chunks(int m, int k);
 // ------------------------------------------------------ product sumcheck --
 // parts[0][b] = sum_{x<h} A[x] B[x], parts[1][b] = sum (2A[x+h]-A[x])(2B[x+h]-B[x]); h = 2^(l-1)
 int prod_round(const EF *A, const EF *B, int l, EF *parts, hipStream_t s);
+// The same round on tables of length 2^l, first folding them in place from
+// 2^(l+1) by c when fold is set.
+int prod_round_fused(EF *A, EF *B, int l, bool fold, EF c, EF *parts, hipStream_t s);
 // in-place fold of one EF table by (1-c, c)
 void fold_ef(EF *A, int l, const EF *c, hipStream_t s);
 // fold one or two EF tables by a by-value challenge (B may be nullptr)

@@ -196,22 +196,33 @@ void add_constraints_w(wl_ctx *c, EF *W, int m, const std::vector<Constraint> &c
 
 struct SumcheckState {
     DBuf<EF> f, W;
-    int m = 0;
+    int m = 0;       // tables hold 2^m entries (2^(m+1) while a fold is pending)
     EF sum;
+    bool pending = false;  // fold by pc not yet applied (the next round kernel applies it)
+    EF pc;
 };
 
+void flush_fold(wl_ctx *c, SumcheckState &s) {
+    if (!s.pending) return;
+    wk::fold_ef2_v(s.f, s.W, s.m + 1, s.pc, c->stream);
+    s.pending = false;
+}
+
 // GPU sumcheck round pair (f, W), evaluations at 0,1,2 sent (whir-p3 convention).
+// The fold by the sampled challenge is left pending for the next round's kernel.
 EF whir_sc_round(wl_ctx *c, Transcript &ts, const P2Dev *pdev, SumcheckState &s, int pow_bits) {
     int nb = wk::dot_blocks((size_t)1 << (s.m - 1));
     DBuf<EF> parts(c->pool, 2 * (size_t)nb);
-    wk::prod_round(s.f, s.W, s.m, parts, c->stream);
+    wk::prod_round_fused(s.f, s.W, s.m, s.pending, s.pc, parts, c->stream);
     std::vector<EF> h02 = reduce_many(c, parts, 2, nb);
+    s.pending = false;
     EF h[3] = {h02[0], esub(s.sum, h02[0]), h02[1]};
     ts.add_ef(h, 3);
     grind(c, ts, pdev, pow_bits);
     EF r = ts.sample_ef();
     s.sum = wh::interp3(h, r);
-    wk::fold_ef2_v(s.f, s.W, s.m, r, c->stream);
+    s.pending = true;
+    s.pc = r;
     s.m--;
     return r;
 }
@@ -417,6 +428,7 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
         for (auto &k : fresh) cs.push_back(k);
         rf.clear();
         for (int k = 0; k < kn; k++) rf.push_back(whir_sc_round(c, ts, pdev, s, rp.folding_pow_bits));
+        flush_fold(c, s);
         for (auto &k : cs) fold_constraint(k, m, rf);
         prev = &nc;
         cur ^= 1;
@@ -615,11 +627,13 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     // ---- inner sumcheck (degree 2, coefficients sent) ----
     std::vector<EF> ich;
     EF isum = inner_sum;
+    bool pend = false;  // fold by the previous challenge, applied inside the next round kernel
+    EF pc = ezero();
     for (int j = 0; j < n; j++) {
         int ll = n - j;
         int nb = wk::dot_blocks((size_t)1 << (ll - 1));
         DBuf<EF> parts(c->pool, 2 * (size_t)nb);
-        wk::prod_round(Mt, B, ll, parts, c->stream);
+        wk::prod_round_fused(Mt, B, ll, pend, pc, parts, c->stream);
         std::vector<EF> h02 = reduce_many(c, parts, 2, nb);
         std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2)};
         std::vector<EF> p = wh::lagrange(xs, {h02[0], esub(isum, h02[0]), h02[1]});
@@ -628,8 +642,10 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
         ich.push_back(cj);
         isum = wh::peval(p, cj);
         grind(c, ts, pdev, pow_bits_sumcheck(sec, 2));
-        wk::fold_ef2_v(Mt, B, ll, cj, c->stream);
+        pend = true;
+        pc = cj;
     }
+    wk::fold_ef2_v(Mt, B, 1, pc, c->stream);
     EF packed_value;
     c->d2h(&packed_value, B.p, 1);
     Mt.reset();
