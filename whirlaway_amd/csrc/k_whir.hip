@@ -471,69 +471,82 @@ __global__ void __launch_bounds__(256) k_merkle_level(F *nodes, size_t cnt, cons
     compress_node(nodes, cnt + i, pc);
 }
 
-// Poseidon2-16 with the state spread over a lane quad: lane q holds elements
-// [4q, 4q + 4), one M4 block, so the external layer needs two xor-shuffle
-// steps for the cross-block sums and the internal layer one quad-wide sum.
-// Used where a tree level has too few nodes to fill the GPU: it cuts the
-// dependent instruction chain of one compression by about 2.5x.
-__device__ __forceinline__ F quad_sum(F x) {
-    x = fadd(x, F{(uint32_t)__shfl_xor((int)x.v, 1, 64)});
-    x = fadd(x, F{(uint32_t)__shfl_xor((int)x.v, 2, 64)});
-    return x;
+// Poseidon2-16 spread over one 16-lane DPP row: lane i of the row holds s[i].
+// The external layer needs the quad sum and the next lane of the quad (M4 row
+// r is s + x_r + 2 x_(r+1 mod 4)) and the sum over the row's four quads of each
+// residue class (two row rotations); the internal layer needs the row sum
+// (quad sums, then half mirror and mirror).  Cuts the dependent chain of one
+// compression about 3x against one lane per permutation; used for the tree
+// levels that have too few nodes to fill the GPU.
+template <int CTRL>
+__device__ __forceinline__ F dpp(F x) {
+    return F{(uint32_t)__builtin_amdgcn_update_dpp(0, (int)x.v, CTRL, 0xF, 0xF, true)};
 }
-__device__ __forceinline__ void mds_quad(F *v) {
-    mat4(v);
-#pragma unroll
-    for (int k = 0; k < 4; k++) v[k] = fadd(v[k], quad_sum(v[k]));
+constexpr int DPP_QXOR1 = 0xB1;    // quad_perm [1,0,3,2]
+constexpr int DPP_QXOR2 = 0x4E;    // quad_perm [2,3,0,1]
+constexpr int DPP_QNEXT = 0x39;    // quad_perm [1,2,3,0]
+constexpr int DPP_ROR4 = 0x124;    // row_ror:4
+constexpr int DPP_ROR8 = 0x128;    // row_ror:8
+constexpr int DPP_MIRROR = 0x140;  // row_mirror
+constexpr int DPP_HMIRROR = 0x141; // row_half_mirror
+
+__device__ __forceinline__ F ext_row16(F x) {
+    F s = fadd(x, dpp<DPP_QXOR1>(x));
+    s = fadd(s, dpp<DPP_QXOR2>(s));
+    const F nx = dpp<DPP_QNEXT>(x);
+    const F y = fadd(fadd(s, x), fadd(nx, nx));
+    F t = fadd(y, dpp<DPP_ROR4>(y));
+    t = fadd(t, dpp<DPP_ROR8>(t));
+    return fadd(y, t);
 }
-__device__ __forceinline__ void p16_quad(F *v, int q, const P2Consts16 &k) {
-    mds_quad(v);
+__device__ __forceinline__ F row_sum16(F x) {
+    F s = fadd(x, dpp<DPP_QXOR1>(x));
+    s = fadd(s, dpp<DPP_QXOR2>(s));
+    s = fadd(s, dpp<DPP_HMIRROR>(s));
+    return fadd(s, dpp<DPP_MIRROR>(s));
+}
+// all 64 lanes must call this (DPP reads neighbours); i = lane & 15
+__device__ __forceinline__ F p16_row(F x, int i, const P2Consts16 &k) {
+    x = ext_row16(x);
 #pragma unroll 1
-    for (int r = 0; r < 4; r++) {
-#pragma unroll
-        for (int i = 0; i < 4; i++) v[i] = fcube(fadd(v[i], k.ext_init[r][4 * q + i]));
-        mds_quad(v);
-    }
-    F d[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) d[i] = F{DDIAG.d16[4 * q + i]};
+    for (int r = 0; r < 4; r++) x = ext_row16(fcube(fadd(x, k.ext_init[r][i])));
+    const F d = F{DDIAG.d16[i]};
 #pragma unroll 1
     for (int r = 0; r < 20; r++) {
-        if (q == 0) v[0] = fcube(fadd(v[0], k.internal[r]));
-        F sum = quad_sum(fadd(fadd(v[0], v[1]), fadd(v[2], v[3])));
-#pragma unroll
-        for (int i = 0; i < 4; i++) v[i] = fadd(sum, fmul(v[i], d[i]));
+        const F y = fcube(fadd(x, k.internal[r]));
+        if (i == 0) x = y;
+        x = fadd(row_sum16(x), fmul(x, d));
     }
 #pragma unroll 1
-    for (int r = 0; r < 4; r++) {
-#pragma unroll
-        for (int i = 0; i < 4; i++) v[i] = fcube(fadd(v[i], k.ext_term[r][4 * q + i]));
-        mds_quad(v);
-    }
+    for (int r = 0; r < 4; r++) x = ext_row16(fcube(fadd(x, k.ext_term[r][i])));
+    return x;
 }
-// compress node i = (children 2i, 2i+1) by the quad whose lane index is q
-__device__ __forceinline__ void compress_node_quad(F *nodes, size_t i, int q, const P2DevT *pc) {
-    const F *src = nodes + 2 * i * 8 + 4 * q;  // left child = elements 0..7, right = 8..15
-    F v[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) v[j] = src[j];
-    p16_quad(v, q, pc->p16);
-    if (q < 2) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) nodes[i * 8 + 4 * q + j] = v[j];
-    }
-}
-__global__ void __launch_bounds__(256) k_merkle_level_q(F *nodes, size_t cnt, const P2DevT *pc) {
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t i = t >> 2;
-    if (i >= cnt) return;  // whole quads exit together
-    compress_node_quad(nodes, cnt + i, (int)(t & 3), pc);
-}
-__global__ void __launch_bounds__(1024) k_merkle_top(F *nodes, size_t cnt, const P2DevT *pc) {
-    // levels with parents [cnt, 2cnt), [cnt/2, cnt), ... , [1, 2) in one block of quads
-    const int q = threadIdx.x & 3, quad = threadIdx.x >> 2, nq = blockDim.x >> 2;
-    for (size_t c = cnt; c >= 1; c >>= 1) {
-        for (size_t i = quad; i < c; i += nq) compress_node_quad(nodes, c + i, q, pc);
+
+// Tree levels with parents [cnt, 2cnt), [cnt/2, cnt), ... (`levels` of them): block
+// b owns the 16 parents [cnt + 16b, cnt + 16b + 16) of the first level and the
+// subtree above them; group g (16 lanes) compresses parent g of each level.  The
+// children of the first level come from nodes, the digests of later levels stay
+// in LDS (and are written to nodes).
+__global__ void __launch_bounds__(256) k_merkle_sub(F *nodes, size_t cnt, int levels, const P2DevT *pc) {
+    __shared__ F kid[32 * 8], par[16 * 8];
+    const int g = threadIdx.x >> 4, i = threadIdx.x & 15;
+    const size_t pb = (size_t)1 << (levels - 1);  // parents per block at the first level
+    for (int w = threadIdx.x; w < (int)(2 * pb * 8); w += blockDim.x)
+        kid[w] = nodes[(2 * (cnt + blockIdx.x * pb)) * 8 + w];
+    __syncthreads();
+    for (int l = 0; l < levels; l++) {
+        const int np = (int)(pb >> l);
+        const size_t first = (cnt >> l) + (size_t)blockIdx.x * np;
+        // every group runs the permutation (DPP needs whole rows); only g < np stores
+        const int gp = g < np ? g : 0;
+        F x = kid[gp * 16 + i];
+        x = p16_row(x, i, pc->p16);
+        if (g < np && i < 8) {
+            par[g * 8 + i] = x;
+            nodes[(first + g) * 8 + i] = x;
+        }
+        __syncthreads();
+        for (int w = threadIdx.x; w < np * 8; w += blockDim.x) kid[w] = par[w];
         __syncthreads();
     }
 }
@@ -550,16 +563,20 @@ void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s) {
     const P2DevT *p = (const P2DevT *)pc;
     size_t rows = (size_t)1 << log_rows;
     size_t cnt = rows >> 1;
-    // wide levels: one thread per node; narrower ones: one lane quad per node
+    // wide levels: one thread per node
     while (cnt >= ((size_t)1 << 15)) {
         k_merkle_level<<<nblk(cnt), 256, 0, s>>>(nodes, cnt, p);
         cnt >>= 1;
     }
-    while (cnt > 256) {
-        k_merkle_level_q<<<nblk(cnt * 4), 256, 0, s>>>(nodes, cnt, p);
-        cnt >>= 1;
+    // narrow levels: subtrees of up to 5 levels per block, 16 lanes per compression
+    while (cnt >= 1) {
+        int lg = 0;
+        while (((size_t)1 << (lg + 1)) <= cnt) lg++;
+        const int levels = lg + 1 < 5 ? lg + 1 : 5;
+        const size_t pb = (size_t)1 << (levels - 1);
+        k_merkle_sub<<<(unsigned)(cnt / pb), 256, 0, s>>>(nodes, cnt, levels, p);
+        cnt >>= levels;
     }
-    if (cnt >= 1) k_merkle_top<<<1, 1024, 0, s>>>(nodes, cnt, p);
 }
 
 __global__ void k_merkle_open(const F *cw, const F *nodes, int width, int log_rows, int la,
