@@ -683,7 +683,7 @@ void p2_permute_batch(const void *pc, int width, F *states, int n, hipStream_t s
 using AccF2 = AccF;
 
 constexpr int T_JG = 2;
-__global__ void k_whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts) {
+__global__ void k_whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts, size_t nz_len) {
     const size_t X = (size_t)1 << (m - k);
     const int U = 1 << k;
     const int u = blockIdx.y;
@@ -691,7 +691,9 @@ __global__ void k_whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *pa
     AccF2 acc[T_JG];
     const size_t st = (size_t)gridDim.x * blockDim.x;
     const F *fu = f + (size_t)u * X;
-    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < X; x += st) {
+    // f is zero from nz_len on: chunks past it contribute zero sums
+    const size_t xend = (size_t)u * X >= nz_len ? 0 : (nz_len - (size_t)u * X < X ? nz_len - (size_t)u * X : X);
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < xend; x += st) {
         F v = fu[x];
 #pragma unroll
         for (int jj = 0; jj < T_JG; jj++)
@@ -709,23 +711,23 @@ int whir_T_chunks(int m, int k) {
     unsigned nc = nblk(X, 256 * 16);
     return nc < 1 ? 1 : (int)nc;
 }
-int whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts, hipStream_t s) {
+int whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts, size_t nz_len, hipStream_t s) {
     unsigned nc = (unsigned)whir_T_chunks(m, k);
-    k_whir_T_f<<<dim3(nc, 1 << k, (nE + T_JG - 1) / T_JG), 256, 0, s>>>(f, m, k, E, nE, parts);
+    k_whir_T_f<<<dim3(nc, 1 << k, (nE + T_JG - 1) / T_JG), 256, 0, s>>>(f, m, k, E, nE, parts, nz_len);
     return (int)nc;
 }
 
-__global__ void k_whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out) {
+__global__ void k_whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, size_t nz_len) {
     const size_t X = (size_t)1 << (m - k);
     size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= X) return;
     AccF2 acc;
-    for (int u = 0; u < (1 << k); u++) acc.add(eqr[u], f[(size_t)u * X + x]);
+    for (int u = 0; u < (1 << k) && (size_t)u * X + x < nz_len; u++) acc.add(eqr[u], f[(size_t)u * X + x]);
     out[x] = acc.get();
 }
-void whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, hipStream_t s) {
+void whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, size_t nz_len, hipStream_t s) {
     size_t X = (size_t)1 << (m - k);
-    k_whir_fold_f<<<nblk(X), 256, 0, s>>>(f, m, k, eqr, out);
+    k_whir_fold_f<<<nblk(X), 256, 0, s>>>(f, m, k, eqr, out, nz_len);
 }
 
 __global__ void k_build_w(EF *W, size_t Nw, int lo, int nF, const EF *scalF, const F *hiF,

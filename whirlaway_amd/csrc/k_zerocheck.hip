@@ -49,12 +49,6 @@ __device__ __forceinline__ EF add_rc(EF t, F rc) {
     return t;
 }
 
-struct SharedConsts {
-    EF ap[AIR_NCONS];
-    F rc[AIR_NRC];
-    ChainLin cl;
-};
-
 }  // namespace
 
 // PTS points per 256-thread block; LANES = 256 / PTS lanes per point.
@@ -63,18 +57,12 @@ __global__ void __launch_bounds__(256) k_zc_round_w(const EF *U, size_t stride, 
                                                    const AirConsts *air, const ChainLin *cl,
                                                    const EF *ap, const EF *eq, EF *parts) {
     constexpr int LANES = 256 / PTS;
-    __shared__ SharedConsts K;
     __shared__ EF cols[PTS][NC];
     __shared__ EF sb[PTS][128];
     __shared__ EF red[4];
-    // constants: alpha powers, round constants, chain coefficients
-    {
-        const uint32_t *srcs[3] = {(const uint32_t *)ap, (const uint32_t *)air->rc, (const uint32_t *)cl};
-        uint32_t *dsts[3] = {(uint32_t *)K.ap, (uint32_t *)K.rc, (uint32_t *)&K.cl};
-        const int words[3] = {AIR_NCONS * 8, AIR_NRC, (int)(sizeof(ChainLin) / 4)};
-        for (int a = 0; a < 3; a++)
-            for (int i = threadIdx.x; i < words[a]; i += 256) dsts[a][i] = srcs[a][i];
-    }
+    // constants (alpha powers, round constants, chain coefficients) are read through
+    // the caches: keeping them out of LDS leaves room for one more block per CU
+    struct { const EF *ap; const F *rc; const ChainLin &cl; } K{ap, air->rc, *cl};
     const int pt = threadIdx.x / LANES, lane = threadIdx.x % LANES;
     const size_t h = (size_t)1 << (l - 1);
     const int zi = blockIdx.y;
@@ -98,40 +86,49 @@ __global__ void __launch_bounds__(256) k_zc_round_w(const EF *U, size_t stride, 
     EF acc = ezero();
     if (act) {
         // phase 1: S-boxes (112 full-round tasks) and the linear chain (20 + 16 tasks)
-        for (int t = lane; t < 148; t += LANES) {
-            if (t < 112) {
-                int r8 = t >> 4, i = t & 15;  // 0..3 beginning rounds, 4..6 ending rounds 1..3
-                EF prev;
-                F rc;
-                int slot;
-                if (r8 < 4) {
-                    prev = r8 == 0 ? mds_row(cw + 1, i) : cw[17 + 16 * (r8 - 1) + i];
-                    rc = K.rc[r8 * 16 + i];
-                    slot = r8;
-                } else {
-                    int er = r8 - 3;  // 1..3
-                    prev = cw[101 + 16 * (er - 1) + i];
-                    rc = K.rc[84 + er * 16 + i];
-                    slot = 4 + er;
-                }
-                sb[pt][slot * 16 + i] = ecube(add_rc(prev, rc));
-            } else if (t < 132) {
-                int r = t - 112;
-                AccF d;
-#pragma unroll 4
-                for (int j = 0; j < 16; j++) d.add(cw[65 + j], K.cl.A[r][j]);
-                for (int q = 0; q < r; q++) d.add(cw[81 + q], K.cl.B[r][q]);
-                EF tt = ecube(add_rc(d.get(), K.rc[64 + r]));
-                acc = eadd(acc, emul(K.ap[65 + r], esub(tt, cw[81 + r])));
+        auto sbox = [&](int t) {  // t < 112
+            int r8 = t >> 4, i = t & 15;  // 0..3 beginning rounds, 4..6 ending rounds 1..3
+            EF prev;
+            F rc;
+            int slot;
+            if (r8 < 4) {
+                prev = r8 == 0 ? mds_row(cw + 1, i) : cw[17 + 16 * (r8 - 1) + i];
+                rc = K.rc[r8 * 16 + i];
+                slot = r8;
             } else {
-                int i = t - 132;
-                AccF d;
-#pragma unroll 4
-                for (int j = 0; j < 16; j++) d.add(cw[65 + j], K.cl.Ei[i][j]);
-#pragma unroll 4
-                for (int q = 0; q < 20; q++) d.add(cw[81 + q], K.cl.Ep[i][q]);
-                sb[pt][64 + i] = ecube(add_rc(d.get(), K.rc[84 + i]));
+                int er = r8 - 3;  // 1..3
+                prev = cw[101 + 16 * (er - 1) + i];
+                rc = K.rc[84 + er * 16 + i];
+                slot = 4 + er;
             }
+            sb[pt][slot * 16 + i] = ecube(add_rc(prev, rc));
+        };
+        // chain task h < 36: h < 20 is partial round h (its S-box input and constraint),
+        // h >= 20 the input of ending round 0, element h - 20.  One code path: 16
+        // chain-input terms and 20 post_sbox terms (B[r][q] = 0 for q >= r).
+        auto chain = [&](int h) {
+            const F *ca = h < 20 ? K.cl.A[h] : K.cl.Ei[h - 20];
+            const F *cb = h < 20 ? K.cl.B[h] : K.cl.Ep[h - 20];
+            AccF d;
+#pragma unroll 4
+            for (int j = 0; j < 16; j++) d.add(cw[65 + j], ca[j]);
+#pragma unroll 4
+            for (int q = 0; q < 20; q++) d.add(cw[81 + q], cb[q]);
+            EF tt = ecube(add_rc(d.get(), K.rc[h < 20 ? 64 + h : 84 + h - 20]));
+            if (h < 20) acc = eadd(acc, emul(K.ap[65 + h], esub(tt, cw[81 + h])));
+            else sb[pt][64 + h - 20] = tt;
+        };
+        if (LANES == 64) {
+            // every pass runs one code path across the wave: 64 S-boxes, the 36
+            // chain tasks, the remaining 48 S-boxes
+            sbox(lane);
+            if (lane < 36) chain(lane);
+            if (lane < 48) sbox(64 + lane);
+        } else {
+            // 256 lanes: wave 0 the first 64 S-boxes, wave 1 the chain, wave 2 the rest
+            if (lane < 64) sbox(lane);
+            else if (lane < 100) chain(lane - 64);
+            else if (lane >= 128 && lane < 176) sbox(lane - 64);
         }
     }
     __syncthreads();

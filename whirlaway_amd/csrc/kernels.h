@@ -118,10 +118,10 @@ void fold_f_to_ef(const F *in, int l, const EF *c, EF *out, hipStream_t s);
 struct P2Dev;  // device Poseidon2 constants (16 and 24)
 
 // T[j][u] partial sums: sum_x f(u*2^(m-k) + x) * E_j(x), E_j EF tables (stride 2^(m-k)).
-// parts layout [j][u][chunk]; returns n_chunks.
-int whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts, hipStream_t s);
-// f'(x) = sum_u eqr[u] f(u*2^(m-k) + x)
-void whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, hipStream_t s);
+// parts layout [j][u][chunk]; returns n_chunks.  f[i] = 0 for i >= nz_len (not read).
+int whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts, size_t nz_len, hipStream_t s);
+// f'(x) = sum_u eqr[u] f(u*2^(m-k) + x)  (f[i] = 0 for i >= nz_len, not read)
+void whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, size_t nz_len, hipStream_t s);
 // W[x] = sum_j scal[j] * Ehi_j[x >> lo] * Elo_j[x & mask]; F-point tables and EF-point tables
 // (accumulate: W += ..., else W = ...)
 void whir_build_w(EF *W, int m, int lo, int nF, const EF *scalF, const F *hiF, const F *loF,

@@ -252,19 +252,20 @@ EF host_sc_round(Transcript &ts, wl_ctx *c, const P2Dev *pdev, std::vector<EF> &
 
 struct WhirWitness {
     const F *evals;   // device, 2^v
+    size_t nz_len;    // evals[i] = 0 for i >= nz_len
     Codeword cw0;
     std::vector<EF> ood_points, ood_answers;
 };
 
 // evaluate a device F table (2^m) at expand(z, m) with one T-style pass
-EF eval_f_univ(wl_ctx *c, const F *f, int m, const EF &z) {
+EF eval_f_univ(wl_ctx *c, const F *f, int m, size_t nz_len, const EF &z) {
     std::vector<EF> pt = wh::expand_univ(z, m);
     int k = m / 2;
     std::vector<EF> ph(pt.begin(), pt.begin() + k), pl(pt.begin() + k, pt.end());
     DBuf<EF> El = eq_dev(c, pl, eone());
     int nc = wk::whir_T_chunks(m, k);
     DBuf<EF> parts(c->pool, ((size_t)1 << k) * nc);
-    wk::whir_T_f(f, m, k, El, 1, parts, c->stream);
+    wk::whir_T_f(f, m, k, El, 1, parts, nz_len, c->stream);
     DBuf<EF> T(c->pool, (size_t)1 << k);
     wk::reduce_parts(parts, 1 << k, nc, T, c->stream);
     DBuf<EF> Eh = eq_dev(c, ph, eone());
@@ -293,6 +294,7 @@ void whir_commit(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &t
     size_t colen = (size_t)1 << log_m;
     int n_nonzero = (int)std::min<size_t>(n_cols, (nonzero_len + colen - 1) / colen);
     w.evals = evals;
+    w.nz_len = nonzero_len;
     commit_codeword(c, pdev, w.cw0, evals, n_cols, n_nonzero, log_m, v + cfg.starting_log_inv_rate - f0);
     w.cw0.base = true;
     w.cw0.fold = f0;
@@ -300,7 +302,7 @@ void whir_commit(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &t
     c->d2h(root, w.cw0.nodes.p + 8, 8);
     ts.add_base(root, 8);
     for (int i = 0; i < cfg.commit_ood_samples; i++) w.ood_points.push_back(ts.sample_ef());
-    for (int i = 0; i < cfg.commit_ood_samples; i++) w.ood_answers.push_back(eval_f_univ(c, evals, v, w.ood_points[i]));
+    for (int i = 0; i < cfg.commit_ood_samples; i++) w.ood_answers.push_back(eval_f_univ(c, evals, v, w.nz_len, w.ood_points[i]));
     ts.add_ef(w.ood_answers.data(), w.ood_answers.size());
 }
 
@@ -334,7 +336,7 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
     }
     int nch = wk::whir_T_chunks(v, f0);
     DBuf<EF> parts(c->pool, (size_t)nC * U * nch);
-    wk::whir_T_f(w.evals, v, f0, E, nC, parts, c->stream);
+    wk::whir_T_f(w.evals, v, f0, E, nC, parts, w.nz_len, c->stream);
     std::vector<EF> T = reduce_many(c, parts, nC * U, nch);
     std::vector<std::vector<EF>> Tj(nC);
     for (int j = 0; j < nC; j++) Tj[j].assign(T.begin() + (size_t)j * U, T.begin() + (size_t)(j + 1) * U);
@@ -376,7 +378,7 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
     {
         std::vector<EF> eqr = wh::eq_host(rf);
         DBuf<EF> deq = upload_ef(c, eqr);
-        wk::whir_fold_f(w.evals, v, f0, deq, s.f, c->stream);
+        wk::whir_fold_f(w.evals, v, f0, deq, s.f, w.nz_len, c->stream);
     }
     s.W = DBuf<EF>(c->pool, (size_t)1 << X);
     add_constraints_w(c, s.W, X, cs, true);
