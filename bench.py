@@ -44,6 +44,40 @@ MODELS = {
 }
 
 
+def run_micro(args):
+    """Kernel micro configs at full size: (a) 2^24 base-field values folded 24 times
+    by extension challenges (benches/folding_benchmarks.rs shape), (b) the eq-weighted
+    a^2 b product sumcheck over n = 24 variables.  Device time with HIP events;
+    algorithmic bytes: (a) read 2^24 F + write/read each EF table once, (b) per round
+    read the two tables and the eq table and write the folded tables."""
+    import numpy as np
+    import whirlaway_amd as wa
+    from whirlaway_amd import sumcheck, utils
+    from whirlaway_amd.air import bench_inputs
+    P = wa.P
+    rng = np.random.default_rng(args.seed + 24)
+    ctx = wa.Context(0)
+    m = rng.integers(0, P, 1 << 24, dtype=np.uint32)
+    ch = rng.integers(0, P, (24, 8), dtype=np.uint32)
+    utils.fold_chain(ctx, m, ch)
+    fold_ms = min(utils.fold_chain(ctx, m, ch)[2] for _ in range(3))
+    ef_tables = sum(1 << (23 - i) for i in range(24))
+    fold_bytes = 4 * (1 << 24) + 32 * ef_tables + 32 * (ef_tables - (1 << 23))
+    a = rng.integers(0, P, 1 << 24, dtype=np.uint32)
+    b = rng.integers(0, P, 1 << 24, dtype=np.uint32)
+    z = rng.integers(0, P, (24, 8), dtype=np.uint32)
+    _, _, p16, _ = bench_inputs(args.seed, 4, with_inputs=False)
+    sumcheck.prove_a2b(ctx, p16, a, b, z)
+    sc_ms = min(sumcheck.prove_a2b(ctx, p16, a, b, z)["ms"] for _ in range(2))
+    # round 0 reads a, b (F) and eq (EF, 2^23); later rounds read 2 EF tables and eq
+    sc_bytes = (8 + 32) * (1 << 23) + sum((64 + 32) * (1 << (23 - j)) + 64 * (1 << (23 - j)) for j in range(1, 24))
+    ctx.close()
+    return {"fold_2^24_x24": {"ms": round(fold_ms, 4), "GB_per_s": round(fold_bytes / fold_ms / 1e6, 1),
+                               "hbm_frac": round(fold_bytes / fold_ms / 1e6 / HBM_PEAK_GBS, 4)},
+            "sumcheck_a2b_n24": {"ms": round(sc_ms, 4), "GB_per_s": round(sc_bytes / sc_ms / 1e6, 1),
+                                 "hbm_frac": round(sc_bytes / sc_ms / 1e6 / HBM_PEAK_GBS, 4)}}
+
+
 def _requests(seed, count, lo, hi):
     """count proof requests with log_n uniform in {lo..hi} and inputs drawn per request
     (src/examples/poseidon2.rs:104-151 with seed = request seed)."""
@@ -163,6 +197,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=256, help="serving leg: requests (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=120.0, help="serving leg: Poisson arrivals per second")
     ap.add_argument("--slots", type=int, default=4, help="engine slots (concurrent proofs)")
+    ap.add_argument("--no-micro", action="store_true", help="skip the kernel micro leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -262,6 +297,7 @@ def main():
             cpu["note"] = "CPU and GPU proofs differ"
         b.close()
 
+    micro = None if args.no_micro else run_micro(args)
     batched = run_batched(args) if args.batch_jobs > 0 else None
     serving = run_serving(args) if args.serve_requests > 0 else None
 
@@ -286,6 +322,7 @@ def main():
         "cpu_baseline": cpu,
         "phase_ms_per_step": dict(zip(PHASES, [round(x / args.steps, 4) for x in phases])),
         "pcie_inclusive": pcie,
+        "micro": micro,
         "batched": batched,
         "serving": serving,
     }

@@ -170,3 +170,41 @@ def test_trace_generation(gpu_ctx, oracle, n):
     w.close()
     table.close()
     assert np.array_equal(got, oracle.Bench(n, 0).trace())
+
+
+def test_fold_chain_full_size(gpu_ctx, oracle):
+    """Kernel micro (a) at its full size: 2^24 base-field values folded 24 times by
+    extension challenges, every intermediate table bit-exact against the oracle."""
+    from whirlaway_amd import utils
+    log_n = folds = 24
+    _say(f"fold_chain full size log_n={log_n}")
+    rng = np.random.default_rng(24)
+    m = rng.integers(0, P, 1 << log_n, dtype=np.uint32)
+    ch = rng.integers(0, P, (folds, 8), dtype=np.uint32)
+    final, allt, ms = utils.fold_chain(gpu_ctx, m, ch, all_tables=True)
+    cur, off = None, 0
+    for i in range(folds):
+        sc = np.concatenate([(np.array(pf.esub(pf.ef(1), pf.ef(ch[i])), dtype=np.uint32)), ch[i]])
+        cur = oracle.fold_f_large(m, sc, 1) if i == 0 else oracle.fold_ef_large(cur, sc, 1)
+        ln = cur.size
+        assert np.array_equal(allt[off: off + ln], cur), i
+        off += ln
+    assert np.array_equal(final, cur)
+
+
+def test_sumcheck_a2b_full_size(gpu_ctx, oracle):
+    """Kernel micro (b) at its full size: n = 24, proof words, challenges and final
+    values bit-exact against the oracle."""
+    from whirlaway_amd import sumcheck
+    n = 24
+    _say(f"sumcheck a2b full size n={n}")
+    b = oracle.Bench(5, seed=11)
+    rng = np.random.default_rng(1024)
+    a = rng.integers(0, P, 1 << n, dtype=np.uint32)
+    bb = rng.integers(0, P, 1 << n, dtype=np.uint32)
+    z = rng.integers(0, P, (n, 8), dtype=np.uint32)
+    got = sumcheck.prove_a2b(gpu_ctx, b.p2_consts(16), a, bb, z)
+    want = b.sc_a2b(a, bb, z.reshape(-1).copy())
+    for k in ("proof", "challenges", "final_evals", "final_sum"):
+        assert np.array_equal(got[k], want[k]), k
+    b.close()
