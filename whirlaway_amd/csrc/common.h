@@ -46,8 +46,8 @@ __device__ __forceinline__ void p16_permute(F *s, const C &k) { p2_permute<16, 2
 template <typename C>
 __device__ __forceinline__ void p24_permute(F *s, const C &k) { p2_permute<24, 23>(s, k, DDIAG.d24); }
 
-// Lazy accumulator of EF x F products: 64-bit per coefficient, reduced every
-// 4 terms (4 products of values < P stay below 2^64).
+// Lazy accumulator of EF x F products: 64-bit per coefficient (4 products of
+// values < P stay below 2^64 after a fold64), reduced once in get().
 struct AccF {
     uint64_t s[8];
     EF r;
@@ -62,10 +62,16 @@ struct AccF {
         for (int i = 0; i < 8; i++) { r.c[i] = fadd(r.c[i], F{redc64(s[i])}); s[i] = 0; }
         n = 0;
     }
+    // sums stay below 2^64: shrink them (fold64, one multiply-add each) after every
+    // fourth product and reduce only in get()
     __device__ __forceinline__ void add(const EF &a, F b) {
 #pragma unroll
         for (int i = 0; i < 8; i++) s[i] += (uint64_t)a.c[i].v * b.v;
-        if (++n == 4) flush();
+        if (++n == 4) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) s[i] = fold64(s[i]);
+            n = 0;
+        }
     }
     __device__ __forceinline__ EF get() { flush(); return r; }
 };

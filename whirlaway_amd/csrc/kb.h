@@ -63,6 +63,11 @@ KB_HD uint32_t redc64(uint64_t x) {  // any x < 2^64
     return hi < mh ? rb : r;
 }
 
+// s mod P-preserving shrink of a 64-bit sum: hi * (2^32 mod P) + lo < 2^57.1, so
+// four more products below P^2 can be added without overflow (one v_mad_u64_u32).
+constexpr uint32_t C32 = 0x01fffffeu;  // 2^32 mod P
+KB_HD uint64_t fold64(uint64_t s) { return (uint64_t)(uint32_t)(s >> 32) * C32 + (uint32_t)s; }
+
 KB_HD F fadd(F a, F b) { uint32_t s = a.v + b.v; return F{umin(s, s - P)}; }
 KB_HD F fsub(F a, F b) { uint32_t d = a.v - b.v; return F{umin(d, d + P)}; }
 KB_HD F fneg(F a) { return F{a.v ? P - a.v : 0u}; }
@@ -97,8 +102,8 @@ KB_HD bool eeq(const EF &a, const EF &b) { for (int i = 0; i < 8; i++) if (a.c[i
 
 KB_HD F ftimes3(F a) { return fadd(fdbl(a), a); }
 
-// c_k = sum_{i<=k} a_i b_{k-i} + sum_{i>k} a_i (3 b_{k+8-i}); two 4-term
-// 64-bit partial sums per coefficient, each reduced once.
+// c_k = sum_{i<=k} a_i b_{k-i} + sum_{i>k} a_i (3 b_{k+8-i}); eight products
+// per coefficient in one 64-bit sum (shrunk by fold64 after four), one reduction.
 KB_HD EF emul(const EF &a, const EF &b) {
     uint32_t b3[8];
 #pragma unroll
@@ -106,14 +111,14 @@ KB_HD EF emul(const EF &a, const EF &b) {
     EF r;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        uint64_t s0 = 0, s1 = 0;
+        uint64_t s = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             uint32_t bb = i <= k ? b.c[k - i].v : b3[k + 8 - i];
-            uint64_t p = (uint64_t)a.c[i].v * bb;
-            if (i < 4) s0 += p; else s1 += p;
+            s += (uint64_t)a.c[i].v * bb;
+            if (i == 3) s = fold64(s);
         }
-        r.c[k] = fadd(F{redc64(s0)}, F{redc64(s1)});
+        r.c[k] = F{redc64(s)};
     }
     return r;
 }
