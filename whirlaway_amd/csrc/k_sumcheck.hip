@@ -646,6 +646,52 @@ int a2b_round_f(const F *a, const F *b, int l, const EF *eq, EF *parts, hipStrea
     k_a2b_round<true><<<dim3(nb, all4 ? 4 : 3), 256, 0, s>>>(a, b, h, eq, parts);
     return (int)nb;
 }
+// Rounds >= 1 of the a^2 b sumcheck in one pass per round: fold the previous
+// round's tables (length 4h) by its challenge c into tables of length 2h (from
+// the base-field inputs a0, b0 in round 1, in place afterwards), halve the eq
+// table (eq(z[j+1..], x) = eq(z[j..], (0, x)) + eq(z[j..], (1, x))), and sum
+// eq * A_z^2 B_z for z = 0, 2, 3 with one read of every table.
+template <bool FIRST>
+__global__ void __launch_bounds__(256) k_a2b_round_fused(const F *a0, const F *b0, EF *A, EF *B, size_t h,
+                                                         EF c, EF *eq, EF *parts) {
+    EF acc[3] = {ezero(), ezero(), ezero()};
+    const size_t st = (size_t)gridDim.x * blockDim.x;
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < h; x += st) {
+        EF al, ah, bl, bh;
+        if (FIRST) {
+            auto fl = [&](const F *t, size_t i) { return eadd(efrom(t[i]), emulf(c, fsub(t[i + 2 * h], t[i]))); };
+            al = fl(a0, x); ah = fl(a0, x + h); bl = fl(b0, x); bh = fl(b0, x + h);
+        } else {
+            auto fl = [&](const EF *t, size_t i) { return eadd(t[i], emul(c, esub(t[i + 2 * h], t[i]))); };
+            al = fl(A, x); ah = fl(A, x + h); bl = fl(B, x); bh = fl(B, x + h);
+        }
+        A[x] = al; A[x + h] = ah; B[x] = bl; B[x + h] = bh;
+        const EF e = eadd(eq[x], eq[x + h]);
+        eq[x] = e;
+        const EF da = esub(ah, al), db = esub(bh, bl);
+        // z = 0
+        acc[0] = eadd(acc[0], emul(e, emul(esqr(al), bl)));
+        // z = 2: lo + 2 (hi - lo); z = 3: one more difference
+        EF a2 = eadd(al, edbl(da)), b2 = eadd(bl, edbl(db));
+        acc[1] = eadd(acc[1], emul(e, emul(esqr(a2), b2)));
+        EF a3 = eadd(a2, da), b3 = eadd(b2, db);
+        acc[2] = eadd(acc[2], emul(e, emul(esqr(a3), b3)));
+    }
+    __shared__ EF lds[16];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        EF t = block_sum(acc[i], lds);
+        if (threadIdx.x == 0) parts[(size_t)i * gridDim.x + blockIdx.x] = t;
+    }
+}
+int a2b_round_fused(const F *a0, const F *b0, EF *A, EF *B, int l, bool first, EF c, EF *eq, EF *parts,
+                    hipStream_t s) {
+    size_t h = (size_t)1 << (l - 1);
+    unsigned nb = dot_blocks(h);
+    if (first) k_a2b_round_fused<true><<<nb, 256, 0, s>>>(a0, b0, A, B, h, c, eq, parts);
+    else k_a2b_round_fused<false><<<nb, 256, 0, s>>>(a0, b0, A, B, h, c, eq, parts);
+    return (int)nb;
+}
 int a2b_round_ef(const EF *a, const EF *b, int l, const EF *eq, EF *parts, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
     unsigned nb = dot_blocks(h);

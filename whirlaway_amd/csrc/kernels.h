@@ -111,6 +111,11 @@ void fold_ef_large(const EF *in, int l, const EF *sc, int k, EF *out, hipStream_
 // all4); F inputs (base) or EF; parts [z][block]
 int a2b_round_f(const F *a, const F *b, int l, const EF *eq, EF *parts, hipStream_t s, bool all4);
 int a2b_round_ef(const EF *a, const EF *b, int l, const EF *eq, EF *parts, hipStream_t s);
+// a^2 b rounds >= 1 fused with the fold by the previous challenge c and the
+// halving of eq: tables of length 2^l after the fold (from a0, b0 when first, in
+// place in A, B otherwise), eq of length 2^l halved in place; parts [3][block].
+int a2b_round_fused(const F *a0, const F *b0, EF *A, EF *B, int l, bool first, EF c, EF *eq, EF *parts,
+                    hipStream_t s);
 // fold an F table by (1-c, c) into an EF table of half size
 void fold_f_to_ef(const F *in, int l, const EF *c, EF *out, hipStream_t s);
 

@@ -109,29 +109,34 @@ int wl_sumcheck_a2b(wl_ctx *c, const uint32_t *p16, const uint32_t *a, const uin
         upload_canon(c, (uint32_t *)da.p, a, N);
         upload_canon(c, (uint32_t *)db.p, b, N);
         HIP_CHECK(hipEventRecord(c->ev0, c->stream));
-        // claimed sum: (1 - z0) S(0) + z0 S(1) over the top variable
+        // eq(z[1..], .) once; every later round halves it inside its kernel
+        DBuf<EF> eq = eq_dev(c, std::vector<EF>(zz.begin() + 1, zz.end()), eone());
+        // round 0 on the base-field tables at z = 0..3: z = 0, 1 give the claimed sum
+        // (1 - z0) S(0) + z0 S(1), z = 0, 2, 3 the round's evaluations
         EF sum;
+        std::vector<EF> v3(3);
         {
-            DBuf<EF> eq = eq_dev(c, std::vector<EF>(zz.begin() + 1, zz.end()), eone());
             int nb = wk::dot_blocks(N / 2);
             DBuf<EF> parts(c->pool, 4 * (size_t)nb);
             wk::a2b_round_f(da, db, (int)n, eq, parts, c->stream, true);
             std::vector<EF> s4 = reduce_many(c, parts, 4, nb);
             sum = eadd(emul(esub(eone(), zz[0]), s4[0]), emul(zz[0], s4[1]));
+            v3 = {s4[0], s4[2], s4[3]};
         }
         EF missing = eone();
         bool has_missing = false;
         std::vector<EF> ch;
-        DBuf<EF> ea(c->pool, N / 2), eb(c->pool, N / 2);
+        DBuf<EF> ea(c->pool, std::max<size_t>(N / 2, 1)), eb(c->pool, std::max<size_t>(N / 2, 1));
         for (uint32_t j = 0; j < n; j++) {
             int l = (int)(n - j);
             EF r = zz[j];
-            DBuf<EF> eq = eq_dev(c, std::vector<EF>(zz.begin() + 1 + j, zz.end()), eone());
-            int nb = wk::dot_blocks((size_t)1 << (l - 1));
-            DBuf<EF> parts(c->pool, 3 * (size_t)nb);
-            if (j == 0) wk::a2b_round_f(da, db, l, eq, parts, c->stream, false);
-            else wk::a2b_round_ef(ea, eb, l, eq, parts, c->stream);
-            std::vector<EF> v3 = reduce_many(c, parts, 3, nb);
+            if (j > 0) {
+                // fold by the previous challenge, halve eq, evaluate: one pass
+                int nb = wk::dot_blocks((size_t)1 << (l - 1));
+                DBuf<EF> parts(c->pool, 3 * (size_t)nb);
+                wk::a2b_round_fused(da, db, ea, eb, l, j == 1, ch.back(), eq, parts, c->stream);
+                v3 = reduce_many(c, parts, 3, nb);
+            }
             EF e0 = v3[0], e2 = v3[1], e3 = v3[2];
             if (has_missing) { e0 = emul(e0, missing); e2 = emul(e2, missing); e3 = emul(e3, missing); }
             EF e1 = emul(esub(sum, emul(e0, esub(eone(), r))), wh::einv(r));
@@ -146,24 +151,30 @@ int wl_sumcheck_a2b(wl_ctx *c, const uint32_t *p16, const uint32_t *a, const uin
             EF f = eadd(emul(esub(eone(), r), esub(eone(), cj)), emul(r, cj));
             missing = has_missing ? emul(f, missing) : f;
             has_missing = true;
-            DBuf<EF> dc = upload_ef(c, std::vector<EF>(1, cj));
-            if (j == 0) {
-                wk::fold_f_to_ef(da, l, dc, ea, c->stream);
-                wk::fold_f_to_ef(db, l, dc, eb, c->stream);
+        }
+        // the last fold (tables of length 2) on the host
+        EF fa, fb;
+        {
+            EF lo[2], hi[2];
+            if (n == 1) {
+                F t[4];
+                c->d2h(t, da.p, 2);
+                c->d2h(t + 2, db.p, 2);
+                lo[0] = efrom(t[0]); hi[0] = efrom(t[1]); lo[1] = efrom(t[2]); hi[1] = efrom(t[3]);
             } else {
-                wk::fold_ef(ea, l, dc, c->stream);
-                wk::fold_ef(eb, l, dc, c->stream);
+                EF t[4];
+                c->d2h(t, ea.p, 2);
+                c->d2h(t + 2, eb.p, 2);
+                lo[0] = t[0]; hi[0] = t[1]; lo[1] = t[2]; hi[1] = t[3];
             }
-            HIP_CHECK(hipStreamSynchronize(c->stream));
+            fa = eadd(lo[0], emul(ch.back(), esub(hi[0], lo[0])));
+            fb = eadd(lo[1], emul(ch.back(), esub(hi[1], lo[1])));
         }
         HIP_CHECK(hipEventRecord(c->ev1, c->stream));
         HIP_CHECK(hipEventSynchronize(c->ev1));
         float t = 0;
         HIP_CHECK(hipEventElapsedTime(&t, c->ev0, c->ev1));
         if (ms) *ms = t;
-        EF fa, fb;
-        c->d2h(&fa, ea.p, 1);
-        c->d2h(&fb, eb.p, 1);
         for (uint32_t j = 0; j < n; j++) wh::from_mont_ef(ch[j], challenges + 8 * j);
         wh::from_mont_ef(fa, final_evals);
         wh::from_mont_ef(fb, final_evals + 8);
