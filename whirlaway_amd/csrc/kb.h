@@ -219,13 +219,23 @@ KB_HD void internal_apply_all(F *s, F sum) {
         internal_apply_all<W, I + 1>(s, sum);
     }
 }
+// sum of W reduced values: pairs add without overflow (each < P < 2^31), the
+// W/2 pair sums accumulate in 64 bits, and one fold + two conditional
+// subtractions bring the total below P (the total is < W/2 * 2^32).
+template <int W>
+KB_HD F fsum_lazy(const F *s) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < W; i += 2) acc += (uint32_t)(s[i].v + s[i + 1].v);
+    uint64_t t = (uint64_t)(uint32_t)(acc >> 32) * C32 + (uint32_t)acc;  // < 2^32 + W/2 * 2^25 < 2.1 P
+    t = t >= P ? t - P : t;
+    t = t >= P ? t - P : t;
+    return F{(uint32_t)t};
+}
 // s = (J + diag(d)) s with shifts and adds only
 template <int W>
 KB_HD void internal_layer_fast(F *s) {
-    F sum = s[0];
-#pragma unroll
-    for (int i = 1; i < W; i++) sum = fadd(sum, s[i]);
-    internal_apply_all<W>(s, sum);
+    internal_apply_all<W>(s, fsum_lazy<W>(s));
 }
 
 template <int W>
