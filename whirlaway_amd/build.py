@@ -19,6 +19,8 @@ FLAGS = [
     # packed FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
+    # host code: AVX2 for the transcript's Poseidon2 (every x86-64 server CPU in use has it)
+    "-Xarch_host", "-mavx2",
 ]
 
 

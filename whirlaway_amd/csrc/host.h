@@ -10,6 +10,7 @@
 #include <stdexcept>
 #include <vector>
 #include "common.h"
+#include "host_p16.h"
 
 namespace wh {
 using namespace kb;
@@ -156,7 +157,11 @@ struct Transcript {
     void duplex() {
         for (int i = 0; i < n_in; i++) state[i] = in_buf[i];
         n_in = 0;
+#if !defined(__HIP_DEVICE_COMPILE__)
+        HostP16::permute(state, *p16);
+#else
         p2_permute<16, 20>(state, *p16, DIAG.d16);
+#endif
         for (int i = 0; i < 8; i++) out_buf[i] = state[i];
         n_out = 8;
     }
