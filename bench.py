@@ -33,7 +33,8 @@ MEASURED_TRAFFIC_N17 = {
     1: (92.23, 5.75),     # k_merkle_leaves
     3: (92.93, 139.00),   # k_ntt2_a + k_ntt2_b (one rs_encode, averaged over the 4 commits)
 }
-PHASES = ["commit", "zerocheck_round0", "zerocheck_rounds", "batching", "inner_sumcheck", "whir_open"]
+PHASES = ["commit", "zerocheck_round0", "zerocheck_rounds", "batching", "inner_sumcheck", "whir_open",
+          "host_wait_for_device"]
 MODELS = {
     1: ("merkle_leaves (Poseidon2-24 sponge)", "valu", 982.0),
     2: ("zerocheck_rounds (EF AIR eval, rounds>=1)", "valu", 31168.0),

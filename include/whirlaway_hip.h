@@ -97,8 +97,9 @@ int wl_witness_download(const wl_witness *w, uint32_t *out);
 
 /* Accumulated per-phase wall times in ms since the last reset: [commit,
  * zerocheck round 0, zerocheck rounds >= 1, inner sums + batching, inner
- * sumcheck, WHIR opening]. */
-void wl_phase_times(wl_ctx *ctx, double *out6, int reset);
+ * sumcheck, WHIR opening, host time spent waiting for device results]
+ * (no counterpart in the reference; used by the benchmark). */
+void wl_phase_times(wl_ctx *ctx, double *out7, int reset);
 
 /* Live timing of kernel classes with HIP events on the context's stream (no
  * counterpart in the reference; used by the benchmark's roofline line).

@@ -46,7 +46,7 @@ class Context:
             self.ptr = None
 
     def phase_times(self, reset=True):
-        out = (ctypes.c_double * 6)()
+        out = (ctypes.c_double * 7)()
         lib().wl_phase_times(self.ptr, out, int(reset))
         return list(out)
 

@@ -175,6 +175,50 @@ struct wl_ctx {
             }
         }
         __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        if (mbox && __atomic_load_n(&mbox[1], __ATOMIC_RELAXED) != 0)
+            throw std::runtime_error("a device wait for a host value timed out");
+        phase_ms[6] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    // ---- host values for pre-enqueued device work ----
+    // Kernels of round j + 1 are enqueued before the host has computed round j's
+    // challenge: an await_host kernel ahead of them blocks the stream until the
+    // host releases the value (a store into host-mapped memory), so the host's
+    // launch calls stay off the critical path.  Ids increase by one per value;
+    // mbox[0] holds the last released id, mbox[1] a timeout report.
+    static constexpr int MB_HEAD = 64, MB_SLOTS = 256, MB_WORDS = 64;
+    uint32_t *mbox = nullptr, *mbox_dev = nullptr, *args_dev = nullptr;
+    uint32_t mbox_next = 0;
+    uint64_t await_ticks = 0;
+    struct HostSlot { uint32_t id; uint32_t *host; uint32_t *dev; };
+    void ensure_mbox() {
+        if (mbox) return;
+        size_t words = MB_HEAD + (size_t)MB_SLOTS * MB_WORDS;
+        HIP_CHECK(hipHostMalloc((void **)&mbox, words * 4, hipHostMallocMapped | hipHostMallocCoherent));
+        memset(mbox, 0, words * 4);
+        HIP_CHECK(hipHostGetDevicePointer((void **)&mbox_dev, mbox, 0));
+        HIP_CHECK(hipMalloc((void **)&args_dev, (size_t)MB_SLOTS * MB_WORDS * 4));
+        int khz = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+        await_ticks = (uint64_t)(khz > 0 ? khz : 100000) * 1000ull * 10ull;  // 10 s
+    }
+    // enqueue a wait for the next host value (<= MB_WORDS words), readable by later
+    // kernels at .dev
+    HostSlot await_value(int words) {
+        ensure_mbox();
+        if (words > MB_WORDS) throw std::runtime_error("host value too large");
+        uint32_t id = ++mbox_next;
+        size_t off = MB_HEAD + (size_t)(id % MB_SLOTS) * MB_WORDS;
+        HostSlot sl{id, mbox + off, args_dev + (off - MB_HEAD)};
+        wk::await_host(mbox_dev, id, mbox_dev + off, sl.dev, words, await_ticks, mbox_dev + 1, stream);
+        return sl;
+    }
+    void release(const HostSlot &sl, const void *data, size_t bytes) {
+        memcpy(sl.host, data, bytes);
+        __atomic_store_n(&mbox[0], sl.id, __ATOMIC_RELEASE);
+    }
+    // error paths: let every enqueued wait finish
+    void release_all() {
+        if (mbox) __atomic_store_n(&mbox[0], mbox_next, __ATOMIC_RELEASE);
     }
     // copy `bytes` of device memory to dst through the mapped buffer (waits)
     void fetch(void *dst, const void *src, size_t bytes) {
@@ -288,6 +332,22 @@ inline std::vector<EF> reduce_many(wl_ctx *c, const EF *parts, int n_items, int 
     return r;
 }
 inline EF reduce_one(wl_ctx *c, const EF *parts, int n_parts) { return reduce_many(c, parts, 1, n_parts)[0]; }
+// reduce_many in two halves: enqueue the reduction and its publish (n_items <= 4),
+// then, later, wait for it.  Publishes share the mapped buffer, so a second one
+// may be enqueued only behind an await_value the host releases after collecting
+// the first.
+inline uint32_t reduce_async(wl_ctx *c, const EF *parts, int n_items, int n_parts) {
+    c->ensure_mapped();
+    uint32_t s = ++c->seq;
+    wk::reduce_publish(parts, n_items, n_parts, (EF *)c->mapped_dev, c->flag_dev(), s, c->stream);
+    return s;
+}
+inline std::vector<EF> collect(wl_ctx *c, uint32_t s, int n_items) {
+    c->wait_flag(s);
+    std::vector<EF> r(n_items);
+    memcpy(r.data(), c->mapped, (size_t)n_items * sizeof(EF));
+    return r;
+}
 // Stream-ordered upload (no synchronization; device buffers are recycled in
 // stream order, so reuse by later work is safe).
 inline DBuf<EF> upload_ef(wl_ctx *c, const std::vector<EF> &v) {

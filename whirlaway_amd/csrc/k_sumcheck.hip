@@ -358,6 +358,54 @@ void fold_cols_strided(EF *U, int n_cols, size_t stride, int l, const EF *c, hip
 
 // column c < n_cols: fold by (1 - ch, ch); the extra row blockIdx.y == n_cols
 // halves the eq table for the next round.
+// Wait for a value the host writes after this kernel was enqueued: thread 0 polls
+// *flag (host-mapped, system scope) until it reaches id, then the block copies
+// the words into device memory for the kernels queued behind it.  The wait is
+// bounded by timeout_ticks of the constant-rate clock; on expiry id is written
+// to *err and nothing is copied.
+__global__ void k_await_host(const uint32_t *flag, uint32_t id, const uint32_t *src, uint32_t *dst, int words,
+                             uint64_t timeout_ticks, uint32_t *err) {
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        const uint64_t t0 = wall_clock64();
+        int good = 1;
+        while ((int)(__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) - id) < 0) {
+            __builtin_amdgcn_s_sleep(8);
+            if (wall_clock64() - t0 > timeout_ticks) { good = 0; break; }
+        }
+        ok = good;
+    }
+    __syncthreads();
+    if (!ok) {
+        if (threadIdx.x == 0) __hip_atomic_store(err, id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return;
+    }
+    for (int i = threadIdx.x; i < words; i += blockDim.x)
+        dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+void await_host(const uint32_t *flag, uint32_t id, const uint32_t *src, uint32_t *dst, int words,
+                uint64_t timeout_ticks, uint32_t *err, hipStream_t s) {
+    k_await_host<<<1, 64, 0, s>>>(flag, id, src, dst, words, timeout_ticks, err);
+}
+
+__global__ void k_fold_cols_p(EF *U, size_t stride, int l, int n_cols, const EF *chp, EF *E) {
+    const size_t h = (size_t)1 << (l - 1);
+    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= h) return;
+    if ((int)blockIdx.y == n_cols) {
+        size_t q = h >> 1;
+        if (x < q) E[x] = eadd(E[x], E[x + q]);
+        return;
+    }
+    const EF ch = *chp;
+    EF *col = U + (size_t)blockIdx.y * stride;
+    EF a = col[x];
+    col[x] = eadd(a, emul(ch, esub(col[x + h], a)));
+}
+void fold_cols_p(EF *U, int n_cols, size_t stride, int l, const EF *c, EF *E, hipStream_t s) {
+    size_t h = (size_t)1 << (l - 1);
+    k_fold_cols_p<<<dim3(nblk(h), n_cols + (E && h > 1 ? 1 : 0)), 256, 0, s>>>(U, stride, l, n_cols, c, E);
+}
 __global__ void k_fold_cols_v(EF *U, size_t stride, int l, int n_cols, EF ch, EF *E) {
     const size_t h = (size_t)1 << (l - 1);
     const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -509,9 +557,11 @@ int prod_round(const EF *A, const EF *B, int l, EF *parts, hipStream_t s) {
 // by the previous challenge c when `fold` is set (the tables then hold 4h
 // entries and are folded in place to 2h: thread x owns x, x+h, x+2h, x+3h), so
 // each table is read once per round.  parts as k_prod_round.
-__global__ void __launch_bounds__(256) k_prod_round_f(EF *A, EF *B, size_t h, int fold, EF c, EF *parts) {
+__global__ void __launch_bounds__(256) k_prod_round_f(EF *A, EF *B, size_t h, int fold, const EF *cp,
+                                                     EF *parts) {
     EF a0 = ezero(), a2 = ezero();
     const size_t st = (size_t)gridDim.x * blockDim.x;
+    const EF c = fold ? *cp : ezero();
     for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < h; x += st) {
         EF al = A[x], ah = A[x + h], bl = B[x], bh = B[x + h];
         if (fold) {
@@ -535,11 +585,23 @@ __global__ void __launch_bounds__(256) k_prod_round_f(EF *A, EF *B, size_t h, in
         parts[gridDim.x + blockIdx.x] = t2;
     }
 }
-int prod_round_fused(EF *A, EF *B, int l, bool fold, EF c, EF *parts, hipStream_t s) {
+int prod_round_fused(EF *A, EF *B, int l, bool fold, const EF *c, EF *parts, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
     unsigned nb = dot_blocks(h);
     k_prod_round_f<<<nb, 256, 0, s>>>(A, B, h, fold ? 1 : 0, c, parts);
     return (int)nb;
+}
+__global__ void k_fold_ef2_p(EF *A, EF *B, size_t h, const EF *cp) {
+    size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= h) return;
+    const EF c = *cp;
+    EF *T = blockIdx.y ? B : A;
+    EF a = T[x];
+    T[x] = eadd(a, emul(c, esub(T[x + h], a)));
+}
+void fold_ef2_p(EF *A, EF *B, int l, const EF *c, hipStream_t s) {
+    size_t h = (size_t)1 << (l - 1);
+    k_fold_ef2_p<<<dim3(nblk(h), B ? 2 : 1), 256, 0, s>>>(A, B, h, c);
 }
 __global__ void k_fold_ef(EF *A, size_t h, const EF *c) {
     size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -71,6 +71,14 @@ int zc_round_wave_blocks(int l);
 void fold_cols_strided(EF *U, int n_cols, size_t stride, int l, const EF *c, hipStream_t s);
 // by-value challenge; also halves the eq table E (length 2^(l-1)) when E != nullptr
 void fold_cols_v(EF *U, int n_cols, size_t stride, int l, EF c, EF *E, hipStream_t s);
+// the same with the challenge in device memory
+void fold_cols_p(EF *U, int n_cols, size_t stride, int l, const EF *c, EF *E, hipStream_t s);
+// Block the stream until the host sets *flag >= id (host-mapped, written after
+// enqueue), then copy `words` words from the host-mapped src to device dst.
+// Bounded: after timeout_ticks of the constant-rate clock it writes id to *err
+// (host-mapped) and returns without copying.
+void await_host(const uint32_t *flag, uint32_t id, const uint32_t *src, uint32_t *dst, int words,
+                uint64_t timeout_ticks, uint32_t *err, hipStream_t s);
 // down inner sums: parts[c][j] = sum_{x'} eqr[x'] * down_c(j*M + x'), M = 2^(log_n-skips)
 void zc_down_parts(const F *trace, int log_n, int skips, const EF *eqr, EF *parts, hipStream_t s);
 // out[c] = sum_j S[j] * parts[c][j]
@@ -96,8 +104,10 @@ int whir_T_chunks(int m, int k);
 // parts[0][b] = sum_{x<h} A[x] B[x], parts[1][b] = sum (2A[x+h]-A[x])(2B[x+h]-B[x]); h = 2^(l-1)
 int prod_round(const EF *A, const EF *B, int l, EF *parts, hipStream_t s);
 // The same round on tables of length 2^l, first folding them in place from
-// 2^(l+1) by c when fold is set.
-int prod_round_fused(EF *A, EF *B, int l, bool fold, EF c, EF *parts, hipStream_t s);
+// 2^(l+1) by *c (device memory) when fold is set.
+int prod_round_fused(EF *A, EF *B, int l, bool fold, const EF *c, EF *parts, hipStream_t s);
+// fold_ef2_v with the challenge in device memory
+void fold_ef2_p(EF *A, EF *B, int l, const EF *c, hipStream_t s);
 // in-place fold of one EF table by (1-c, c)
 void fold_ef(EF *A, int l, const EF *c, hipStream_t s);
 // fold one or two EF tables by a by-value challenge (B may be nullptr)

@@ -196,35 +196,54 @@ void add_constraints_w(wl_ctx *c, EF *W, int m, const std::vector<Constraint> &c
 
 struct SumcheckState {
     DBuf<EF> f, W;
-    int m = 0;       // tables hold 2^m entries (2^(m+1) while a fold is pending)
+    int m = 0;  // tables hold 2^m entries
     EF sum;
-    bool pending = false;  // fold by pc not yet applied (the next round kernel applies it)
-    EF pc;
 };
 
-void flush_fold(wl_ctx *c, SumcheckState &s) {
-    if (!s.pending) return;
-    wk::fold_ef2_v(s.f, s.W, s.m + 1, s.pc, c->stream);
-    s.pending = false;
-}
-
-// GPU sumcheck round pair (f, W), evaluations at 0,1,2 sent (whir-p3 convention).
-// The fold by the sampled challenge is left pending for the next round's kernel.
-EF whir_sc_round(wl_ctx *c, Transcript &ts, const P2Dev *pdev, SumcheckState &s, int pow_bits) {
-    int nb = wk::dot_blocks((size_t)1 << (s.m - 1));
-    DBuf<EF> parts(c->pool, 2 * (size_t)nb);
-    wk::prod_round_fused(s.f, s.W, s.m, s.pending, s.pc, parts, c->stream);
-    std::vector<EF> h02 = reduce_many(c, parts, 2, nb);
-    s.pending = false;
-    EF h[3] = {h02[0], esub(s.sum, h02[0]), h02[1]};
-    ts.add_ef(h, 3);
-    grind(c, ts, pdev, pow_bits);
-    EF r = ts.sample_ef();
-    s.sum = wh::interp3(h, r);
-    s.pending = true;
-    s.pc = r;
-    s.m--;
-    return r;
+// kn GPU sumcheck rounds on (f, W), evaluations at 0, 1, 2 sent (whir-p3
+// convention).  Every round after the first is fused with the fold by the
+// previous challenge and, unless grinding sits between a round and its
+// challenge, enqueued behind a device wait for that challenge before the host
+// computes it.  The tables end folded by the last challenge.
+std::vector<EF> whir_sc_rounds(wl_ctx *c, Transcript &ts, const P2Dev *pdev, SumcheckState &s, int kn,
+                               int pow_bits) {
+    std::vector<EF> rs;
+    auto enqueue = [&](int m, bool fold, const EF *cdev, DBuf<EF> &parts) {
+        int nb = wk::dot_blocks((size_t)1 << (m - 1));
+        parts = DBuf<EF>(c->pool, 2 * (size_t)nb);
+        wk::prod_round_fused(s.f, s.W, m, fold, cdev, parts, c->stream);
+        return reduce_async(c, parts, 2, nb);
+    };
+    const bool pipe = pow_bits <= 0;  // grinding launches device work before the challenge exists
+    DBuf<EF> pcur, pnext;
+    uint32_t seq = enqueue(s.m, false, nullptr, pcur);
+    for (int k = 0; k < kn; k++) {
+        wl_ctx::HostSlot slot{};
+        uint32_t seq_next = 0;
+        if (pipe) {
+            slot = c->await_value(8);
+            if (k + 1 < kn) seq_next = enqueue(s.m - 1, true, (const EF *)slot.dev, pnext);
+            else wk::fold_ef2_p(s.f, s.W, s.m, (const EF *)slot.dev, c->stream);
+        }
+        std::vector<EF> h02 = collect(c, seq, 2);
+        EF h[3] = {h02[0], esub(s.sum, h02[0]), h02[1]};
+        ts.add_ef(h, 3);
+        grind(c, ts, pdev, pow_bits);
+        EF r = ts.sample_ef();
+        s.sum = wh::interp3(h, r);
+        if (pipe) {
+            c->release(slot, &r, sizeof(EF));
+        } else {
+            DBuf<EF> dr = upload_ef(c, std::vector<EF>(1, r));
+            if (k + 1 < kn) seq_next = enqueue(s.m - 1, true, dr.p, pnext);
+            else wk::fold_ef2_p(s.f, s.W, s.m, dr.p, c->stream);
+        }
+        rs.push_back(r);
+        seq = seq_next;
+        std::swap(pcur, pnext);
+        s.m--;
+    }
+    return rs;
 }
 
 // Host sumcheck round on small tables (final rounds).
@@ -428,9 +447,7 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
         }
         add_constraints_w(c, s.W, m, fresh, false);
         for (auto &k : fresh) cs.push_back(k);
-        rf.clear();
-        for (int k = 0; k < kn; k++) rf.push_back(whir_sc_round(c, ts, pdev, s, rp.folding_pow_bits));
-        flush_fold(c, s);
+        rf = whir_sc_rounds(c, ts, pdev, s, kn, rp.folding_pow_bits);
         for (auto &k : cs) fold_constraint(k, m, rf);
         prev = &nc;
         cur ^= 1;
@@ -468,7 +485,9 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
 
     auto tp = std::chrono::steady_clock::now();
     auto lap = [&](int i) {
+        auto tw = std::chrono::steady_clock::now();
         HIP_CHECK(hipStreamSynchronize(c->stream));
+        c->phase_ms[6] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
         auto now = std::chrono::steady_clock::now();
         c->phase_ms[i] += std::chrono::duration<double, std::milli>(now - tp).count();
         tp = now;
@@ -542,17 +561,25 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     lap(1);
     // ---- zerocheck rounds 1 .. n - skips ----
     int l = n - skips;
-    // eq(zc[1 + j ..], .) for round j; each fold launch halves it for the next round
+    // eq(zc[1 + j ..], .) for round j; each fold launch halves it for the next round.
+    // Pipelined: the fold by round j's challenge and round j + 1 are enqueued
+    // behind a device wait for that challenge before the host computes it.
     DBuf<EF> eq = eq_dev(c, std::vector<EF>(zc.begin() + 2, zc.end()), eone());
+    auto enqueue_round = [&](int ll, DBuf<EF> &parts) {
+        size_t nbx = (size_t)wk::zc_round_wave_blocks(ll);
+        parts = DBuf<EF>(c->pool, 3 * nbx);
+        KTimer kt(c, KT_ZC_ROUND, 3.0 * (double)((size_t)1 << (ll - 1)));
+        wk::zc_round_wave(U, M, ll, t->air_dev, t->chain_dev, dap, eq, parts, c->stream);
+        return reduce_async(c, parts, 3, (int)nbx);
+    };
+    DBuf<EF> parts_cur, parts_next;
+    uint32_t seq_cur = nz > 1 ? enqueue_round(l, parts_cur) : 0;
     for (int j = 1; j < nz; j++) {
         EF r = zc[j];
-        size_t nbx = (size_t)wk::zc_round_wave_blocks(l);
-        DBuf<EF> parts(c->pool, 3 * nbx);
-        {
-            KTimer kt(c, KT_ZC_ROUND, 3.0 * (double)((size_t)1 << (l - 1)));
-            wk::zc_round_wave(U, M, l, t->air_dev, t->chain_dev, dap, eq, parts, c->stream);
-        }
-        std::vector<EF> v3 = reduce_many(c, parts, 3, (int)nbx);
+        wl_ctx::HostSlot slot = c->await_value(8);
+        wk::fold_cols_p(U, AIR_NCOLS, M, l, (const EF *)slot.dev, eq, c->stream);
+        uint32_t seq_next = j + 1 < nz ? enqueue_round(l - 1, parts_next) : 0;
+        std::vector<EF> v3 = collect(c, seq_cur, 3);
         EF e0 = emul(v3[0], missing), e2 = emul(v3[1], missing), e3 = emul(v3[2], missing);
         EF e1 = emul(esub(sum, emul(e0, esub(eone(), r))), wh::einv(r));
         std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2), fsmall(3)};
@@ -560,11 +587,13 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
         p = wh::pmul(p, {esub(eone(), r), esub(edbl(r), eone())});
         ts.add_ef(p.data(), p.size());
         EF cj = ts.sample_ef();
+        c->release(slot, &cj, sizeof(EF));
         ch.push_back(cj);
         sum = wh::peval(p, cj);
         grind(c, ts, pdev, pow_bits_sumcheck(sec, 3 + 1));
         missing = emul(eadd(emul(esub(eone(), r), esub(eone(), cj)), emul(r, cj)), missing);
-        wk::fold_cols_v(U, AIR_NCOLS, M, l, cj, eq, c->stream);
+        seq_cur = seq_next;
+        std::swap(parts_cur, parts_next);
         l--;
     }
     eq.reset();
@@ -629,25 +658,34 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     // ---- inner sumcheck (degree 2, coefficients sent) ----
     std::vector<EF> ich;
     EF isum = inner_sum;
-    bool pend = false;  // fold by the previous challenge, applied inside the next round kernel
-    EF pc = ezero();
+    // pipelined as the zerocheck: round j + 1 (fused with the fold by c_j) waits
+    // on the device for c_j
+    auto enqueue_inner = [&](int ll, bool fold, const EF *cdev, DBuf<EF> &parts) {
+        int nb = wk::dot_blocks((size_t)1 << (ll - 1));
+        parts = DBuf<EF>(c->pool, 2 * (size_t)nb);
+        wk::prod_round_fused(Mt, B, ll, fold, cdev, parts, c->stream);
+        return reduce_async(c, parts, 2, nb);
+    };
+    DBuf<EF> iparts_cur, iparts_next;
+    uint32_t iseq = enqueue_inner(n, false, nullptr, iparts_cur);
     for (int j = 0; j < n; j++) {
         int ll = n - j;
-        int nb = wk::dot_blocks((size_t)1 << (ll - 1));
-        DBuf<EF> parts(c->pool, 2 * (size_t)nb);
-        wk::prod_round_fused(Mt, B, ll, pend, pc, parts, c->stream);
-        std::vector<EF> h02 = reduce_many(c, parts, 2, nb);
+        wl_ctx::HostSlot slot = c->await_value(8);
+        uint32_t iseq_next = 0;
+        if (j + 1 < n) iseq_next = enqueue_inner(ll - 1, true, (const EF *)slot.dev, iparts_next);
+        else wk::fold_ef2_p(Mt, B, 1, (const EF *)slot.dev, c->stream);
+        std::vector<EF> h02 = collect(c, iseq, 2);
         std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2)};
         std::vector<EF> p = wh::lagrange(xs, {h02[0], esub(isum, h02[0]), h02[1]});
         ts.add_ef(p.data(), p.size());
         EF cj = ts.sample_ef();
+        c->release(slot, &cj, sizeof(EF));
         ich.push_back(cj);
         isum = wh::peval(p, cj);
         grind(c, ts, pdev, pow_bits_sumcheck(sec, 2));
-        pend = true;
-        pc = cj;
+        iseq = iseq_next;
+        std::swap(iparts_cur, iparts_next);
     }
-    wk::fold_ef2_v(Mt, B, 1, pc, c->stream);
     EF packed_value;
     c->d2h(&packed_value, B.p, 1);
     Mt.reset();
@@ -711,6 +749,7 @@ wl_ctx *wl_create(int device) {
 void wl_destroy(wl_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    c->release_all();
     (void)hipStreamSynchronize(c->stream);
     for (auto &kv : c->tw) (void)hipFree(kv.second);
     for (auto &kv : c->ntt_tab) (void)hipFree(kv.second);
@@ -718,6 +757,8 @@ void wl_destroy(wl_ctx *c) {
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->ring) (void)hipHostFree(c->ring);
     if (c->mapped) (void)hipHostFree(c->mapped);
+    if (c->mbox) (void)hipHostFree(c->mbox);
+    if (c->args_dev) (void)hipFree(c->args_dev);
     for (auto &e : c->kt_free) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (auto &v : c->kt_pairs)
         for (auto &e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
@@ -774,7 +815,13 @@ static int prove_common(wl_table *t, const wl_air_settings *st, const uint32_t *
     P2Dev ph;
     load_p2(p16c, p24c, ph);
     DBuf<P2Dev> pd = upload_p2(c, ph);
-    std::vector<uint32_t> proof = air_prove(t, st, ph, pd.p, packed);
+    std::vector<uint32_t> proof;
+    try {
+        proof = air_prove(t, st, ph, pd.p, packed);
+    } catch (...) {
+        c->release_all();  // device waits for host values must not outlive the failure
+        throw;
+    }
     if (proof_len) *proof_len = proof.size();
     if (proof_out) {
         if (proof_cap < proof.size()) { g_err = "proof buffer too small"; return WL_ERR_BUFFER; }
@@ -897,8 +944,8 @@ int wl_kernel_timing_read(wl_ctx *c, int kernel_class, long *launches, double *t
     }
 }
 
-void wl_phase_times(wl_ctx *c, double *out6, int reset) {
-    for (int i = 0; i < 6; i++) out6[i] = c->phase_ms[i];
+void wl_phase_times(wl_ctx *c, double *out7, int reset) {
+    for (int i = 0; i < 7; i++) out7[i] = c->phase_ms[i];
     if (reset) for (int i = 0; i < 8; i++) c->phase_ms[i] = 0;
 }
 
