@@ -212,6 +212,12 @@ struct wl_ctx {
         wk::await_host(mbox_dev, id, mbox_dev + off, sl.dev, words, await_ticks, mbox_dev + 1, stream);
         return sl;
     }
+    HostSlot next_slot() {
+        ensure_mbox();
+        uint32_t id = ++mbox_next;
+        size_t off = MB_HEAD + (size_t)(id % MB_SLOTS) * MB_WORDS;
+        return HostSlot{id, mbox + off, args_dev + (off - MB_HEAD)};
+    }
     void release(const HostSlot &sl, const void *data, size_t bytes) {
         memcpy(sl.host, data, bytes);
         __atomic_store_n(&mbox[0], sl.id, __ATOMIC_RELEASE);
@@ -341,6 +347,18 @@ inline uint32_t reduce_async(wl_ctx *c, const EF *parts, int n_items, int n_part
     uint32_t s = ++c->seq;
     wk::reduce_publish(parts, n_items, n_parts, (EF *)c->mapped_dev, c->flag_dev(), s, c->stream);
     return s;
+}
+// reduce_async whose kernel then waits for the host's next value (the returned
+// slot, released after collect(seq))
+inline wl_ctx::HostSlot reduce_async_await(wl_ctx *c, const EF *parts, int n_items, int n_parts, int words,
+                                           uint32_t &seq) {
+    c->ensure_mapped();
+    wl_ctx::HostSlot sl = c->next_slot();
+    seq = ++c->seq;
+    size_t off = (size_t)(sl.host - c->mbox);
+    wk::reduce_publish_await(parts, n_items, n_parts, (EF *)c->mapped_dev, c->flag_dev(), seq, c->mbox_dev, sl.id,
+                             c->mbox_dev + off, sl.dev, words, c->await_ticks, c->mbox_dev + 1, c->stream);
+    return sl;
 }
 inline std::vector<EF> collect(wl_ctx *c, uint32_t s, int n_items) {
     c->wait_flag(s);

@@ -92,6 +92,56 @@ __global__ void k_reduce_publish(const EF *in, int n_items, int n_parts, EF *dst
         *flag = seq;
     }
 }
+// thread 0 waits (bounded) until *aflag >= id, then the block copies `words` words
+// of the host value into device memory; on timeout id goes to *err instead
+__device__ void await_copy(const uint32_t *aflag, uint32_t id, const uint32_t *src, uint32_t *dst, int words,
+                           uint64_t timeout_ticks, uint32_t *err) {
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        const uint64_t t0 = wall_clock64();
+        int good = 1;
+        while ((int)(__hip_atomic_load(aflag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) - id) < 0) {
+            __builtin_amdgcn_s_sleep(8);
+            if (wall_clock64() - t0 > timeout_ticks) { good = 0; break; }
+        }
+        ok = good;
+    }
+    __syncthreads();
+    if (!ok) {
+        if (threadIdx.x == 0) __hip_atomic_store(err, id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return;
+    }
+    for (int i = threadIdx.x; i < words; i += blockDim.x)
+        dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+// k_reduce_publish followed by the wait for the host's answer (the next round's
+// challenge), in one launch: a sumcheck round then costs one kernel besides its
+// evaluation kernel.
+__global__ void k_reduce_publish_await(const EF *in, int n_items, int n_parts, EF *dst, volatile uint32_t *flag,
+                                       uint32_t seq, const uint32_t *aflag, uint32_t id, const uint32_t *src,
+                                       uint32_t *adst, int words, uint64_t timeout_ticks, uint32_t *err) {
+    __shared__ EF lds[16];
+    for (int it = 0; it < n_items; it++) {
+        const EF *p = in + (size_t)it * n_parts;
+        EF acc = ezero();
+        for (int i = threadIdx.x; i < n_parts; i += blockDim.x) acc = eadd(acc, p[i]);
+        EF t = block_sum(acc, lds);
+        if (threadIdx.x == 0) dst[it] = t;
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        *flag = seq;
+    }
+    await_copy(aflag, id, src, adst, words, timeout_ticks, err);
+}
+void reduce_publish_await(const EF *in, int n_items, int n_parts, EF *host_dst, volatile uint32_t *flag,
+                          uint32_t seq, const uint32_t *aflag, uint32_t id, const uint32_t *src, uint32_t *dst,
+                          int words, uint64_t timeout_ticks, uint32_t *err, hipStream_t s) {
+    k_reduce_publish_await<<<1, 256, 0, s>>>(in, n_items, n_parts, host_dst, flag, seq, aflag, id, src, dst, words,
+                                             timeout_ticks, err);
+}
 void reduce_publish(const EF *in, int n_items, int n_parts, EF *host_dst, volatile uint32_t *flag,
                     uint32_t seq, hipStream_t s) {
     k_reduce_publish<<<1, 256, 0, s>>>(in, n_items, n_parts, host_dst, flag, seq);
@@ -365,23 +415,7 @@ void fold_cols_strided(EF *U, int n_cols, size_t stride, int l, const EF *c, hip
 // to *err and nothing is copied.
 __global__ void k_await_host(const uint32_t *flag, uint32_t id, const uint32_t *src, uint32_t *dst, int words,
                              uint64_t timeout_ticks, uint32_t *err) {
-    __shared__ int ok;
-    if (threadIdx.x == 0) {
-        const uint64_t t0 = wall_clock64();
-        int good = 1;
-        while ((int)(__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) - id) < 0) {
-            __builtin_amdgcn_s_sleep(8);
-            if (wall_clock64() - t0 > timeout_ticks) { good = 0; break; }
-        }
-        ok = good;
-    }
-    __syncthreads();
-    if (!ok) {
-        if (threadIdx.x == 0) __hip_atomic_store(err, id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        return;
-    }
-    for (int i = threadIdx.x; i < words; i += blockDim.x)
-        dst[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    await_copy(flag, id, src, dst, words, timeout_ticks, err);
 }
 void await_host(const uint32_t *flag, uint32_t id, const uint32_t *src, uint32_t *dst, int words,
                 uint64_t timeout_ticks, uint32_t *err, hipStream_t s) {

@@ -37,6 +37,11 @@ void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_
 void reduce_publish(const EF *in, int n_items, int n_parts, EF *host_dst, volatile uint32_t *flag,
                     uint32_t seq, hipStream_t s);
 
+// reduce_publish, then (same launch) wait for the host's value id and copy it to
+// dst, as await_host
+void reduce_publish_await(const EF *in, int n_items, int n_parts, EF *host_dst, volatile uint32_t *flag,
+                          uint32_t seq, const uint32_t *aflag, uint32_t id, const uint32_t *src, uint32_t *dst,
+                          int words, uint64_t timeout_ticks, uint32_t *err, hipStream_t s);
 // eq(point, .) over n <= 32 variables, big-endian, times scale (point by value).
 struct EqPoint { EF p[32]; EF scale; int n; };
 void eq_table_v(const EqPoint &pt, EF *out, hipStream_t s);

@@ -208,21 +208,24 @@ struct SumcheckState {
 std::vector<EF> whir_sc_rounds(wl_ctx *c, Transcript &ts, const P2Dev *pdev, SumcheckState &s, int kn,
                                int pow_bits) {
     std::vector<EF> rs;
-    auto enqueue = [&](int m, bool fold, const EF *cdev, DBuf<EF> &parts) {
+    const bool pipe = pow_bits <= 0;  // grinding launches device work before the challenge exists
+    auto enqueue = [&](int m, const EF *cdev, DBuf<EF> &parts, uint32_t &seq) {
         int nb = wk::dot_blocks((size_t)1 << (m - 1));
         parts = DBuf<EF>(c->pool, 2 * (size_t)nb);
-        wk::prod_round_fused(s.f, s.W, m, fold, cdev, parts, c->stream);
-        return reduce_async(c, parts, 2, nb);
+        wk::prod_round_fused(s.f, s.W, m, cdev != nullptr, cdev, parts, c->stream);
+        wl_ctx::HostSlot sl{};
+        if (pipe) sl = reduce_async_await(c, parts, 2, nb, 8, seq);
+        else seq = reduce_async(c, parts, 2, nb);
+        return sl;
     };
-    const bool pipe = pow_bits <= 0;  // grinding launches device work before the challenge exists
     DBuf<EF> pcur, pnext;
-    uint32_t seq = enqueue(s.m, false, nullptr, pcur);
+    uint32_t seq = 0;
+    wl_ctx::HostSlot slot = enqueue(s.m, nullptr, pcur, seq);
     for (int k = 0; k < kn; k++) {
-        wl_ctx::HostSlot slot{};
+        wl_ctx::HostSlot slot_next{};
         uint32_t seq_next = 0;
         if (pipe) {
-            slot = c->await_value(8);
-            if (k + 1 < kn) seq_next = enqueue(s.m - 1, true, (const EF *)slot.dev, pnext);
+            if (k + 1 < kn) slot_next = enqueue(s.m - 1, (const EF *)slot.dev, pnext, seq_next);
             else wk::fold_ef2_p(s.f, s.W, s.m, (const EF *)slot.dev, c->stream);
         }
         std::vector<EF> h02 = collect(c, seq, 2);
@@ -235,11 +238,12 @@ std::vector<EF> whir_sc_rounds(wl_ctx *c, Transcript &ts, const P2Dev *pdev, Sum
             c->release(slot, &r, sizeof(EF));
         } else {
             DBuf<EF> dr = upload_ef(c, std::vector<EF>(1, r));
-            if (k + 1 < kn) seq_next = enqueue(s.m - 1, true, dr.p, pnext);
+            if (k + 1 < kn) slot_next = enqueue(s.m - 1, dr.p, pnext, seq_next);
             else wk::fold_ef2_p(s.f, s.W, s.m, dr.p, c->stream);
         }
         rs.push_back(r);
         seq = seq_next;
+        slot = slot_next;
         std::swap(pcur, pnext);
         s.m--;
     }
@@ -565,20 +569,29 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     // Pipelined: the fold by round j's challenge and round j + 1 are enqueued
     // behind a device wait for that challenge before the host computes it.
     DBuf<EF> eq = eq_dev(c, std::vector<EF>(zc.begin() + 2, zc.end()), eone());
-    auto enqueue_round = [&](int ll, DBuf<EF> &parts) {
+    // round ll's kernels: the fold by the previous challenge (from 2^(ll+1), when
+    // cdev is given), the evaluation, and the reduction + publish that then waits
+    // for this round's challenge (the returned slot)
+    auto enqueue_round = [&](int ll, const EF *cdev, DBuf<EF> &parts, uint32_t &seq) {
+        if (cdev) wk::fold_cols_p(U, AIR_NCOLS, M, ll + 1, cdev, eq, c->stream);
         size_t nbx = (size_t)wk::zc_round_wave_blocks(ll);
         parts = DBuf<EF>(c->pool, 3 * nbx);
-        KTimer kt(c, KT_ZC_ROUND, 3.0 * (double)((size_t)1 << (ll - 1)));
-        wk::zc_round_wave(U, M, ll, t->air_dev, t->chain_dev, dap, eq, parts, c->stream);
-        return reduce_async(c, parts, 3, (int)nbx);
+        {
+            KTimer kt(c, KT_ZC_ROUND, 3.0 * (double)((size_t)1 << (ll - 1)));
+            wk::zc_round_wave(U, M, ll, t->air_dev, t->chain_dev, dap, eq, parts, c->stream);
+        }
+        return reduce_async_await(c, parts, 3, (int)nbx, 8, seq);
     };
     DBuf<EF> parts_cur, parts_next;
-    uint32_t seq_cur = nz > 1 ? enqueue_round(l, parts_cur) : 0;
+    uint32_t seq_cur = 0;
+    wl_ctx::HostSlot slot_cur{};
+    if (nz > 1) slot_cur = enqueue_round(l, nullptr, parts_cur, seq_cur);
     for (int j = 1; j < nz; j++) {
         EF r = zc[j];
-        wl_ctx::HostSlot slot = c->await_value(8);
-        wk::fold_cols_p(U, AIR_NCOLS, M, l, (const EF *)slot.dev, eq, c->stream);
-        uint32_t seq_next = j + 1 < nz ? enqueue_round(l - 1, parts_next) : 0;
+        uint32_t seq_next = 0;
+        wl_ctx::HostSlot slot_next{};
+        if (j + 1 < nz) slot_next = enqueue_round(l - 1, (const EF *)slot_cur.dev, parts_next, seq_next);
+        else wk::fold_cols_p(U, AIR_NCOLS, M, l, (const EF *)slot_cur.dev, eq, c->stream);
         std::vector<EF> v3 = collect(c, seq_cur, 3);
         EF e0 = emul(v3[0], missing), e2 = emul(v3[1], missing), e3 = emul(v3[2], missing);
         EF e1 = emul(esub(sum, emul(e0, esub(eone(), r))), wh::einv(r));
@@ -587,12 +600,13 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
         p = wh::pmul(p, {esub(eone(), r), esub(edbl(r), eone())});
         ts.add_ef(p.data(), p.size());
         EF cj = ts.sample_ef();
-        c->release(slot, &cj, sizeof(EF));
+        c->release(slot_cur, &cj, sizeof(EF));
         ch.push_back(cj);
         sum = wh::peval(p, cj);
         grind(c, ts, pdev, pow_bits_sumcheck(sec, 3 + 1));
         missing = emul(eadd(emul(esub(eone(), r), esub(eone(), cj)), emul(r, cj)), missing);
         seq_cur = seq_next;
+        slot_cur = slot_next;
         std::swap(parts_cur, parts_next);
         l--;
     }
@@ -660,30 +674,32 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     EF isum = inner_sum;
     // pipelined as the zerocheck: round j + 1 (fused with the fold by c_j) waits
     // on the device for c_j
-    auto enqueue_inner = [&](int ll, bool fold, const EF *cdev, DBuf<EF> &parts) {
+    auto enqueue_inner = [&](int ll, const EF *cdev, DBuf<EF> &parts, uint32_t &seq) {
         int nb = wk::dot_blocks((size_t)1 << (ll - 1));
         parts = DBuf<EF>(c->pool, 2 * (size_t)nb);
-        wk::prod_round_fused(Mt, B, ll, fold, cdev, parts, c->stream);
-        return reduce_async(c, parts, 2, nb);
+        wk::prod_round_fused(Mt, B, ll, cdev != nullptr, cdev, parts, c->stream);
+        return reduce_async_await(c, parts, 2, nb, 8, seq);
     };
     DBuf<EF> iparts_cur, iparts_next;
-    uint32_t iseq = enqueue_inner(n, false, nullptr, iparts_cur);
+    uint32_t iseq = 0;
+    wl_ctx::HostSlot islot = enqueue_inner(n, nullptr, iparts_cur, iseq);
     for (int j = 0; j < n; j++) {
         int ll = n - j;
-        wl_ctx::HostSlot slot = c->await_value(8);
         uint32_t iseq_next = 0;
-        if (j + 1 < n) iseq_next = enqueue_inner(ll - 1, true, (const EF *)slot.dev, iparts_next);
-        else wk::fold_ef2_p(Mt, B, 1, (const EF *)slot.dev, c->stream);
+        wl_ctx::HostSlot islot_next{};
+        if (j + 1 < n) islot_next = enqueue_inner(ll - 1, (const EF *)islot.dev, iparts_next, iseq_next);
+        else wk::fold_ef2_p(Mt, B, 1, (const EF *)islot.dev, c->stream);
         std::vector<EF> h02 = collect(c, iseq, 2);
         std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2)};
         std::vector<EF> p = wh::lagrange(xs, {h02[0], esub(isum, h02[0]), h02[1]});
         ts.add_ef(p.data(), p.size());
         EF cj = ts.sample_ef();
-        c->release(slot, &cj, sizeof(EF));
+        c->release(islot, &cj, sizeof(EF));
         ich.push_back(cj);
         isum = wh::peval(p, cj);
         grind(c, ts, pdev, pow_bits_sumcheck(sec, 2));
         iseq = iseq_next;
+        islot = islot_next;
         std::swap(iparts_cur, iparts_next);
     }
     EF packed_value;
