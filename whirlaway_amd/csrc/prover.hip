@@ -680,30 +680,56 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
         wk::prod_round_fused(Mt, B, ll, cdev != nullptr, cdev, parts, c->stream);
         return reduce_async_await(c, parts, 2, nb, 8, seq);
     };
+    // the last rounds run on the host: from tables of 2^HOST_TAIL entries a round
+    // costs the host less than one device round trip
+    const int HOST_TAIL = 9;
+    const int gpu_rounds = n > HOST_TAIL ? n - HOST_TAIL : 0;
+    auto inner_round = [&](const EF &h0, const EF &h2) {
+        std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2)};
+        std::vector<EF> p = wh::lagrange(xs, {h0, esub(isum, h0), h2});
+        ts.add_ef(p.data(), p.size());
+        EF cj = ts.sample_ef();
+        ich.push_back(cj);
+        isum = wh::peval(p, cj);
+        return std::make_pair(cj, p);
+    };
     DBuf<EF> iparts_cur, iparts_next;
     uint32_t iseq = 0;
-    wl_ctx::HostSlot islot = enqueue_inner(n, nullptr, iparts_cur, iseq);
-    for (int j = 0; j < n; j++) {
+    wl_ctx::HostSlot islot{};
+    if (gpu_rounds > 0) islot = enqueue_inner(n, nullptr, iparts_cur, iseq);
+    for (int j = 0; j < gpu_rounds; j++) {
         int ll = n - j;
         uint32_t iseq_next = 0;
         wl_ctx::HostSlot islot_next{};
-        if (j + 1 < n) islot_next = enqueue_inner(ll - 1, (const EF *)islot.dev, iparts_next, iseq_next);
-        else wk::fold_ef2_p(Mt, B, 1, (const EF *)islot.dev, c->stream);
+        if (j + 1 < gpu_rounds) islot_next = enqueue_inner(ll - 1, (const EF *)islot.dev, iparts_next, iseq_next);
+        else wk::fold_ef2_p(Mt, B, ll, (const EF *)islot.dev, c->stream);
         std::vector<EF> h02 = collect(c, iseq, 2);
-        std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2)};
-        std::vector<EF> p = wh::lagrange(xs, {h02[0], esub(isum, h02[0]), h02[1]});
-        ts.add_ef(p.data(), p.size());
-        EF cj = ts.sample_ef();
+        EF cj = inner_round(h02[0], h02[1]).first;
         c->release(islot, &cj, sizeof(EF));
-        ich.push_back(cj);
-        isum = wh::peval(p, cj);
         grind(c, ts, pdev, pow_bits_sumcheck(sec, 2));
         iseq = iseq_next;
         islot = islot_next;
         std::swap(iparts_cur, iparts_next);
     }
-    EF packed_value;
-    c->d2h(&packed_value, B.p, 1);
+    const size_t tail = (size_t)1 << (n - gpu_rounds);
+    std::vector<EF> hM(tail), hB(tail);
+    c->d2h(hM.data(), Mt.p, tail);
+    c->d2h(hB.data(), B.p, tail);
+    for (size_t len = tail; len > 1; len >>= 1) {
+        size_t h = len / 2;
+        EF h0 = ezero(), h2 = ezero();
+        for (size_t i = 0; i < h; i++) {
+            h0 = eadd(h0, emul(hM[i], hB[i]));
+            h2 = eadd(h2, emul(esub(edbl(hM[i + h]), hM[i]), esub(edbl(hB[i + h]), hB[i])));
+        }
+        EF cj = inner_round(h0, h2).first;
+        grind(c, ts, pdev, pow_bits_sumcheck(sec, 2));
+        for (size_t i = 0; i < h; i++) {
+            hM[i] = eadd(hM[i], emul(cj, esub(hM[i + h], hM[i])));
+            hB[i] = eadd(hB[i], emul(cj, esub(hB[i + h], hB[i])));
+        }
+    }
+    EF packed_value = hB[0];
     Mt.reset();
     B.reset();
     lap(4);
