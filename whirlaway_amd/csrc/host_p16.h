@@ -61,17 +61,39 @@ struct HostP16 {
             hi = vcube(vadd(hi, ld(k.ext_init[r] + 8)));
             mds_light(lo, hi);
         }
-        for (int r = 0; r < 20; r++) {
-            kb::F s0{(uint32_t)_mm256_cvtsi256_si32(lo)};
-            s0 = kb::fcube(kb::fadd(s0, k.internal[r]));
-            lo = _mm256_blend_epi32(lo, _mm256_set1_epi32((int)s0.v), 0x01);
-            __m256i u = vadd(lo, hi);
+        // Partial rounds with lane 0 kept as a scalar: with x the state entering a
+        // round, y = (x0 + rc)^3, S = sum_{i>=1} x_i and s = S + y, the next state is
+        // x0' = s + d0 y and x_i' = s + d_i x_i (i >= 1), and
+        // S' = 15 s + sum_{i>=1} d_i x_i.  The vector products d_i x_i and their sum
+        // depend only on x, so they overlap the scalar S-box chain.
+        const __m256i lane0 = _mm256_setr_epi32(-1, 0, 0, 0, 0, 0, 0, 0);
+        auto hsum_rest = [&](__m256i a, __m256i b) {  // sum of lanes 1..15 of (a, b)
+            __m256i u = vadd(_mm256_andnot_si256(lane0, a), b);
             u = vadd(u, _mm256_permute2x128_si256(u, u, 0x01));
             u = vadd(u, _mm256_shuffle_epi32(u, 0x4E));
-            u = vadd(u, _mm256_shuffle_epi32(u, 0xB1));  // every lane: the state sum
-            lo = vadd(u, vmul(lo, dlo));
-            hi = vadd(u, vmul(hi, dhi));
+            u = vadd(u, _mm256_shuffle_epi32(u, 0xB1));
+            return kb::F{(uint32_t)_mm256_cvtsi256_si32(u)};
+        };
+        const kb::F fifteen = kb::to_mont(15), d0{kb::DIAG.d16[0]};
+        kb::F x0{(uint32_t)_mm256_cvtsi256_si32(lo)};
+        kb::F rest = hsum_rest(lo, hi);
+        __m256i vlo = vmul(lo, dlo), vhi = vmul(hi, dhi);
+        kb::F w = hsum_rest(vlo, vhi);
+        for (int r = 0; r < 20; r++) {
+            const kb::F y = kb::fcube(kb::fadd(x0, k.internal[r]));
+            const kb::F sr = kb::fadd(rest, y);
+            x0 = kb::fadd(sr, kb::fmul(d0, y));
+            rest = kb::fadd(kb::fmul(fifteen, sr), w);
+            const __m256i sv = _mm256_set1_epi32((int)sr.v);
+            lo = vadd(sv, vlo);
+            hi = vadd(sv, vhi);
+            if (r + 1 < 20) {
+                vlo = vmul(lo, dlo);
+                vhi = vmul(hi, dhi);
+                w = hsum_rest(vlo, vhi);
+            }
         }
+        lo = _mm256_blend_epi32(lo, _mm256_set1_epi32((int)x0.v), 0x01);
         for (int r = 0; r < 4; r++) {
             lo = vcube(vadd(lo, ld(k.ext_term[r])));
             hi = vcube(vadd(hi, ld(k.ext_term[r] + 8)));
