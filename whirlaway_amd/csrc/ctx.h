@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <map>
 #include <stdexcept>
@@ -148,10 +149,26 @@ struct wl_ctx {
     // Device -> host results of each protocol round: a kernel writes them into
     // host-mapped pinned memory and then stores a sequence number; the host
     // waits on that flag (far cheaper than a copy plus a stream synchronize).
+    // word offsets in the mapped buffer of results read after later publishes
+    static constexpr size_t ISUM_OFF = (size_t)1 << 19, GRIND_OFF = ((size_t)1 << 19) + 256;
     uint32_t *mapped = nullptr;      // host view
     uint32_t *mapped_dev = nullptr;  // device view
     size_t mapped_words = 0;
     uint32_t seq = 0;
+    // device words that kernels leave as they found them: {smallest grinding
+    // witness, grinding blocks done} armed as {0xffffffff, 0}, then the block
+    // counter of multi-block publishes (0)
+    uint32_t *sync_dev = nullptr;
+    uint32_t *sync_words() {
+        if (sync_dev) return sync_dev;
+        HIP_CHECK(hipMalloc((void **)&sync_dev, 4 * sizeof(uint32_t)));
+        const uint32_t armed[4] = {0xffffffffu, 0u, 0u, 0u};
+        HIP_CHECK(hipMemcpyAsync(sync_dev, armed, sizeof(armed), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        return sync_dev;
+    }
+    uint32_t *grind_state() { return sync_words(); }
+    uint32_t *publish_counter() { return sync_words() + 2; }
     void ensure_mapped() {
         if (mapped) return;
         mapped_words = (size_t)1 << 20;
@@ -165,12 +182,13 @@ struct wl_ctx {
         volatile uint32_t *f = (volatile uint32_t *)(mapped + mapped_words);
         long spins = 0;
         auto t0 = std::chrono::steady_clock::now();
-        while (*f != s) {
+        // sequence numbers are published in stream order, so a later one implies s
+        while ((int32_t)(*f - s) < 0) {
             if (++spins % 4096 == 0) {
                 double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 if (el > 20.0) {
                     HIP_CHECK(hipStreamSynchronize(stream));
-                    if (*f != s) throw std::runtime_error("device result was not published");
+                    if ((int32_t)(*f - s) < 0) throw std::runtime_error("device result was not published");
                 }
             }
         }
@@ -239,7 +257,7 @@ struct wl_ctx {
             return;
         }
         uint32_t s = ++seq;
-        wk::publish((const uint32_t *)src, words, mapped_dev, flag_dev(), s, stream);
+        wk::publish((const uint32_t *)src, words, mapped_dev, flag_dev(), s, publish_counter(), stream);
         wait_flag(s);
         memcpy(dst, mapped, bytes);
     }
@@ -325,7 +343,7 @@ struct KTimer {
 inline std::vector<EF> reduce_many(wl_ctx *c, const EF *parts, int n_items, int n_parts) {
     std::vector<EF> r(n_items);
     c->ensure_mapped();
-    if (n_items <= 4) {
+    if (n_items <= 16) {  // one block, a wave per item
         uint32_t s = ++c->seq;
         wk::reduce_publish(parts, n_items, n_parts, (EF *)c->mapped_dev, c->flag_dev(), s, c->stream);
         c->wait_flag(s);
@@ -342,10 +360,12 @@ inline EF reduce_one(wl_ctx *c, const EF *parts, int n_parts) { return reduce_ma
 // then, later, wait for it.  Publishes share the mapped buffer, so a second one
 // may be enqueued only behind an await_value the host releases after collecting
 // the first.
-inline uint32_t reduce_async(wl_ctx *c, const EF *parts, int n_items, int n_parts) {
+// `word_off` places the result away from the start of the mapped buffer, where
+// the next publishes land, so that it can be read after later ones.
+inline uint32_t reduce_async(wl_ctx *c, const EF *parts, int n_items, int n_parts, size_t word_off = 0) {
     c->ensure_mapped();
     uint32_t s = ++c->seq;
-    wk::reduce_publish(parts, n_items, n_parts, (EF *)c->mapped_dev, c->flag_dev(), s, c->stream);
+    wk::reduce_publish(parts, n_items, n_parts, (EF *)(c->mapped_dev + word_off), c->flag_dev(), s, c->stream);
     return s;
 }
 // reduce_async whose kernel then waits for the host's next value (the returned
@@ -366,6 +386,30 @@ inline std::vector<EF> collect(wl_ctx *c, uint32_t s, int n_items) {
     memcpy(r.data(), c->mapped, (size_t)n_items * sizeof(EF));
     return r;
 }
+// Several small host arrays sent with one copy: add() returns each one's word
+// offset (256-byte aligned) in one pool buffer, whose device pointers at() gives
+// after send().
+struct Uploads {
+    wl_ctx *c;
+    std::vector<uint32_t> h;
+    DBuf<uint32_t> d;
+    explicit Uploads(wl_ctx *ctx) : c(ctx) {}
+    template <class T>
+    size_t add(const T *p, size_t n) {
+        const size_t off = h.size(), words = (n * sizeof(T) + 3) / 4;
+        h.resize(off + std::max<size_t>(64, (words + 63) / 64 * 64), 0u);
+        if (n) memcpy(h.data() + off, (const void *)p, n * sizeof(T));
+        return off;
+    }
+    template <class T>
+    size_t add(const std::vector<T> &v) { return add(v.data(), v.size()); }
+    void send() {
+        d = DBuf<uint32_t>(c->pool, std::max<size_t>(1, h.size()));
+        c->h2d_async(d.p, h.data(), h.size());
+    }
+    template <class T>
+    T *at(size_t off) { return (T *)(d.p + off); }
+};
 // Stream-ordered upload (no synchronization; device buffers are recycled in
 // stream order, so reuse by later work is safe).
 inline DBuf<EF> upload_ef(wl_ctx *c, const std::vector<EF> &v) {
@@ -428,20 +472,24 @@ inline void grind(wl_ctx *c, wh::Transcript &ts, const P2Dev *pdev, int bits) {
     for (int i = 0; i < 16; i++) st.s[i] = ts.state[i];
     for (int i = 0; i < ts.n_in; i++) st.s[i] = ts.in_buf[i];
     int pos = ts.n_in;  // observe(w) places w here; the sample then duplexes
-    DBuf<uint32_t> found(c->pool, 1);
     uint32_t mask = (uint32_t)((1ull << bits) - 1);
     uint32_t w = 0xffffffffu;
-    // 2^(bits+2) candidates per launch: the chance that a launch finds no witness is e^-4
-    const uint32_t chunk = 1u << std::min(22, std::max(12, bits + 2));
+    uint32_t *gs = c->grind_state();
+    // 2^(bits+2) candidates per launch below 16 bits (a miss: chance e^-4),
+    // 2^(bits+1) from 16 bits on (e^-2; 2^17 candidates fill 256 CUs with two
+    // waves per SIMD), between 2^14 and 2^22
+    const int lg = std::min(22, std::max(14, bits < 16 ? bits + 2 : bits + 1));
+    const uint64_t chunk = (uint64_t)1 << lg;
     for (uint64_t base = 0; base < P && w == 0xffffffffu; base += chunk) {
-        uint32_t init = 0xffffffffu;
-        c->h2d_async(found.p, &init, 1);
         uint32_t cnt = (uint32_t)std::min<uint64_t>(chunk, P - base);
+        const uint32_t s = ++c->seq;
         {
             KTimer kt(c, KT_GRIND, (double)cnt);
-            wk::grind_v(pdev, st, pos, mask, (uint32_t)base, cnt, found, c->stream);
+            wk::grind_pass(pdev, st, pos, mask, (uint32_t)base, cnt, gs, c->mapped_dev + wl_ctx::GRIND_OFF,
+                           c->flag_dev(), s, c->stream);
         }
-        c->d2h(&w, found.p, 1);
+        c->wait_flag(s);
+        w = c->mapped[wl_ctx::GRIND_OFF];
     }
     if (w == 0xffffffffu) throw std::runtime_error("grinding found no witness");
     ts.observe(to_mont(w));

@@ -60,8 +60,41 @@ void reduce_parts(const EF *in, int n_items, int n_parts, EF *out, hipStream_t s
     k_reduce_parts<<<n_items, 256, 0, s>>>(in, n_parts, out);
 }
 
-__global__ void k_publish(const uint32_t *src, size_t n, uint32_t *dst, volatile uint32_t *flag, uint32_t seq) {
-    for (size_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+// blocks copy interleaved 16-byte pieces; with several blocks the last one to
+// finish (counter, left at 0) stores the flag
+__global__ void k_publish(const uint32_t *src, size_t n, uint32_t *dst, volatile uint32_t *flag, uint32_t seq,
+                          uint32_t *counter) {
+    const size_t n4 = n / 4, stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool al = ((((uintptr_t)src) | ((uintptr_t)dst)) & 15) == 0;
+    if (al) {
+        for (size_t i = t; i < n4; i += stride) ((uint4 *)dst)[i] = ((const uint4 *)src)[i];
+        for (size_t i = 4 * n4 + t; i < n; i += stride) dst[i] = src[i];
+    } else {
+        for (size_t i = t; i < n; i += stride) dst[i] = src[i];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (gridDim.x > 1) {
+        if (atomicAdd(counter, 1u) != gridDim.x - 1) return;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __threadfence_system();
+    *flag = seq;
+}
+void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
+             uint32_t *counter, hipStream_t s) {
+    // one block up to 16 KiB, then a block per 16 KiB, at most 64
+    const int blocks = (int)std::min<size_t>(64, std::max<size_t>(1, (n + 4095) / 4096));
+    k_publish<<<blocks, 1024, 0, s>>>(src, n, host_dst, flag, seq, counter);
+}
+// rows of row_words words, stride_words apart in device memory, packed into host memory
+__global__ void k_publish_rows(const uint32_t *src, int rows, int row_words, size_t stride_words, uint32_t *dst,
+                               volatile uint32_t *flag, uint32_t seq) {
+    const int n = rows * row_words;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        dst[i] = src[(size_t)(i / row_words) * stride_words + (size_t)(i % row_words)];
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -69,22 +102,27 @@ __global__ void k_publish(const uint32_t *src, size_t n, uint32_t *dst, volatile
         *flag = seq;
     }
 }
-void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
-             hipStream_t s) {
-    k_publish<<<1, 1024, 0, s>>>(src, n, host_dst, flag, seq);
+void publish_rows(const uint32_t *src, int rows, int row_words, size_t stride_words, uint32_t *host_dst,
+                  volatile uint32_t *flag, uint32_t seq, hipStream_t s) {
+    k_publish_rows<<<1, 1024, 0, s>>>(src, rows, row_words, stride_words, host_dst, flag, seq);
 }
 
-// one block: item i reduced by the threads of wave(s) in turn, written to host memory
-__global__ void k_reduce_publish(const EF *in, int n_items, int n_parts, EF *dst, volatile uint32_t *flag,
-                                 uint32_t seq) {
-    __shared__ EF lds[16];
-    for (int it = 0; it < n_items; it++) {
+// one block: item i summed by wave i mod (waves per block), no block barriers,
+// written to dst
+__device__ __forceinline__ void reduce_items(const EF *in, int n_items, int n_parts, EF *dst) {
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int it = threadIdx.x >> 6; it < n_items; it += nw) {
         const EF *p = in + (size_t)it * n_parts;
         EF acc = ezero();
-        for (int i = threadIdx.x; i < n_parts; i += blockDim.x) acc = eadd(acc, p[i]);
-        EF t = block_sum(acc, lds);
-        if (threadIdx.x == 0) dst[it] = t;
+        for (int i = lane; i < n_parts; i += 64) acc = eadd(acc, p[i]);
+        acc = wave_sum(acc);
+        if (lane == 0) dst[it] = acc;
     }
+}
+// one block: the item sums written to host memory, then the flag
+__global__ void k_reduce_publish(const EF *in, int n_items, int n_parts, EF *dst, volatile uint32_t *flag,
+                                 uint32_t seq) {
+    reduce_items(in, n_items, n_parts, dst);
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -120,14 +158,7 @@ __device__ void await_copy(const uint32_t *aflag, uint32_t id, const uint32_t *s
 __global__ void k_reduce_publish_await(const EF *in, int n_items, int n_parts, EF *dst, volatile uint32_t *flag,
                                        uint32_t seq, const uint32_t *aflag, uint32_t id, const uint32_t *src,
                                        uint32_t *adst, int words, uint64_t timeout_ticks, uint32_t *err) {
-    __shared__ EF lds[16];
-    for (int it = 0; it < n_items; it++) {
-        const EF *p = in + (size_t)it * n_parts;
-        EF acc = ezero();
-        for (int i = threadIdx.x; i < n_parts; i += blockDim.x) acc = eadd(acc, p[i]);
-        EF t = block_sum(acc, lds);
-        if (threadIdx.x == 0) dst[it] = t;
-    }
+    reduce_items(in, n_items, n_parts, dst);
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0) {

@@ -622,44 +622,49 @@ void merkle_open(const F *cw, const F *nodes, int width, int log_rows, int la, c
 }
 
 // ----------------------------------------------------------------- grinding --
-__global__ void __launch_bounds__(256) k_grind(const P2DevT *pc, const F *st, int pos, uint32_t mask,
-                                              uint32_t base, uint32_t count, uint32_t *found) {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= count) return;
-    uint32_t w = base + g;
-    if (w >= P) return;
-    F s[16];
+// One launch tests the candidates base + i, i < count (one per thread), keeping
+// the smallest witness in state[0] (atomicMin); the last block to finish writes
+// it (0xffffffff: none) to host memory, re-arms state = {0xffffffff, 0} and
+// stores seq into *flag, so a grind costs one launch and no upload.  All the
+// candidates of a launch finish together, so the launch size (not an early
+// exit) sets the work: the host takes 2 to 4 x 2^bits and launches again on a
+// miss.
+__global__ void __launch_bounds__(256) k_grind_pass(const P2DevT *pc, State16 st, int pos, uint32_t mask,
+                                                   uint32_t base, uint32_t count, uint32_t *state,
+                                                   uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t w = base + i;
+    if (i < count && w < P) {
+        F s[16];
 #pragma unroll
-    for (int i = 0; i < 16; i++) s[i] = st[i];
-    s[pos] = to_mont(w);
-    p16_permute(s, pc->p16);
-    if ((from_mont(s[7]) & mask) == 0) atomicMin(found, w);
-}
-__global__ void __launch_bounds__(256) k_grind_v(const P2DevT *pc, State16 st, int pos, uint32_t mask,
-                                                uint32_t base, uint32_t count, uint32_t *found) {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= count) return;
-    uint32_t w = base + g;
-    if (w >= P) return;
-    F s[16];
+        for (int k = 0; k < 16; k++) s[k] = st.s[k];
 #pragma unroll
-    for (int i = 0; i < 16; i++) s[i] = st.s[i];
-    // pos is uniform; select without dynamic indexing of the state
-#pragma unroll
-    for (int i = 0; i < 16; i++)
-        if (i == pos) s[i] = to_mont(w);
-    p16_permute(s, pc->p16);
-    if ((from_mont(s[7]) & mask) == 0) atomicMin(found, w);
+        for (int k = 0; k < 16; k++)
+            if (k == pos) s[k] = to_mont(w);
+        p16_permute(s, pc->p16);
+        if ((from_mont(s[7]) & mask) == 0) atomicMin(state, w);
+    }
+    __shared__ uint32_t last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(state + 1, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        host_dst[0] = __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(state, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(state + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence_system();
+        *flag = seq;
+    }
 }
-void grind_v(const void *pc, State16 st, int pos, uint32_t mask, uint32_t base, uint32_t count,
-             uint32_t *found, hipStream_t s) {
-    k_grind_v<<<nblk(count), 256, 0, s>>>((const P2DevT *)pc, st, pos, mask, base, count, found);
+void grind_pass(const void *pc, State16 st, int pos, uint32_t mask, uint32_t base, uint32_t count,
+                uint32_t *state, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq, hipStream_t s) {
+    k_grind_pass<<<nblk(count), 256, 0, s>>>((const P2DevT *)pc, st, pos, mask, base, count, state, host_dst, flag,
+                                            seq);
 }
-void grind(const void *pc, const F *state16, int pos, uint32_t mask, uint32_t base, uint32_t count,
-           uint32_t *found, hipStream_t s) {
-    k_grind<<<nblk(count), 256, 0, s>>>((const P2DevT *)pc, state16, pos, mask, base, count, found);
-}
-
 __global__ void k_p2_batch(const P2DevT *pc, int width, F *st, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

@@ -31,8 +31,12 @@ void fill_zero(void *d, size_t bytes, hipStream_t s);
 void reduce_parts(const EF *in, int n_items, int n_parts, EF *out, hipStream_t s);
 // Copy n words to host-mapped memory, then (after a system-scope fence) store
 // seq into *flag; the host waits on the flag instead of synchronizing the stream.
+// counter: a device word at 0, used (and left at 0) when several blocks copy.
 void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
-             hipStream_t s);
+             uint32_t *counter, hipStream_t s);
+// rows x row_words words, stride_words apart, packed into host_dst, then flag = seq
+void publish_rows(const uint32_t *src, int rows, int row_words, size_t stride_words, uint32_t *host_dst,
+                  volatile uint32_t *flag, uint32_t seq, hipStream_t s);
 // reduce_parts whose outputs go straight to host-mapped memory, then the flag.
 void reduce_publish(const EF *in, int n_items, int n_parts, EF *host_dst, volatile uint32_t *flag,
                     uint32_t seq, hipStream_t s);
@@ -178,12 +182,10 @@ void merkle_open(const F *cw, const F *nodes, int width, int log_rows, int la, c
 // folded query values: out[q] = sum_b eqr[b] * row_q[b], rows of 2^fold F (base)
 // or 2^fold EF (8 F each, coefficient-wise) values
 void fold_rows(const F *rows, int nq, int fold, bool base, const EF *eqr, EF *out, hipStream_t s);
-// minimal w in [base, base + count) with P16(state with w at position pos)[7] & mask == 0,
-// result via atomicMin into *found (init 0xffffffff)
-void grind_v(const void *p16, State16 st, int pos, uint32_t mask, uint32_t base, uint32_t count,
-             uint32_t *found, hipStream_t s);
-void grind(const void *p16, const F *state16, int pos, uint32_t mask, uint32_t base, uint32_t count,
-           uint32_t *found, hipStream_t s);
+// minimal w in [base, base + count) with P16(state with w at position pos)[7] & mask == 0
+// (0xffffffff: none) to host_dst[0], then seq to *flag; state = {0xffffffff, 0} on entry and is left so
+void grind_pass(const void *p16, State16 st, int pos, uint32_t mask, uint32_t base, uint32_t count,
+                uint32_t *state, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq, hipStream_t s);
 // device Poseidon2 (tests): permute n states of width W in place
 void p2_permute_batch(const void *consts, int width, F *states, int n, hipStream_t s);
 

@@ -88,14 +88,13 @@ std::vector<EF> open_queries(wl_ctx *c, Transcript &ts, const Codeword &cwd, con
     size_t nval = eqr ? (size_t)nq * 8 : 0;
     // one device block [rows | paths | folded values] so that one fetch brings it all
     DBuf<F> buf(c->pool, nrow + npath + nval);
-    DBuf<uint32_t> didx(c->pool, nq);
-    c->h2d_async(didx.p, q.data(), nq);
-    wk::merkle_open(cwd.cw, cwd.nodes, cwd.width, cwd.log_rows, cwd.la, didx, nq, buf.p, buf.p + nrow, c->stream);
-    DBuf<EF> deq;
-    if (eqr) {
-        deq = upload_ef(c, *eqr);
-        wk::fold_rows(buf.p, nq, cwd.fold, cwd.base, deq, (EF *)(buf.p + nrow + npath), c->stream);
-    }
+    Uploads up(c);
+    const size_t o_idx = up.add(q), o_eq = eqr ? up.add(*eqr) : 0;
+    up.send();
+    wk::merkle_open(cwd.cw, cwd.nodes, cwd.width, cwd.log_rows, cwd.la, up.at<uint32_t>(o_idx), nq, buf.p,
+                    buf.p + nrow, c->stream);
+    if (eqr)
+        wk::fold_rows(buf.p, nq, cwd.fold, cwd.base, up.at<EF>(o_eq), (EF *)(buf.p + nrow + npath), c->stream);
     std::vector<F> host(nrow + npath + nval);
     c->d2h(host.data(), buf.p, host.size());
     for (int i = 0; i < nq; i++) {
@@ -177,13 +176,13 @@ void add_constraints_w(wl_ctx *c, EF *W, int m, const std::vector<Constraint> &c
     int nF = (int)sF.size(), nE = (int)sE.size();
     DBuf<F> hiF(c->pool, std::max<size_t>(1, nF * HI)), loF(c->pool, std::max<size_t>(1, nF * LO));
     DBuf<EF> hiE(c->pool, std::max<size_t>(1, nE * HI)), loE(c->pool, std::max<size_t>(1, nE * LO));
-    DBuf<EF> dsF = upload_ef(c, sF), dsE = upload_ef(c, sE);
+    Uploads up(c);
+    const size_t o_sF = up.add(sF), o_sE = up.add(sE), o_yl = up.add(yl), o_yh = up.add(yh);
+    up.send();
+    const EF *dsF = up.at<EF>(o_sF), *dsE = up.at<EF>(o_sE);
     if (nF) {
-        DBuf<F> dyl(c->pool, nF), dyh(c->pool, nF);
-        c->h2d_async(dyl.p, yl.data(), nF);
-        c->h2d_async(dyh.p, yh.data(), nF);
-        wk::eq_univ_f_tables(dyh, nF, hi, hiF, c->stream);
-        wk::eq_univ_f_tables(dyl, nF, lo, loF, c->stream);
+        wk::eq_univ_f_tables(up.at<F>(o_yh), nF, hi, hiF, c->stream);
+        wk::eq_univ_f_tables(up.at<F>(o_yl), nF, lo, loF, c->stream);
     }
     for (int j = 0; j < nE; j++) {
         std::vector<EF> pt = cons_point(*efc[j], m);
@@ -488,10 +487,9 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     c->ring_reset();
 
     auto tp = std::chrono::steady_clock::now();
+    // phase boundaries on the host's timeline (no stream synchronization: the
+    // next phase's launches queue behind the current one's)
     auto lap = [&](int i) {
-        auto tw = std::chrono::steady_clock::now();
-        HIP_CHECK(hipStreamSynchronize(c->stream));
-        c->phase_ms[6] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
         auto now = std::chrono::steady_clock::now();
         c->phase_ms[i] += std::chrono::duration<double, std::milli>(now - tp).count();
         tp = now;
@@ -613,27 +611,32 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     eq.reset();
     lap(2);
     // ---- inner sums ----
+    // up (column heads of the folded table) is published first and absorbed while
+    // the device computes the down sums, published behind it.
     std::vector<EF> up(AIR_NCOLS), dn(AIR_NCOLS);
-    {
-        std::vector<EF> tmp((size_t)AIR_NCOLS);
-        uint32_t *st2 = c->stage((size_t)AIR_NCOLS * 8);
-        HIP_CHECK(hipMemcpy2DAsync(st2, sizeof(EF), U.p, M * sizeof(EF), sizeof(EF), AIR_NCOLS,
-                                   hipMemcpyDeviceToHost, c->stream));
-        HIP_CHECK(hipStreamSynchronize(c->stream));
-        memcpy(up.data(), st2, AIR_NCOLS * sizeof(EF));
-    }
+    c->ensure_mapped();
+    const size_t DN_OFF = (size_t)AIR_NCOLS * 8;
+    const uint32_t s_up = ++c->seq;
+    wk::publish_rows((const uint32_t *)U.p, AIR_NCOLS, 8, M * 8, c->mapped_dev, c->flag_dev(), s_up, c->stream);
     U.reset();
     std::vector<EF> chr(ch.begin() + 1, ch.end());
     DBuf<EF> eqr = eq_dev(c, chr, eone());
+    uint32_t s_dn;
     {
         DBuf<EF> parts(c->pool, (size_t)AIR_NCOLS * K);
         wk::zc_down_parts(packed, n, skips, eqr, parts, c->stream);
         DBuf<EF> dS = upload_ef(c, wh::selectors_ef(skips, ch[0]));
         DBuf<EF> out(c->pool, AIR_NCOLS);
         wk::zc_down_combine(parts, skips, dS, out, c->stream);
-        c->d2h(dn.data(), out.p, AIR_NCOLS);
+        s_dn = ++c->seq;
+        wk::publish((const uint32_t *)out.p, DN_OFF, c->mapped_dev + DN_OFF, c->flag_dev(), s_dn, c->publish_counter(),
+                    c->stream);
     }
+    c->wait_flag(s_up);
+    memcpy(up.data(), c->mapped, AIR_NCOLS * sizeof(EF));
     ts.add_ef(up.data(), up.size());
+    c->wait_flag(s_dn);
+    memcpy(dn.data(), c->mapped + DN_OFF, AIR_NCOLS * sizeof(EF));
     ts.add_ef(dn.data(), dn.size());
     grind(c, ts, pdev, pow_bits_air(sec, lc));
     std::vector<EF> cbs(lc);
@@ -659,19 +662,29 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     DBuf<EF> E = eq_dev(c, pt, eone());
     DBuf<EF> Mt(c->pool, N);
     wk::build_matrix(E, n, da2, Mt, c->stream);
-    EF inner_sum;
+    // the inner sum is first needed with the first inner round's values: publish
+    // it aside and read it then
+    const size_t ISUM_OFF = wl_ctx::ISUM_OFF;
+    uint32_t s_isum;
     {
         int nb = wk::dot_blocks(N);
         DBuf<EF> parts(c->pool, nb);
         wk::seg_dot(mixed, E, N, 1, parts, c->stream);
-        inner_sum = reduce_one(c, parts, nb);
+        s_isum = reduce_async(c, parts, 1, nb, ISUM_OFF);
     }
     mixed.reset();
     E.reset();
     lap(3);
     // ---- inner sumcheck (degree 2, coefficients sent) ----
     std::vector<EF> ich;
-    EF isum = inner_sum;
+    EF isum;
+    bool have_isum = false;
+    auto read_isum = [&]() {
+        if (have_isum) return;
+        c->wait_flag(s_isum);
+        memcpy(&isum, c->mapped + ISUM_OFF, sizeof(EF));
+        have_isum = true;
+    };
     // pipelined as the zerocheck: round j + 1 (fused with the fold by c_j) waits
     // on the device for c_j
     auto enqueue_inner = [&](int ll, const EF *cdev, DBuf<EF> &parts, uint32_t &seq) {
@@ -685,6 +698,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     const int HOST_TAIL = 9;
     const int gpu_rounds = n > HOST_TAIL ? n - HOST_TAIL : 0;
     auto inner_round = [&](const EF &h0, const EF &h2) {
+        read_isum();
         std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2)};
         std::vector<EF> p = wh::lagrange(xs, {h0, esub(isum, h0), h2});
         ts.add_ef(p.data(), p.size());
@@ -781,6 +795,7 @@ wl_ctx *wl_create(int device) {
         HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreate(&c->ev0));
         HIP_CHECK(hipEventCreate(&c->ev1));
+        (void)c->sync_words();  // synchronizes once, before any device wait can be pending
         return c;
     } catch (const std::exception &e) {
         g_err = e.what();
@@ -801,6 +816,7 @@ void wl_destroy(wl_ctx *c) {
     if (c->mapped) (void)hipHostFree(c->mapped);
     if (c->mbox) (void)hipHostFree(c->mbox);
     if (c->args_dev) (void)hipFree(c->args_dev);
+    if (c->sync_dev) (void)hipFree(c->sync_dev);
     for (auto &e : c->kt_free) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (auto &v : c->kt_pairs)
         for (auto &e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
@@ -860,6 +876,7 @@ static int prove_common(wl_table *t, const wl_air_settings *st, const uint32_t *
     std::vector<uint32_t> proof;
     try {
         proof = air_prove(t, st, ph, pd.p, packed);
+        HIP_CHECK(hipStreamSynchronize(c->stream));  // nothing of this proof left in flight
     } catch (...) {
         c->release_all();  // device waits for host values must not outlive the failure
         throw;
