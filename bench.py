@@ -24,7 +24,11 @@ sys.path.insert(0, ROOT)
 # VALU classes count field multiplications (F-mul) as in the per-unit figures of the
 # work model: Poseidon2-24 982, Poseidon2-16 616 F-mul per permutation; one EF
 # AIR evaluation point 31168 F-mul; one skip-round point 5700 F-mul.
-VALU_PEAK_TFMUL = 256 * 64 * 2.4e9 / 4 / 3 / 1e12   # quarter-rate 32-bit multiplies, 3 per F-mul
+# VALU peak: 256 CUs x 64 lanes x 2.4 GHz full-rate instructions (the 32-bit integer
+# multiplies v_mad_u64_u32 / v_mul_lo_u32 / v_mul_hi_u32 issue at full rate on gfx950,
+# as the live Montgomery-product probe below shows) over the 6 instructions a
+# Montgomery product needs at least (3 multiplies, subtract, add of p, unsigned min).
+VALU_PEAK_TFMUL = 256 * 64 * 2.4e9 / 6 / 1e12
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of each kernel class at 2^17 permutations, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE passes (profiles/prove_n17_hbm_traffic.md), fetch scaled x2.0
@@ -276,6 +280,7 @@ def main():
     roofline = {
         "bound": bound, "achieved": round(achieved, 4), "peak": round(peak, 4), "unit": unit,
         "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": name,
+        "fmul_probe_TFmul": round(ctx.fmul_rate() / 1e12, 4),
         "avg_launch_ms": round(avg_ms, 5), "launches_per_step": launches / args.steps,
         "kernel_ms_per_step": {MODELS[k][0].split(" ")[0]: round(v[1] / args.steps, 4)
                                for k, v in per_class.items()},

@@ -113,6 +113,11 @@ int wl_kernel_timing(wl_ctx *ctx, uint32_t class_mask);
 int wl_kernel_timing_read(wl_ctx *ctx, int kernel_class, long *launches, double *total_ms,
                           double *work);
 
+/* Measured throughput of the Montgomery product (F x F) on the context's
+ * device: 2^20 threads, four independent product chains each (no counterpart in
+ * the reference; used by the benchmark beside its VALU roofline). */
+int wl_fmul_rate(wl_ctx *ctx, double *fmul_per_s);
+
 /* Device memory of a context's pool: bytes held (live + cached) and the peak of
  * live bytes since the last reset (no counterpart in the reference). */
 void wl_mem_stats(wl_ctx *ctx, uint64_t *held, uint64_t *peak_live, int reset_peak);

@@ -63,6 +63,7 @@ def lib():
         u64 = ctypes.c_uint64
         u64p = ctypes.POINTER(u64)
         L.wl_mem_stats.argtypes = [vp, u64p, u64p, ctypes.c_int]
+        L.wl_fmul_rate.argtypes = [vp, dp]
         L.wl_job_footprint.restype = u64
         L.wl_job_footprint.argtypes = [ctypes.c_uint32]
         L.wl_engine_create.restype = vp

@@ -56,6 +56,12 @@ class Context:
         lib().wl_mem_stats(self.ptr, ctypes.byref(held), ctypes.byref(peak), int(reset_peak))
         return held.value, peak.value
 
+    def fmul_rate(self):
+        """Measured Montgomery products per second on this context's device."""
+        r = ctypes.c_double(0.0)
+        check(lib().wl_fmul_rate(self.ptr, ctypes.byref(r)))
+        return r.value
+
     KERNEL_CLASSES = {1: "merkle_leaves", 2: "zerocheck_rounds", 3: "rs_encode_ntt",
                       4: "zerocheck_round0", 5: "grinding", 6: "merkle_nodes"}
 

@@ -63,9 +63,10 @@ struct HostP16 {
         }
         // Partial rounds with lane 0 kept as a scalar: with x the state entering a
         // round, y = (x0 + rc)^3, S = sum_{i>=1} x_i and s = S + y, the next state is
-        // x0' = s + d0 y and x_i' = s + d_i x_i (i >= 1), and
+        // x0' = s + d0 y = S - y (d0 = -2) and x_i' = s + d_i x_i (i >= 1), and
         // S' = 15 s + sum_{i>=1} d_i x_i.  The vector products d_i x_i and their sum
-        // depend only on x, so they overlap the scalar S-box chain.
+        // depend only on x, so they overlap the scalar S-box chain, whose step is
+        // one subtraction, the round constant and the cube.
         const __m256i lane0 = _mm256_setr_epi32(-1, 0, 0, 0, 0, 0, 0, 0);
         auto hsum_rest = [&](__m256i a, __m256i b) {  // sum of lanes 1..15 of (a, b)
             __m256i u = vadd(_mm256_andnot_si256(lane0, a), b);
@@ -74,7 +75,7 @@ struct HostP16 {
             u = vadd(u, _mm256_shuffle_epi32(u, 0xB1));
             return kb::F{(uint32_t)_mm256_cvtsi256_si32(u)};
         };
-        const kb::F fifteen = kb::to_mont(15), d0{kb::DIAG.d16[0]};
+        const kb::F fifteen = kb::to_mont(15);
         kb::F x0{(uint32_t)_mm256_cvtsi256_si32(lo)};
         kb::F rest = hsum_rest(lo, hi);
         __m256i vlo = vmul(lo, dlo), vhi = vmul(hi, dhi);
@@ -82,7 +83,7 @@ struct HostP16 {
         for (int r = 0; r < 20; r++) {
             const kb::F y = kb::fcube(kb::fadd(x0, k.internal[r]));
             const kb::F sr = kb::fadd(rest, y);
-            x0 = kb::fadd(sr, kb::fmul(d0, y));
+            x0 = kb::fsub(rest, y);
             rest = kb::fadd(kb::fmul(fifteen, sr), w);
             const __m256i sv = _mm256_set1_epi32((int)sr.v);
             lo = vadd(sv, vlo);

@@ -224,14 +224,6 @@ void eq_table_v(const EqPoint &pt, EF *out, hipStream_t s) {
     k_eq_one<<<1u << (n - lo), 256, 0, s>>>(pt, lo, out);
 }
 
-__global__ void k_eq_halve(EF *E, size_t h) {
-    size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x < h) E[x] = eadd(E[x], E[x + h]);
-}
-void eq_halve(EF *E, int l, hipStream_t s) {
-    size_t h = (size_t)1 << (l - 1);
-    k_eq_halve<<<nblk(h), 256, 0, s>>>(E, h);
-}
 
 // --------------------------------------------------------- AIR evaluation --
 // Poseidon2Air<16, x^3, 4 + 20 + 4 rounds> constraints (p3-poseidon2-air):
@@ -424,21 +416,6 @@ void zc_fold0(const F *trace, int log_n, int skips, const EF *S, EF *U, hipStrea
     k_zc_fold0<<<dim3(nblk(M), AIR_NCOLS), 256, 0, s>>>(trace, log_n, skips, S, U);
 }
 
-__global__ void k_fold_cols(EF *U, size_t stride, int l, const EF *c) {
-    const size_t h = (size_t)1 << (l - 1);
-    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= h) return;
-    EF *col = U + (size_t)blockIdx.y * stride;
-    EF a = col[x];
-    col[x] = eadd(a, emul(*c, esub(col[x + h], a)));
-}
-void fold_cols_strided(EF *U, int n_cols, size_t stride, int l, const EF *c, hipStream_t s) {
-    size_t h = (size_t)1 << (l - 1);
-    k_fold_cols<<<dim3(nblk(h), n_cols), 256, 0, s>>>(U, stride, l, c);
-}
-
-// column c < n_cols: fold by (1 - ch, ch); the extra row blockIdx.y == n_cols
-// halves the eq table for the next round.
 // Wait for a value the host writes after this kernel was enqueued: thread 0 polls
 // *flag (host-mapped, system scope) until it reaches id, then the block copies
 // the words into device memory for the kernels queued behind it.  The wait is
@@ -453,6 +430,9 @@ void await_host(const uint32_t *flag, uint32_t id, const uint32_t *src, uint32_t
     k_await_host<<<1, 64, 0, s>>>(flag, id, src, dst, words, timeout_ticks, err);
 }
 
+// column c < n_cols: fold by (1 - ch, ch), ch read from device memory (a value
+// an await_host launch placed there); the extra row blockIdx.y == n_cols halves
+// the eq table for the next round.
 __global__ void k_fold_cols_p(EF *U, size_t stride, int l, int n_cols, const EF *chp, EF *E) {
     const size_t h = (size_t)1 << (l - 1);
     const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,24 +450,6 @@ __global__ void k_fold_cols_p(EF *U, size_t stride, int l, int n_cols, const EF 
 void fold_cols_p(EF *U, int n_cols, size_t stride, int l, const EF *c, EF *E, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
     k_fold_cols_p<<<dim3(nblk(h), n_cols + (E && h > 1 ? 1 : 0)), 256, 0, s>>>(U, stride, l, n_cols, c, E);
-}
-__global__ void k_fold_cols_v(EF *U, size_t stride, int l, int n_cols, EF ch, EF *E) {
-    const size_t h = (size_t)1 << (l - 1);
-    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= h) return;
-    if ((int)blockIdx.y == n_cols) {
-        // eq table of length h -> h / 2
-        size_t q = h >> 1;
-        if (x < q) E[x] = eadd(E[x], E[x + q]);
-        return;
-    }
-    EF *col = U + (size_t)blockIdx.y * stride;
-    EF a = col[x];
-    col[x] = eadd(a, emul(ch, esub(col[x + h], a)));
-}
-void fold_cols_v(EF *U, int n_cols, size_t stride, int l, EF c, EF *E, hipStream_t s) {
-    size_t h = (size_t)1 << (l - 1);
-    k_fold_cols_v<<<dim3(nblk(h), n_cols + (E && h > 1 ? 1 : 0)), 256, 0, s>>>(U, stride, l, n_cols, c, E);
 }
 
 // down inner sums
@@ -589,39 +551,12 @@ int seg_dot(const EF *A, const EF *E, size_t len, int nseg, EF *parts, hipStream
     k_seg_dot<false><<<dim3(nb, nseg), 256, 0, s>>>(A, E, len, parts);
     return (int)nb;
 }
-int seg_dot_f(const F *A, const EF *E, size_t len, int nseg, EF *parts, hipStream_t s) {
-    unsigned nb = dot_blocks(len);
-    k_seg_dot<true><<<dim3(nb, nseg), 256, 0, s>>>(A, E, len, parts);
-    return (int)nb;
-}
 
-// --------------------------------------------------------- product sumcheck --
-__global__ void k_prod_round(const EF *A, const EF *B, size_t h, EF *parts) {
-    EF a0 = ezero(), a2 = ezero();
-    size_t st = (size_t)gridDim.x * blockDim.x;
-    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < h; x += st) {
-        EF al = A[x], ah = A[x + h], bl = B[x], bh = B[x + h];
-        a0 = eadd(a0, emul(al, bl));
-        a2 = eadd(a2, emul(esub(edbl(ah), al), esub(edbl(bh), bl)));
-    }
-    __shared__ EF lds[16];
-    EF t0 = block_sum(a0, lds);
-    EF t2 = block_sum(a2, lds);
-    if (threadIdx.x == 0) {
-        parts[blockIdx.x] = t0;
-        parts[gridDim.x + blockIdx.x] = t2;
-    }
-}
-int prod_round(const EF *A, const EF *B, int l, EF *parts, hipStream_t s) {
-    size_t h = (size_t)1 << (l - 1);
-    unsigned nb = dot_blocks(h);
-    k_prod_round<<<nb, 256, 0, s>>>(A, B, h, parts);
-    return (int)nb;
-}
 // One round of the product sumcheck on (A, B) of length 2h, fused with the fold
 // by the previous challenge c when `fold` is set (the tables then hold 4h
 // entries and are folded in place to 2h: thread x owns x, x+h, x+2h, x+3h), so
-// each table is read once per round.  parts as k_prod_round.
+// each table is read once per round.  parts[z][block] holds the block sums of
+// A(z) B(z) for z = 0 and 2 (z = 1 follows from the claimed sum).
 __global__ void __launch_bounds__(256) k_prod_round_f(EF *A, EF *B, size_t h, int fold, const EF *cp,
                                                      EF *parts) {
     EF a0 = ezero(), a2 = ezero();
@@ -677,17 +612,6 @@ __global__ void k_fold_ef(EF *A, size_t h, const EF *c) {
 void fold_ef(EF *A, int l, const EF *c, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
     k_fold_ef<<<nblk(h), 256, 0, s>>>(A, h, c);
-}
-__global__ void k_fold_ef2_v(EF *A, EF *B, size_t h, EF c) {
-    size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= h) return;
-    EF *T = blockIdx.y ? B : A;
-    EF a = T[x];
-    T[x] = eadd(a, emul(c, esub(T[x + h], a)));
-}
-void fold_ef2_v(EF *A, EF *B, int l, EF c, hipStream_t s) {
-    size_t h = (size_t)1 << (l - 1);
-    k_fold_ef2_v<<<dim3(nblk(h), B ? 2 : 1), 256, 0, s>>>(A, B, h, c);
 }
 
 // -------------------------------------------------------------------- micro --
@@ -817,12 +741,6 @@ int a2b_round_fused(const F *a0, const F *b0, EF *A, EF *B, int l, bool first, E
     unsigned nb = dot_blocks(h);
     if (first) k_a2b_round_fused<true><<<nb, 256, 0, s>>>(a0, b0, A, B, h, c, eq, parts);
     else k_a2b_round_fused<false><<<nb, 256, 0, s>>>(a0, b0, A, B, h, c, eq, parts);
-    return (int)nb;
-}
-int a2b_round_ef(const EF *a, const EF *b, int l, const EF *eq, EF *parts, hipStream_t s) {
-    size_t h = (size_t)1 << (l - 1);
-    unsigned nb = dot_blocks(h);
-    k_a2b_round<false><<<dim3(nb, 3), 256, 0, s>>>(a, b, h, eq, parts);
     return (int)nb;
 }
 

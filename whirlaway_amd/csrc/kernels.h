@@ -49,9 +49,6 @@ void reduce_publish_await(const EF *in, int n_items, int n_parts, EF *host_dst, 
 // eq(point, .) over n <= 32 variables, big-endian, times scale (point by value).
 struct EqPoint { EF p[32]; EF scale; int n; };
 void eq_table_v(const EqPoint &pt, EF *out, hipStream_t s);
-// E[x] += E[x + 2^(l-1)] for x < 2^(l-1): eq(r[1..], .) from eq(r, .) since the
-// two halves of eq(r, .) sum to eq(r[1..], .).
-void eq_halve(EF *E, int l, hipStream_t s);
 struct State16 { F s[16]; };
 
 // ------------------------------------------------------------ zerocheck --
@@ -75,12 +72,9 @@ void zc_fold0(const F *trace, int log_n, int skips, const EF *S, EF *U, hipStrea
 int zc_round_wave(const EF *U, size_t stride, int l, const AirConsts *air, const ChainLin *cl,
                   const EF *alpha_pows, const EF *eq, EF *parts, hipStream_t s);
 int zc_round_wave_blocks(int l);
-// In-place fold of n_cols EF columns by (1-c, c): col[x] += c (col[x+h] - col[x]);
-// the column stride is unchanged.
-void fold_cols_strided(EF *U, int n_cols, size_t stride, int l, const EF *c, hipStream_t s);
-// by-value challenge; also halves the eq table E (length 2^(l-1)) when E != nullptr
-void fold_cols_v(EF *U, int n_cols, size_t stride, int l, EF c, EF *E, hipStream_t s);
-// the same with the challenge in device memory
+// In-place fold of n_cols EF columns (stride apart) from 2^l to 2^(l-1) by
+// (1 - c, c), c in device memory; also halves the eq table E (length 2^(l-1))
+// when E != nullptr.
 void fold_cols_p(EF *U, int n_cols, size_t stride, int l, const EF *c, EF *E, hipStream_t s);
 // Block the stream until the host sets *flag >= id (host-mapped, written after
 // enqueue), then copy `words` words from the host-mapped src to device dst.
@@ -102,25 +96,20 @@ void mix_updown(const EF *B, int log_n, const EF *a, EF *mixed, hipStream_t s);
 void build_matrix(const EF *E, int log_n, const EF *a, EF *M, hipStream_t s);
 // parts[j][b]: block partials of sum_{i<len} A[j*len + i] * E[i], j < nseg; returns n_blocks
 int seg_dot(const EF *A, const EF *E, size_t len, int nseg, EF *parts, hipStream_t s);
-int seg_dot_f(const F *A, const EF *E, size_t len, int nseg, EF *parts, hipStream_t s);
 
-// number of block partials produced by seg_dot / prod_round / a2b_round over len items
+// number of block partials produced by seg_dot / prod_round_fused / a2b rounds over len items
 int dot_blocks(size_t len);
 // number of x-chunks produced by whir_T_f
 int whir_T_chunks(int m, int k);
 
-// ------------------------------------------------------ product sumcheck --
-// parts[0][b] = sum_{x<h} A[x] B[x], parts[1][b] = sum (2A[x+h]-A[x])(2B[x+h]-B[x]); h = 2^(l-1)
-int prod_round(const EF *A, const EF *B, int l, EF *parts, hipStream_t s);
-// The same round on tables of length 2^l, first folding them in place from
-// 2^(l+1) by *c (device memory) when fold is set.
+// Product sumcheck round on tables of length 2^l (h = 2^(l-1)): parts[0][b] =
+// block sums of A[x] B[x], parts[1][b] of (2A[x+h]-A[x])(2B[x+h]-B[x]); when fold
+// is set the tables are first folded in place from 2^(l+1) by *c (device memory).
 int prod_round_fused(EF *A, EF *B, int l, bool fold, const EF *c, EF *parts, hipStream_t s);
-// fold_ef2_v with the challenge in device memory
+// fold one or two EF tables (B may be nullptr) from 2^l to 2^(l-1) by *c (device memory)
 void fold_ef2_p(EF *A, EF *B, int l, const EF *c, hipStream_t s);
 // in-place fold of one EF table by (1-c, c)
 void fold_ef(EF *A, int l, const EF *c, hipStream_t s);
-// fold one or two EF tables by a by-value challenge (B may be nullptr)
-void fold_ef2_v(EF *A, EF *B, int l, EF c, hipStream_t s);
 
 // -------------------------------------------------------------- micro --
 // fold an F table (2^l) by 2^k EF scalars: out[i] = sum_j s_j in[i + j 2^(l-k)]
@@ -129,7 +118,6 @@ void fold_ef_large(const EF *in, int l, const EF *sc, int k, EF *out, hipStream_
 // eq-weighted a^2 b sumcheck round (degree 3, z in {0,2,3}, or {0,1,2,3} when
 // all4); F inputs (base) or EF; parts [z][block]
 int a2b_round_f(const F *a, const F *b, int l, const EF *eq, EF *parts, hipStream_t s, bool all4);
-int a2b_round_ef(const EF *a, const EF *b, int l, const EF *eq, EF *parts, hipStream_t s);
 // a^2 b rounds >= 1 fused with the fold by the previous challenge c and the
 // halving of eq: tables of length 2^l after the fold (from a0, b0 when first, in
 // place in A, B otherwise), eq of length 2^l halved in place; parts [3][block].

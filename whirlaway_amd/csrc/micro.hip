@@ -36,9 +36,41 @@ int guard(wl_ctx *c, Fn fn) {
     }
 }
 
+// Throughput probe of the Montgomery product: four independent chains per
+// thread, 2^20 threads (16 waves per SIMD on 256 CUs).
+__global__ void __launch_bounds__(256) k_fmul_probe(uint32_t *out, int iters) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    F x{g % P}, y{(g * 7u + 3u) % P}, z{(g ^ 0x55u) % P}, w{(g + 99u) % P};
+    for (int i = 0; i < iters; i++) {
+        x = fmul(x, y);
+        y = fmul(y, z);
+        z = fmul(z, w);
+        w = fmul(w, x);
+    }
+    out[g] = x.v ^ y.v ^ z.v ^ w.v;
+}
+
 }  // namespace
 
 extern "C" {
+
+int wl_fmul_rate(wl_ctx *c, double *fmul_per_s) {
+    if (!fmul_per_s) { wl_set_error("null argument"); return WL_ERR_ARG; }
+    return guard(c, [&]() {
+        const int threads = 1 << 20, iters = 1024;
+        DBuf<uint32_t> out(c->pool, threads);
+        hipLaunchKernelGGL(k_fmul_probe, dim3(threads / 256), dim3(256), 0, c->stream, out.p, iters);  // warm-up
+        HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+        hipLaunchKernelGGL(k_fmul_probe, dim3(threads / 256), dim3(256), 0, c->stream, out.p, iters);
+        HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+        HIP_CHECK(hipEventSynchronize(c->ev1));
+        HIP_CHECK(hipGetLastError());
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *fmul_per_s = (double)threads * iters * 4 / (ms * 1e-3);
+        return WL_OK;
+    });
+}
 
 int wl_fold_large(wl_ctx *c, const uint32_t *in, int in_is_ef, uint32_t log_n, const uint32_t *scalars,
                   uint32_t log_k, uint32_t *out) {
