@@ -2,9 +2,9 @@
 // device code of the MI355X prover.
 //
 // F: p = 2^31 - 2^24 + 1 held in Montgomery form (R = 2^32) in one 32-bit VALU
-// lane; products are 64-bit (v_mul_lo/v_mul_hi) and reduced by REDC, which for
-// this p needs no division.  Sums of up to four products are accumulated in
-// 64 bits before one reduction (redc64).
+// lane; products are 64-bit (v_mad_u64_u32) and reduced by REDC, which for this
+// p needs no division.  Sums of up to four products are accumulated in 64 bits
+// before one reduction (redc64).
 // EF: F[X]/(X^8 - 3), eight consecutive u32 (coefficient 0 first), matching
 // BinomialExtensionField<KoalaBear, 8> named at src/examples/poseidon2.rs:31.
 // Poseidon2: Plonky3 layer structure, S-box x^3, width 16 (20 partial rounds)
@@ -30,43 +30,27 @@ struct F { uint32_t v; };
 // min(s, s - P) (wrapping) is s mod P; for d in (-P, P) stored wrapped,
 // min(d, d + P) is d mod P.  One v_min_u32 replaces a compare and a select.
 KB_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
-// lo * MU mod 2^32 with MU = 1 + 2^24 + 2^31: two full-rate shift-adds instead
-// of a quarter-rate v_mul_lo_u32 (the compiler would fold the shifts back into a
-// multiply, so the device path states the two instructions; register-only ALU).
-KB_HD uint32_t mul_mu(uint32_t lo) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint32_t t, r;
-    asm("v_lshl_add_u32 %0, %1, 24, %1" : "=v"(t) : "v"(lo));
-    asm("v_lshl_add_u32 %0, %1, 31, %2" : "=v"(r) : "v"(lo), "v"(t));
-    return r;
-#else
-    return lo + (lo << 24) + (lo << 31);
-#endif
-}
+constexpr uint32_t NEG_PINV = 0x7effffffu;    // -P^-1 mod 2^32
 
-KB_HD uint32_t redc(uint64_t x) {  // x < 2^32 * P  ->  x / 2^32 mod P
-    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    uint32_t m = mul_mu(lo);
-    uint32_t mh = (uint32_t)(((uint64_t)m * P) >> 32);
-    uint32_t r = hi - mh;  // in (-P, P)
-    return umin(r, r + P);
+// Montgomery reduction in its additive form: m = x (-P^-1) mod 2^32 makes
+// x + m P divisible by 2^32, and for x < 2^32 P the sum stays below
+// 2^33 P < 2^64, so (x + m P) / 2^32 is the high word of one v_mad_u64_u32 and
+// lies in [0, 2P).  Three instructions with the product; the 32-bit multiplies
+// issue at the full VALU rate on gfx950.
+KB_HD uint32_t redc_lazy(uint64_t x) {  // x < 2^32 P  ->  x / 2^32 mod P, in [0, 2P)
+    uint32_t m = (uint32_t)x * NEG_PINV;
+    return (uint32_t)(((uint64_t)m * P + x) >> 32);
 }
-
-KB_HD uint32_t redc64(uint64_t x) {  // any x < 2^64
-    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    uint32_t m = mul_mu(lo);
-    uint32_t mh = (uint32_t)(((uint64_t)m * P) >> 32);
-    uint32_t r = hi - mh;
-    uint32_t rb = r + P;           // result when hi < mh
-    r = umin(r, r - P);            // otherwise r < 2^32 < 3P: two reductions
-    r = umin(r, r - P);
-    return hi < mh ? rb : r;
+KB_HD uint32_t redc(uint64_t x) {  // x < 2^32 P  ->  x / 2^32 mod P
+    uint32_t r = redc_lazy(x);
+    return umin(r, r - P);
 }
 
 // s mod P-preserving shrink of a 64-bit sum: hi * (2^32 mod P) + lo < 2^57.1, so
 // four more products below P^2 can be added without overflow (one v_mad_u64_u32).
 constexpr uint32_t C32 = 0x01fffffeu;  // 2^32 mod P
 KB_HD uint64_t fold64(uint64_t s) { return (uint64_t)(uint32_t)(s >> 32) * C32 + (uint32_t)s; }
+KB_HD uint32_t redc64(uint64_t x) { return redc(fold64(x)); }  // any x < 2^64 (fold64(x) < 2^57.1)
 
 KB_HD F fadd(F a, F b) { uint32_t s = a.v + b.v; return F{umin(s, s - P)}; }
 KB_HD F fsub(F a, F b) { uint32_t d = a.v - b.v; return F{umin(d, d + P)}; }
@@ -77,7 +61,12 @@ KB_HD F fzero() { return F{0}; }
 KB_HD F fone() { return F{ONE_M}; }
 KB_HD F to_mont(uint32_t canon) { return F{redc((uint64_t)canon * R2)}; }
 KB_HD uint32_t from_mont(F a) { return redc((uint64_t)a.v); }
-KB_HD F fcube(F x) { return fmul(fmul(x, x), x); }
+// x^3 with the square left in [0, 2P): x^2 / 2^32 < 1.5 P, so x^2 x < 1.5 P^2 < 2^32 P
+KB_HD F fcube(F x) {
+    uint32_t x2 = redc_lazy((uint64_t)x.v * x.v);
+    uint32_t r = redc_lazy((uint64_t)x2 * x.v);
+    return F{umin(r, r - P)};
+}
 KB_HD bool feq(F a, F b) { return a.v == b.v; }
 
 inline F fpow(F a, uint64_t e) {
@@ -165,7 +154,7 @@ KB_HD void mds_light(F *s) {
     for (int i = 0; i < W; i++) s[i] = fadd(s[i], sums[i & 3]);
 }
 
-// x * 2^-k (1 <= k <= 24) without a multiplication: with x = hi 2^k + lo and
+// x * 2^-k (2 <= k <= 24) without a multiplication: with x = hi 2^k + lo and
 // 2^-k = -127 * 2^(24-k) mod P (since 127 * 2^24 = P - 1),
 // x 2^-k = hi + P - lo * 127 * 2^(24-k), a value in [1, 2P).
 // Works on Montgomery residues because scaling by a constant commutes with R.
@@ -191,14 +180,16 @@ constexpr DiagEntry DIAG24_SPEC[24] = {{1, 0, 2}, {0, 0, 1}, {0, 0, 2}, {0, 1, 1
                                        {0, 1, 4}, {0, 1, 5}, {0, 1, 6}, {0, 1, 24}, {1, 1, 8}, {1, 1, 3},
                                        {1, 1, 4}, {1, 1, 5}, {1, 1, 6}, {1, 1, 7}, {1, 1, 9}, {1, 1, 24}};
 
+// Montgomery image (v 2^32 mod P) of a small constant
+constexpr uint32_t cmont_small(uint32_t v) { return (uint32_t)(((uint64_t)v << 32) % P); }
+
 template <int W, int I>
 KB_HD F diag_scale(F x) {  // |d_I| * x
     constexpr DiagEntry e = W == 16 ? DIAG16_SPEC[I] : DIAG24_SPEC[I];
     if constexpr (e.kind == 0) {
         if constexpr (e.val == 1) return x;
         else if constexpr (e.val == 2) return fdbl(x);
-        else if constexpr (e.val == 3) return fadd(fdbl(x), x);
-        else return fdbl(fdbl(x));
+        else return fmul(x, F{cmont_small(e.val)});  // 3, 4: one product (5 instructions)
     } else if constexpr (e.val == 1) {
         return fhalf(x);
     } else {
