@@ -436,6 +436,28 @@ inline DBuf<EF> eq_dev(wl_ctx *c, const std::vector<EF> &pt, const EF &scale) {
 inline void eq_into(wl_ctx *c, const std::vector<EF> &pt, const EF &scale, EF *out) {
     wk::eq_table_v(eq_point(pt, scale), out, c->stream);
 }
+// several eq tables (point, scale, destination) in one launch
+struct EqJob {
+    std::vector<EF> pt;
+    EF scale;
+    EF *out;
+};
+inline void eq_batch(wl_ctx *c, const std::vector<EqJob> &jobs) {
+    if (jobs.empty()) return;
+    if (jobs.size() == 1) { eq_into(c, jobs[0].pt, jobs[0].scale, jobs[0].out); return; }
+    std::vector<wk::EqPoint> pts;
+    std::vector<EF *> outs;
+    int max_n = 0;
+    for (const auto &j : jobs) {
+        pts.push_back(eq_point(j.pt, j.scale));
+        outs.push_back(j.out);
+        max_n = std::max(max_n, (int)j.pt.size());
+    }
+    Uploads up(c);
+    const size_t o_pts = up.add(pts), o_outs = up.add(outs);
+    up.send();
+    wk::eq_tables(up.at<wk::EqPoint>(o_pts), up.at<EF *>(o_outs), (int)jobs.size(), max_n, c->stream);
+}
 
 inline void load_p2(const uint32_t *p16c, const uint32_t *p24c, P2Dev &host) {
     memset(&host, 0, sizeof host);

@@ -183,10 +183,9 @@ void reduce_publish(const EF *in, int n_items, int n_parts, EF *host_dst, volati
 // (lo = min(n, 8)): the table over the lo low variables is built by doublings in
 // LDS while a product tree over 32 slots forms the block's factor from the high
 // variables (bits of b); entry = factor * low[t].  Latency ~max(lo, 5) EF products.
-__global__ void __launch_bounds__(256) k_eq_one(EqPoint pt, int lo, EF *out) {
+__device__ __forceinline__ void eq_block(const EqPoint &pt, int lo, uint32_t b, EF *out) {
     __shared__ EF ta[256], tb[256], ha[32], hb[32];
     const int t = threadIdx.x, hi = pt.n - lo;
-    const uint32_t b = blockIdx.x;
     if (t < 32) {
         EF f = eone();
         if (t < hi) {
@@ -218,10 +217,22 @@ __global__ void __launch_bounds__(256) k_eq_one(EqPoint pt, int lo, EF *out) {
     }
     if (t < (1 << lo)) out[((size_t)b << lo) + t] = emul(hs[0], src[t]);
 }
+__global__ void __launch_bounds__(256) k_eq_one(EqPoint pt, int lo, EF *out) { eq_block(pt, lo, blockIdx.x, out); }
 void eq_table_v(const EqPoint &pt, EF *out, hipStream_t s) {
     const int n = pt.n;
     const int lo = n < 8 ? n : 8;
     k_eq_one<<<1u << (n - lo), 256, 0, s>>>(pt, lo, out);
+}
+// several tables in one launch: table j uses blocks (x, j) with x < 2^(n_j - lo_j)
+__global__ void __launch_bounds__(256) k_eq_batch(const EqPoint *pts, EF *const *outs) {
+    const EqPoint &pt = pts[blockIdx.y];
+    const int lo = pt.n < 8 ? pt.n : 8;
+    if (blockIdx.x >= (1u << (pt.n - lo))) return;
+    eq_block(pt, lo, blockIdx.x, outs[blockIdx.y]);
+}
+void eq_tables(const EqPoint *pts, EF *const *outs, int count, int max_n, hipStream_t s) {
+    const int lo = max_n < 8 ? max_n : 8;
+    if (count > 0) k_eq_batch<<<dim3(1u << (max_n - lo), count), 256, 0, s>>>(pts, outs);
 }
 
 

@@ -537,13 +537,17 @@ __global__ void __launch_bounds__(256) k_merkle_sub(F *nodes, size_t cnt, int le
     for (int l = 0; l < levels; l++) {
         const int np = (int)(pb >> l);
         const size_t first = (cnt >> l) + (size_t)blockIdx.x * np;
-        // every group runs the permutation (DPP needs whole rows); only g < np stores
-        const int gp = g < np ? g : 0;
-        F x = kid[gp * 16 + i];
-        x = p16_row(x, i, pc->p16);
-        if (g < np && i < 8) {
-            par[g * 8 + i] = x;
-            nodes[(first + g) * 8 + i] = x;
+        // DPP reads stay inside a 16-lane row, so a wave (4 groups) with no parent
+        // at this level skips the permutation; in a wave that has one, every group
+        // runs it and only g < np stores
+        if ((g & ~3) < np) {
+            const int gp = g < np ? g : 0;
+            F x = kid[gp * 16 + i];
+            x = p16_row(x, i, pc->p16);
+            if (g < np && i < 8) {
+                par[g * 8 + i] = x;
+                nodes[(first + g) * 8 + i] = x;
+            }
         }
         __syncthreads();
         for (int w = threadIdx.x; w < np * 8; w += blockDim.x) kid[w] = par[w];

@@ -49,6 +49,9 @@ void reduce_publish_await(const EF *in, int n_items, int n_parts, EF *host_dst, 
 // eq(point, .) over n <= 32 variables, big-endian, times scale (point by value).
 struct EqPoint { EF p[32]; EF scale; int n; };
 void eq_table_v(const EqPoint &pt, EF *out, hipStream_t s);
+// count tables eq(pts[j], .) * scale_j into outs[j] in one launch (pts and outs in
+// device memory); max_n = the largest pts[j].n
+void eq_tables(const EqPoint *pts, EF *const *outs, int count, int max_n, hipStream_t s);
 struct State16 { F s[16]; };
 
 // ------------------------------------------------------------ zerocheck --

@@ -184,12 +184,13 @@ void add_constraints_w(wl_ctx *c, EF *W, int m, const std::vector<Constraint> &c
         wk::eq_univ_f_tables(up.at<F>(o_yh), nF, hi, hiF, c->stream);
         wk::eq_univ_f_tables(up.at<F>(o_yl), nF, lo, loF, c->stream);
     }
+    std::vector<EqJob> eqs;
     for (int j = 0; j < nE; j++) {
         std::vector<EF> pt = cons_point(*efc[j], m);
-        std::vector<EF> ph(pt.begin(), pt.begin() + hi), pl(pt.begin() + hi, pt.end());
-        eq_into(c, ph, eone(), hiE.p + (size_t)j * HI);
-        eq_into(c, pl, eone(), loE.p + (size_t)j * LO);
+        eqs.push_back(EqJob{std::vector<EF>(pt.begin(), pt.begin() + hi), eone(), hiE.p + (size_t)j * HI});
+        eqs.push_back(EqJob{std::vector<EF>(pt.begin() + hi, pt.end()), eone(), loE.p + (size_t)j * LO});
     }
+    eq_batch(c, eqs);
     wk::whir_build_w(W, m, lo, nF, dsF, hiF, loF, nE, dsE, hiE, loE, !zero_first, c->stream);
 }
 
@@ -284,13 +285,13 @@ EF eval_f_univ(wl_ctx *c, const F *f, int m, size_t nz_len, const EF &z) {
     std::vector<EF> pt = wh::expand_univ(z, m);
     int k = m / 2;
     std::vector<EF> ph(pt.begin(), pt.begin() + k), pl(pt.begin() + k, pt.end());
-    DBuf<EF> El = eq_dev(c, pl, eone());
+    DBuf<EF> El(c->pool, (size_t)1 << pl.size()), Eh(c->pool, (size_t)1 << ph.size());
+    eq_batch(c, {EqJob{pl, eone(), El.p}, EqJob{ph, eone(), Eh.p}});
     int nc = wk::whir_T_chunks(m, k);
     DBuf<EF> parts(c->pool, ((size_t)1 << k) * nc);
     wk::whir_T_f(f, m, k, El, 1, parts, nz_len, c->stream);
     DBuf<EF> T(c->pool, (size_t)1 << k);
     wk::reduce_parts(parts, 1 << k, nc, T, c->stream);
-    DBuf<EF> Eh = eq_dev(c, ph, eone());
     int nb = wk::dot_blocks((size_t)1 << k);
     DBuf<EF> p2(c->pool, nb);
     wk::seg_dot(T, Eh, (size_t)1 << k, 1, p2, c->stream);
@@ -350,12 +351,14 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
     int X = v - f0, U = 1 << f0, nC = (int)cs.size();
     DBuf<EF> E(c->pool, (size_t)nC << X);
     std::vector<std::vector<EF>> A(nC);
+    std::vector<EqJob> eqs;
     for (int j = 0; j < nC; j++) {
         std::vector<EF> pt = cons_point(cs[j], v);
-        std::vector<EF> pre(pt.begin(), pt.begin() + f0), suf(pt.begin() + f0, pt.end());
+        std::vector<EF> pre(pt.begin(), pt.begin() + f0);
         A[j] = wh::eq_host(pre);
-        eq_into(c, suf, eone(), E.p + ((size_t)j << X));
+        eqs.push_back(EqJob{std::vector<EF>(pt.begin() + f0, pt.end()), eone(), E.p + ((size_t)j << X)});
     }
+    eq_batch(c, eqs);
     int nch = wk::whir_T_chunks(v, f0);
     DBuf<EF> parts(c->pool, (size_t)nC * U * nch);
     wk::whir_T_f(w.evals, v, f0, E, nC, parts, w.nz_len, c->stream);
@@ -515,8 +518,12 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     std::vector<EF> ch;
     EF sum = ezero(), missing = eone();
     DBuf<EF> U(c->pool, (size_t)AIR_NCOLS * M);
+    // eq(zc[1..], .) for round 0 and eq(zc[2..], .) for round 1 in one launch
+    DBuf<EF> eq1(c->pool, (size_t)1 << (nz - 2));
     {
-        DBuf<EF> eq = eq_dev(c, std::vector<EF>(zc.begin() + 1, zc.end()), eone());
+        DBuf<EF> eq(c->pool, (size_t)1 << (nz - 1));
+        eq_batch(c, {EqJob{std::vector<EF>(zc.begin() + 1, zc.end()), eone(), eq.p},
+                     EqJob{std::vector<EF>(zc.begin() + 2, zc.end()), eone(), eq1.p}});
         int zmax = 3 * (K - 1);
         int nzp = zmax - K + 1;  // z = K .. zmax
         std::vector<F> sel;
@@ -566,7 +573,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     // eq(zc[1 + j ..], .) for round j; each fold launch halves it for the next round.
     // Pipelined: the fold by round j's challenge and round j + 1 are enqueued
     // behind a device wait for that challenge before the host computes it.
-    DBuf<EF> eq = eq_dev(c, std::vector<EF>(zc.begin() + 2, zc.end()), eone());
+    DBuf<EF> eq = std::move(eq1);
     // round ll's kernels: the fold by the previous challenge (from 2^(ll+1), when
     // cdev is given), the evaluation, and the reduction + publish that then waits
     // for this round's challenge (the returned slot)
