@@ -244,8 +244,9 @@ struct wl_ctx {
     void release_all() {
         if (mbox) __atomic_store_n(&mbox[0], mbox_next, __ATOMIC_RELEASE);
     }
-    // copy `bytes` of device memory to dst through the mapped buffer (waits)
-    void fetch(void *dst, const void *src, size_t bytes) {
+    // copy `bytes` of device memory to dst through the mapped buffer (waits); the
+    // first n_canon words are F values delivered in canonical form
+    void fetch(void *dst, const void *src, size_t bytes, size_t n_canon = 0) {
         if (!bytes) return;
         ensure_mapped();
         size_t words = (bytes + 3) / 4;
@@ -253,11 +254,12 @@ struct wl_ctx {
             uint32_t *st = stage(words);
             HIP_CHECK(hipMemcpyAsync(st, src, bytes, hipMemcpyDeviceToHost, stream));
             HIP_CHECK(hipStreamSynchronize(stream));
+            for (size_t i = 0; i < std::min(n_canon, words); i++) st[i] = kb::from_mont(kb::F{st[i]});
             memcpy(dst, st, bytes);
             return;
         }
         uint32_t s = ++seq;
-        wk::publish((const uint32_t *)src, words, mapped_dev, flag_dev(), s, publish_counter(), stream);
+        wk::publish((const uint32_t *)src, words, mapped_dev, flag_dev(), s, publish_counter(), stream, n_canon);
         wait_flag(s);
         memcpy(dst, mapped, bytes);
     }

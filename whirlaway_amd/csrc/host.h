@@ -181,6 +181,7 @@ struct Transcript {
     }
     void add_ef(const EF *v, size_t n) { for (size_t i = 0; i < n; i++) add_base(v[i].c, 8); }
     void hint_base(const F *v, size_t n) { for (size_t i = 0; i < n; i++) data.push_back(from_mont(v[i])); }
+    void hint_canonical(const uint32_t *v, size_t n) { data.insert(data.end(), v, v + n); }
 };
 
 // ----------------------------------------------------------- WHIR schedule --

@@ -60,18 +60,31 @@ void reduce_parts(const EF *in, int n_items, int n_parts, EF *out, hipStream_t s
     k_reduce_parts<<<n_items, 256, 0, s>>>(in, n_parts, out);
 }
 
-// blocks copy interleaved 16-byte pieces; with several blocks the last one to
-// finish (counter, left at 0) stores the flag
+// blocks copy interleaved 16-byte pieces, taking words below n_canon out of
+// Montgomery form on the way; with several blocks the last one to finish
+// (counter, left at 0) stores the flag
+__device__ __forceinline__ uint32_t pub_word(uint32_t v, size_t i, size_t n_canon) {
+    return i < n_canon ? from_mont(F{v}) : v;
+}
 __global__ void k_publish(const uint32_t *src, size_t n, uint32_t *dst, volatile uint32_t *flag, uint32_t seq,
-                          uint32_t *counter) {
+                          uint32_t *counter, size_t n_canon) {
     const size_t n4 = n / 4, stride = (size_t)gridDim.x * blockDim.x;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool al = ((((uintptr_t)src) | ((uintptr_t)dst)) & 15) == 0;
     if (al) {
-        for (size_t i = t; i < n4; i += stride) ((uint4 *)dst)[i] = ((const uint4 *)src)[i];
-        for (size_t i = 4 * n4 + t; i < n; i += stride) dst[i] = src[i];
+        for (size_t i = t; i < n4; i += stride) {
+            uint4 v = ((const uint4 *)src)[i];
+            if (4 * i < n_canon) {
+                v.x = pub_word(v.x, 4 * i, n_canon);
+                v.y = pub_word(v.y, 4 * i + 1, n_canon);
+                v.z = pub_word(v.z, 4 * i + 2, n_canon);
+                v.w = pub_word(v.w, 4 * i + 3, n_canon);
+            }
+            ((uint4 *)dst)[i] = v;
+        }
+        for (size_t i = 4 * n4 + t; i < n; i += stride) dst[i] = pub_word(src[i], i, n_canon);
     } else {
-        for (size_t i = t; i < n; i += stride) dst[i] = src[i];
+        for (size_t i = t; i < n; i += stride) dst[i] = pub_word(src[i], i, n_canon);
     }
     __threadfence_system();
     __syncthreads();
@@ -84,10 +97,10 @@ __global__ void k_publish(const uint32_t *src, size_t n, uint32_t *dst, volatile
     *flag = seq;
 }
 void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
-             uint32_t *counter, hipStream_t s) {
+             uint32_t *counter, hipStream_t s, size_t n_canon) {
     // one block up to 16 KiB, then a block per 16 KiB, at most 64
     const int blocks = (int)std::min<size_t>(64, std::max<size_t>(1, (n + 4095) / 4096));
-    k_publish<<<blocks, 1024, 0, s>>>(src, n, host_dst, flag, seq, counter);
+    k_publish<<<blocks, 1024, 0, s>>>(src, n, host_dst, flag, seq, counter, n_canon);
 }
 // rows of row_words words, stride_words apart in device memory, packed into host memory
 __global__ void k_publish_rows(const uint32_t *src, int rows, int row_words, size_t stride_words, uint32_t *dst,

@@ -32,8 +32,9 @@ void reduce_parts(const EF *in, int n_items, int n_parts, EF *out, hipStream_t s
 // Copy n words to host-mapped memory, then (after a system-scope fence) store
 // seq into *flag; the host waits on the flag instead of synchronizing the stream.
 // counter: a device word at 0, used (and left at 0) when several blocks copy.
+// Words [0, n_canon) are F values converted from Montgomery to canonical form.
 void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
-             uint32_t *counter, hipStream_t s);
+             uint32_t *counter, hipStream_t s, size_t n_canon = 0);
 // rows x row_words words, stride_words apart, packed into host_dst, then flag = seq
 void publish_rows(const uint32_t *src, int rows, int row_words, size_t stride_words, uint32_t *host_dst,
                   volatile uint32_t *flag, uint32_t seq, hipStream_t s);

@@ -95,11 +95,13 @@ std::vector<EF> open_queries(wl_ctx *c, Transcript &ts, const Codeword &cwd, con
                     buf.p + nrow, c->stream);
     if (eqr)
         wk::fold_rows(buf.p, nq, cwd.fold, cwd.base, up.at<EF>(o_eq), (EF *)(buf.p + nrow + npath), c->stream);
-    std::vector<F> host(nrow + npath + nval);
-    c->d2h(host.data(), buf.p, host.size());
+    // rows and paths arrive in canonical form (converted by the copy kernel), the
+    // folded values in Montgomery form
+    std::vector<uint32_t> host(nrow + npath + nval);
+    c->fetch(host.data(), buf.p, host.size() * sizeof(uint32_t), nrow + npath);
     for (int i = 0; i < nq; i++) {
-        ts.hint_base(&host[(size_t)i * cwd.width], cwd.width);
-        ts.hint_base(&host[nrow + (size_t)i * cwd.log_rows * 8], (size_t)cwd.log_rows * 8);
+        ts.hint_canonical(&host[(size_t)i * cwd.width], cwd.width);
+        ts.hint_canonical(&host[nrow + (size_t)i * cwd.log_rows * 8], (size_t)cwd.log_rows * 8);
     }
     if (eqr) {
         vals.resize(nq);
