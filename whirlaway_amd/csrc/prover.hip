@@ -280,24 +280,28 @@ struct WhirWitness {
     size_t nz_len;    // evals[i] = 0 for i >= nz_len
     Codeword cw0;
     std::vector<EF> ood_points, ood_answers;
+    std::vector<std::vector<EF>> ood_T;  // T tables of the OOD points (whir_T_tables)
 };
 
-// evaluate a device F table (2^m) at expand(z, m) with one T-style pass
-EF eval_f_univ(wl_ctx *c, const F *f, int m, size_t nz_len, const EF &z) {
-    std::vector<EF> pt = wh::expand_univ(z, m);
-    int k = m / 2;
-    std::vector<EF> ph(pt.begin(), pt.begin() + k), pl(pt.begin() + k, pt.end());
-    DBuf<EF> El(c->pool, (size_t)1 << pl.size()), Eh(c->pool, (size_t)1 << ph.size());
-    eq_batch(c, {EqJob{pl, eone(), El.p}, EqJob{ph, eone(), Eh.p}});
-    int nc = wk::whir_T_chunks(m, k);
-    DBuf<EF> parts(c->pool, ((size_t)1 << k) * nc);
-    wk::whir_T_f(f, m, k, El, 1, parts, nz_len, c->stream);
-    DBuf<EF> T(c->pool, (size_t)1 << k);
-    wk::reduce_parts(parts, 1 << k, nc, T, c->stream);
-    int nb = wk::dot_blocks((size_t)1 << k);
-    DBuf<EF> p2(c->pool, nb);
-    wk::seg_dot(T, Eh, (size_t)1 << k, 1, p2, c->stream);
-    return reduce_one(c, p2, nb);
+// T_j(u) = sum_x f(u 2^(v-f0) + x) eq(suf_j, x) for u < 2^f0: the packed
+// polynomial summed against the eq tables of the variables after the first f0,
+// for several points in one pass (one eq launch, one pass over f, one reduction).
+// f(pre_j, suf_j) is then sum_u eq(pre_j, u) T_j(u).
+std::vector<std::vector<EF>> whir_T_tables(wl_ctx *c, const F *f, int v, int f0, size_t nz_len,
+                                           const std::vector<std::vector<EF>> &sufs) {
+    const int X = v - f0, U = 1 << f0, nC = (int)sufs.size();
+    std::vector<std::vector<EF>> out(nC);
+    if (nC == 0) return out;
+    DBuf<EF> E(c->pool, (size_t)nC << X);
+    std::vector<EqJob> eqs;
+    for (int j = 0; j < nC; j++) eqs.push_back(EqJob{sufs[j], eone(), E.p + ((size_t)j << X)});
+    eq_batch(c, eqs);
+    const int nch = wk::whir_T_chunks(v, f0);
+    DBuf<EF> parts(c->pool, (size_t)nC * U * nch);
+    wk::whir_T_f(f, v, f0, E, nC, parts, nz_len, c->stream);
+    std::vector<EF> T = reduce_many(c, parts, nC * U, nch);
+    for (int j = 0; j < nC; j++) out[j].assign(T.begin() + (size_t)j * U, T.begin() + (size_t)(j + 1) * U);
+    return out;
 }
 
 EF eval_ef_univ(wl_ctx *c, const EF *f, int m, const EF &z) {
@@ -327,7 +331,21 @@ void whir_commit(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &t
     c->d2h(root, w.cw0.nodes.p + 8, 8);
     ts.add_base(root, 8);
     for (int i = 0; i < cfg.commit_ood_samples; i++) w.ood_points.push_back(ts.sample_ef());
-    for (int i = 0; i < cfg.commit_ood_samples; i++) w.ood_answers.push_back(eval_f_univ(c, evals, v, w.nz_len, w.ood_points[i]));
+    // the OOD answers f(expand(z_i)) through the T tables that the opening's first
+    // f0 sumcheck rounds use again
+    std::vector<std::vector<EF>> pres, sufs;
+    for (const EF &z : w.ood_points) {
+        std::vector<EF> pt = wh::expand_univ(z, v);
+        pres.emplace_back(pt.begin(), pt.begin() + cfg.f0);
+        sufs.emplace_back(pt.begin() + cfg.f0, pt.end());
+    }
+    w.ood_T = whir_T_tables(c, evals, v, cfg.f0, w.nz_len, sufs);
+    for (size_t i = 0; i < w.ood_points.size(); i++) {
+        std::vector<EF> a = wh::eq_host(pres[i]);
+        EF ans = ezero();
+        for (size_t u = 0; u < a.size(); u++) ans = eadd(ans, emul(a[u], w.ood_T[i][u]));
+        w.ood_answers.push_back(ans);
+    }
     ts.add_ef(w.ood_answers.data(), w.ood_answers.size());
 }
 
@@ -350,23 +368,16 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
     }
     // initial f0 rounds: T_j(u) = sum_x f(u, x) eq(pt_j[f0..], x) on the device,
     // then the f0 round polynomials from the 2^f0-entry tables.
-    int X = v - f0, U = 1 << f0, nC = (int)cs.size();
-    DBuf<EF> E(c->pool, (size_t)nC << X);
-    std::vector<std::vector<EF>> A(nC);
-    std::vector<EqJob> eqs;
+    // the OOD constraints come first in cs; their T tables were built at commit time
+    const int X = v - f0, nC = (int)cs.size(), nood = (int)w.ood_T.size();
+    std::vector<std::vector<EF>> A(nC), sufs;
     for (int j = 0; j < nC; j++) {
         std::vector<EF> pt = cons_point(cs[j], v);
-        std::vector<EF> pre(pt.begin(), pt.begin() + f0);
-        A[j] = wh::eq_host(pre);
-        eqs.push_back(EqJob{std::vector<EF>(pt.begin() + f0, pt.end()), eone(), E.p + ((size_t)j << X)});
+        A[j] = wh::eq_host(std::vector<EF>(pt.begin(), pt.begin() + f0));
+        if (j >= nood) sufs.emplace_back(pt.begin() + f0, pt.end());
     }
-    eq_batch(c, eqs);
-    int nch = wk::whir_T_chunks(v, f0);
-    DBuf<EF> parts(c->pool, (size_t)nC * U * nch);
-    wk::whir_T_f(w.evals, v, f0, E, nC, parts, w.nz_len, c->stream);
-    std::vector<EF> T = reduce_many(c, parts, nC * U, nch);
-    std::vector<std::vector<EF>> Tj(nC);
-    for (int j = 0; j < nC; j++) Tj[j].assign(T.begin() + (size_t)j * U, T.begin() + (size_t)(j + 1) * U);
+    std::vector<std::vector<EF>> Tj = w.ood_T;
+    for (auto &t : whir_T_tables(c, w.evals, v, f0, w.nz_len, sufs)) Tj.push_back(std::move(t));
     std::vector<EF> rf;
     for (int t = 0; t < f0; t++) {
         size_t h = Tj[0].size() / 2;
