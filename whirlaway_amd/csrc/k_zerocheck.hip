@@ -9,9 +9,11 @@
 //   - the 20 partial rounds and the input of ending round 0, whose states are
 //     fixed linear combinations of columns (ChainLin), evaluated as dot products,
 //   - the external MDS row and the alpha-weighted constraint of each of the
-//     128 full-round outputs.
-// The folded column values of each point and all constants (alpha powers,
-// round constants, chain coefficients) live in LDS.  Large rounds give each
+//     128 full-round outputs; the 4x4 blocks of every external layer are
+//     computed once per state (32 block tasks, then rows = own block output +
+//     the four blocks' sum of its residue class), not once per row.
+// The folded column values of each point live in LDS, the constants (alpha
+// powers, round constants, chain coefficients) are read through the caches.  Large rounds give each
 // point one wave (4 points per block); small rounds, which are latency-bound,
 // give each point the whole 256-lane block.
 #include "common.h"
@@ -24,24 +26,23 @@ namespace {
 
 constexpr int NC = 165;
 
-__device__ __forceinline__ EF m4row(const EF *v, int r) {
-    // row r of [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]] applied to v[0..4)
-    EF s = eadd(eadd(v[0], v[1]), eadd(v[2], v[3]));
-    switch (r) {
-    case 0: return eadd(s, eadd(v[0], edbl(v[1])));
-    case 1: return eadd(s, eadd(v[1], edbl(v[2])));
-    case 2: return eadd(s, eadd(v[2], edbl(v[3])));
-    default: return eadd(s, eadd(edbl(v[0]), v[3]));
-    }
+// the 4x4 block of the external layer on v[0..4) in place: rows
+// [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]]
+__device__ __forceinline__ void m4_block(EF *v) {
+    const EF t01 = eadd(v[0], v[1]), t23 = eadd(v[2], v[3]);
+    const EF t0123 = eadd(t01, t23);
+    const EF t01123 = eadd(t0123, v[1]), t01233 = eadd(t0123, v[3]);
+    const EF y3 = eadd(t01233, edbl(v[0])), y1 = eadd(t01123, edbl(v[2]));
+    v[0] = eadd(t01123, t01);
+    v[1] = y1;
+    v[2] = eadd(t01233, t23);
+    v[3] = y3;
 }
-
-// row i of the external layer (mds_light): M4 per block, then each lane adds the
-// sum over blocks of its residue class.
-__device__ __forceinline__ EF mds_row(const EF *v, int i) {
-    int r = i & 3;
-    EF own = m4row(v + (i & ~3), r);
-    EF s = eadd(eadd(m4row(v, r), m4row(v + 4, r)), eadd(m4row(v + 8, r), m4row(v + 12, r)));
-    return eadd(s, own);
+// row i of the external layer once every block of y holds its m4_block output:
+// y_i plus the sum over the four blocks of residue class i mod 4
+__device__ __forceinline__ EF mds_out(const EF *y, int i) {
+    const int r = i & 3;
+    return eadd(y[i], eadd(eadd(y[r], y[4 + r]), eadd(y[8 + r], y[12 + r])));
 }
 
 __device__ __forceinline__ EF add_rc(EF t, F rc) {
@@ -83,6 +84,24 @@ __global__ void __launch_bounds__(256) k_zc_round_w(const EF *U, size_t stride, 
         }
     }
     __syncthreads();
+    // the external layer on the 16 input columns (the state entering beginning
+    // round 0): blocks first, then the rows, shared by the point's lanes; kept in
+    // sb[pt][0..16), which the S-box of the same element overwrites after reading
+    EF *ext_in = sb[pt];
+    if (act && lane < 4) {
+        EF v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = cw[1 + 4 * lane + q];
+        m4_block(v);
+#pragma unroll
+        for (int q = 0; q < 4; q++) ext_in[4 * lane + q] = v[q];
+    }
+    __syncthreads();
+    EF ext_row = ezero();
+    if (act && lane < 16) ext_row = mds_out(ext_in, lane);
+    __syncthreads();
+    if (act && lane < 16) ext_in[lane] = ext_row;
+    __syncthreads();
     EF acc = ezero();
     if (act) {
         // phase 1: S-boxes (112 full-round tasks) and the linear chain (20 + 16 tasks)
@@ -92,7 +111,7 @@ __global__ void __launch_bounds__(256) k_zc_round_w(const EF *U, size_t stride, 
             F rc;
             int slot;
             if (r8 < 4) {
-                prev = r8 == 0 ? mds_row(cw + 1, i) : cw[17 + 16 * (r8 - 1) + i];
+                prev = r8 == 0 ? ext_in[i] : cw[17 + 16 * (r8 - 1) + i];
                 rc = K.rc[r8 * 16 + i];
                 slot = r8;
             } else {
@@ -132,11 +151,16 @@ __global__ void __launch_bounds__(256) k_zc_round_w(const EF *U, size_t stride, 
         }
     }
     __syncthreads();
+    // phase 2: the external layer of the 8 full rounds (32 block tasks in place,
+    // then 128 rows) and the full-round constraints
     if (act) {
-        // phase 2: external layer rows and the full-round constraints
+        for (int t = lane; t < 32; t += LANES) m4_block(&sb[pt][4 * t]);
+    }
+    __syncthreads();
+    if (act) {
         for (int t = lane; t < 128; t += LANES) {
             int r8 = t >> 4, i = t & 15;  // 0..3 beginning, 4..7 ending rounds 0..3
-            EF out = mds_row(&sb[pt][r8 * 16], i);
+            EF out = mds_out(&sb[pt][r8 * 16], i);
             int post = r8 < 4 ? 17 + 16 * r8 + i : 101 + 16 * (r8 - 4) + i;
             int k = r8 < 4 ? 1 + 16 * r8 + i : 85 + 16 * (r8 - 4) + i;
             acc = eadd(acc, emul(K.ap[k], esub(out, cw[post])));
