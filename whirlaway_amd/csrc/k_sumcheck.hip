@@ -428,7 +428,19 @@ __global__ void k_zc_fold0(const F *trace, int log_n, int skips, const EF *S, EF
     if (x >= M) return;
     const F *col = trace + (size_t)c * N;
     AccF acc;
-    for (int j = 0; j < K; j++) {
+    int j = 0;
+    for (; j + 8 <= K; j += 8) {  // eight loads in flight per thread
+        F v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            size_t idx = (size_t)(j + q) * M + x;
+            if (idx == N - 1) idx = N - 2;
+            v[q] = col[idx];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc.add(S[j + q], v[q]);
+    }
+    for (; j < K; j++) {
         size_t idx = (size_t)j * M + x;
         if (idx == N - 1) idx = N - 2;
         acc.add(S[j], col[idx]);
@@ -483,9 +495,22 @@ __global__ void k_zc_down_parts(const F *trace, int log_n, int skips, const EF *
     const int c = blockIdx.x, j = blockIdx.y, K = 1 << skips;
     const F *col = trace + (size_t)c * N;
     AccF acc;
-    for (size_t i = threadIdx.x; i < M; i += blockDim.x) {
+    size_t i = threadIdx.x;
+    // eight loads in flight per thread
+    for (; i + 7 * (size_t)blockDim.x < M; i += 8 * (size_t)blockDim.x) {
+        F v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            size_t idx = (size_t)j * M + i + (size_t)q * blockDim.x + 1;
+            if (idx > N - 1) idx = N - 1; /* column_down: last row repeats */
+            v[q] = col[idx];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc.add(eqr[i + (size_t)q * blockDim.x], v[q]);
+    }
+    for (; i < M; i += blockDim.x) {
         size_t idx = (size_t)j * M + i + 1;
-        if (idx > N - 1) idx = N - 1; /* column_down: last row repeats */
+        if (idx > N - 1) idx = N - 1;
         acc.add(eqr[i], col[idx]);
     }
     __shared__ EF lds[16];
@@ -512,7 +537,17 @@ __global__ void k_lincomb(const F *trace, int log_n, int n_cols, const EF *eqc, 
     const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= N) return;
     AccF acc;
-    for (int c = 0; c < n_cols; c++) acc.add(eqc[c], trace[(size_t)c * N + x]);
+    // eight column loads in flight per thread (one at a time leaves the loop
+    // waiting on memory latency)
+    int c = 0;
+    for (; c + 8 <= n_cols; c += 8) {
+        F v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = trace[(size_t)(c + q) * N + x];
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc.add(eqc[c + q], v[q]);
+    }
+    for (; c < n_cols; c++) acc.add(eqc[c], trace[(size_t)c * N + x]);
     B[x] = acc.get();
 }
 void lincomb_cols(const F *trace, int log_n, int n_cols, const EF *eqc, EF *B, hipStream_t s) {

@@ -691,28 +691,39 @@ void p2_permute_batch(const void *pc, int width, F *states, int n, hipStream_t s
 // ------------------------------------------------------- WHIR reductions ----
 using AccF2 = AccF;
 
-constexpr int T_JG = 2;
-__global__ void k_whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts, size_t nz_len) {
+// T_UG rows u per block share each E[j][x] load (E is re-read once per row
+// group instead of once per row); one constraint j per blockIdx.z.
+constexpr int T_UG = 4;
+__global__ void __launch_bounds__(256) k_whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts,
+                                                  size_t nz_len) {
     const size_t X = (size_t)1 << (m - k);
     const int U = 1 << k;
-    const int u = blockIdx.y;
-    const int j0 = blockIdx.z * T_JG;
-    AccF2 acc[T_JG];
+    const int u0 = blockIdx.y * T_UG;
+    const int j = blockIdx.z;
+    AccF2 acc[T_UG];
     const size_t st = (size_t)gridDim.x * blockDim.x;
-    const F *fu = f + (size_t)u * X;
-    // f is zero from nz_len on: chunks past it contribute zero sums
-    const size_t xend = (size_t)u * X >= nz_len ? 0 : (nz_len - (size_t)u * X < X ? nz_len - (size_t)u * X : X);
-    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < xend; x += st) {
-        F v = fu[x];
+    const EF *Ej = E + (size_t)j * X;
+    // f is zero from nz_len on: rows (and row tails) past it contribute zero sums
+    size_t xend[T_UG], xmax = 0;
 #pragma unroll
-        for (int jj = 0; jj < T_JG; jj++)
-            if (j0 + jj < nE) acc[jj].add(E[(size_t)(j0 + jj) * X + x], v);
+    for (int g = 0; g < T_UG; g++) {
+        const size_t base = (size_t)(u0 + g) * X;
+        xend[g] = u0 + g >= U || base >= nz_len ? 0 : (nz_len - base < X ? nz_len - base : X);
+        xmax = xend[g] > xmax ? xend[g] : xmax;
+    }
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < xmax; x += st) {
+        const EF e = Ej[x];
+        F v[T_UG];
+#pragma unroll
+        for (int g = 0; g < T_UG; g++) v[g] = x < xend[g] ? f[(size_t)(u0 + g) * X + x] : fzero();
+#pragma unroll
+        for (int g = 0; g < T_UG; g++) acc[g].add(e, v[g]);
     }
     __shared__ EF lds[16];
 #pragma unroll
-    for (int jj = 0; jj < T_JG; jj++) {
-        EF t = block_sum(acc[jj].get(), lds);
-        if (threadIdx.x == 0 && j0 + jj < nE) parts[((size_t)(j0 + jj) * U + u) * gridDim.x + blockIdx.x] = t;
+    for (int g = 0; g < T_UG; g++) {
+        EF t = block_sum(acc[g].get(), lds);
+        if (threadIdx.x == 0 && u0 + g < U) parts[((size_t)j * U + u0 + g) * gridDim.x + blockIdx.x] = t;
     }
 }
 int whir_T_chunks(int m, int k) {
@@ -722,7 +733,7 @@ int whir_T_chunks(int m, int k) {
 }
 int whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts, size_t nz_len, hipStream_t s) {
     unsigned nc = (unsigned)whir_T_chunks(m, k);
-    k_whir_T_f<<<dim3(nc, 1 << k, (nE + T_JG - 1) / T_JG), 256, 0, s>>>(f, m, k, E, nE, parts, nz_len);
+    k_whir_T_f<<<dim3(nc, ((1 << k) + T_UG - 1) / T_UG, nE), 256, 0, s>>>(f, m, k, E, nE, parts, nz_len);
     return (int)nc;
 }
 
@@ -731,7 +742,18 @@ __global__ void k_whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, 
     size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= X) return;
     AccF2 acc;
-    for (int u = 0; u < (1 << k) && (size_t)u * X + x < nz_len; u++) acc.add(eqr[u], f[(size_t)u * X + x]);
+    // rows u with u X + x < nz_len, eight loads in flight per thread
+    const size_t urem = x < nz_len ? (nz_len - x + X - 1) / X : 0;
+    const int un = (int)(urem < ((size_t)1 << k) ? urem : ((size_t)1 << k));
+    int u = 0;
+    for (; u + 8 <= un; u += 8) {
+        F v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = f[(size_t)(u + q) * X + x];
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc.add(eqr[u + q], v[q]);
+    }
+    for (; u < un; u++) acc.add(eqr[u], f[(size_t)u * X + x]);
     out[x] = acc.get();
 }
 void whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, size_t nz_len, hipStream_t s) {
