@@ -761,24 +761,34 @@ void whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, size_t nz_len
     k_whir_fold_f<<<nblk(X), 256, 0, s>>>(f, m, k, eqr, out, nz_len);
 }
 
+// the hiE tables already carry their constraint's scalar
 __global__ void k_build_w(EF *W, size_t Nw, int lo, int nF, const EF *scalF, const F *hiF,
-                          const F *loF, int nE, const EF *scalE, const EF *hiE, const EF *loE,
-                          bool accumulate) {
+                          const F *loF, int nE, const EF *hiE, const EF *loE, bool accumulate) {
     size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= Nw) return;
     const size_t LO = (size_t)1 << lo, HI = Nw >> lo;
     size_t xh = x >> lo, xl = x & (LO - 1);
     AccF2 acc;
-    for (int j = 0; j < nF; j++) acc.add(scalF[j], fmul(hiF[(size_t)j * HI + xh], loF[(size_t)j * LO + xl]));
+    int j = 0;
+    for (; j + 8 <= nF; j += 8) {  // eight load pairs in flight per thread
+        F hv[8], lv[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            hv[q] = hiF[(size_t)(j + q) * HI + xh];
+            lv[q] = loF[(size_t)(j + q) * LO + xl];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc.add(scalF[j + q], fmul(hv[q], lv[q]));
+    }
+    for (; j < nF; j++) acc.add(scalF[j], fmul(hiF[(size_t)j * HI + xh], loF[(size_t)j * LO + xl]));
     EF r = acc.get();
-    for (int j = 0; j < nE; j++)
-        r = eadd(r, emul(scalE[j], emul(hiE[(size_t)j * HI + xh], loE[(size_t)j * LO + xl])));
+    for (int e = 0; e < nE; e++) r = eadd(r, emul(hiE[(size_t)e * HI + xh], loE[(size_t)e * LO + xl]));
     W[x] = accumulate ? eadd(W[x], r) : r;
 }
 void whir_build_w(EF *W, int m, int lo, int nF, const EF *scalF, const F *hiF, const F *loF, int nE,
-                  const EF *scalE, const EF *hiE, const EF *loE, bool accumulate, hipStream_t s) {
+                  const EF *hiE, const EF *loE, bool accumulate, hipStream_t s) {
     size_t Nw = (size_t)1 << m;
-    k_build_w<<<nblk(Nw), 256, 0, s>>>(W, Nw, lo, nF, scalF, hiF, loF, nE, scalE, hiE, loE, accumulate);
+    k_build_w<<<nblk(Nw), 256, 0, s>>>(W, Nw, lo, nF, scalF, hiF, loF, nE, hiE, loE, accumulate);
 }
 
 __global__ void k_eq_univ_f(const F *ys, int nY, int n, F *out) {

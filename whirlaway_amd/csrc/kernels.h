@@ -138,11 +138,11 @@ struct P2Dev;  // device Poseidon2 constants (16 and 24)
 int whir_T_f(const F *f, int m, int k, const EF *E, int nE, EF *parts, size_t nz_len, hipStream_t s);
 // f'(x) = sum_u eqr[u] f(u*2^(m-k) + x)  (f[i] = 0 for i >= nz_len, not read)
 void whir_fold_f(const F *f, int m, int k, const EF *eqr, EF *out, size_t nz_len, hipStream_t s);
-// W[x] = sum_j scal[j] * Ehi_j[x >> lo] * Elo_j[x & mask]; F-point tables and EF-point tables
-// (accumulate: W += ..., else W = ...)
-void whir_build_w(EF *W, int m, int lo, int nF, const EF *scalF, const F *hiF, const F *loF,
-                  int nE, const EF *scalE, const EF *hiE, const EF *loE, bool accumulate,
-                  hipStream_t s);
+// W[x] = sum_j scalF[j] * hiF_j[x >> lo] * loF_j[x & mask] + sum_e hiE_e[x >> lo] * loE_e[x & mask]
+// (F-point tables; EF-point tables whose hi part already carries the scalar);
+// accumulate: W += ..., else W = ...
+void whir_build_w(EF *W, int m, int lo, int nF, const EF *scalF, const F *hiF, const F *loF, int nE,
+                  const EF *hiE, const EF *loE, bool accumulate, hipStream_t s);
 // eq(expand(y_j, n), .) tables over F for nY points y (device), out [j][2^n]
 void eq_univ_f_tables(const F *ys, int nY, int n, F *out, hipStream_t s);
 // AoS EF (2^m) -> SoA F columns: out[(b*8 + c)*2^(m-k) + i] = in[b*2^(m-k) + i].c[c]

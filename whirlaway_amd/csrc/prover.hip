@@ -179,9 +179,9 @@ void add_constraints_w(wl_ctx *c, EF *W, int m, const std::vector<Constraint> &c
     DBuf<F> hiF(c->pool, std::max<size_t>(1, nF * HI)), loF(c->pool, std::max<size_t>(1, nF * LO));
     DBuf<EF> hiE(c->pool, std::max<size_t>(1, nE * HI)), loE(c->pool, std::max<size_t>(1, nE * LO));
     Uploads up(c);
-    const size_t o_sF = up.add(sF), o_sE = up.add(sE), o_yl = up.add(yl), o_yh = up.add(yh);
+    const size_t o_sF = up.add(sF), o_yl = up.add(yl), o_yh = up.add(yh);
     up.send();
-    const EF *dsF = up.at<EF>(o_sF), *dsE = up.at<EF>(o_sE);
+    const EF *dsF = up.at<EF>(o_sF);
     if (nF) {
         wk::eq_univ_f_tables(up.at<F>(o_yh), nF, hi, hiF, c->stream);
         wk::eq_univ_f_tables(up.at<F>(o_yl), nF, lo, loF, c->stream);
@@ -189,11 +189,11 @@ void add_constraints_w(wl_ctx *c, EF *W, int m, const std::vector<Constraint> &c
     std::vector<EqJob> eqs;
     for (int j = 0; j < nE; j++) {
         std::vector<EF> pt = cons_point(*efc[j], m);
-        eqs.push_back(EqJob{std::vector<EF>(pt.begin(), pt.begin() + hi), eone(), hiE.p + (size_t)j * HI});
+        eqs.push_back(EqJob{std::vector<EF>(pt.begin(), pt.begin() + hi), sE[j], hiE.p + (size_t)j * HI});
         eqs.push_back(EqJob{std::vector<EF>(pt.begin() + hi, pt.end()), eone(), loE.p + (size_t)j * LO});
     }
     eq_batch(c, eqs);
-    wk::whir_build_w(W, m, lo, nF, dsF, hiF, loF, nE, dsE, hiE, loE, !zero_first, c->stream);
+    wk::whir_build_w(W, m, lo, nF, dsF, hiF, loF, nE, hiE, loE, !zero_first, c->stream);
 }
 
 struct SumcheckState {
