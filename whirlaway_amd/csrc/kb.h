@@ -238,29 +238,88 @@ KB_HD void internal_layer(F *s, const uint32_t *diag) {
     for (int i = 0; i < W; i++) s[i] = fadd(sum, fmul(s[i], F{diag[i]}));
 }
 
+// ---- full rounds with a 64-bit external layer ----
+// The S-box output is left in [0, 1.78P) (no final correction) and the external
+// layer adds in 64 bits without reductions (rows of M4 weigh 7, the block sums
+// six blocks: outputs < 88 P < 2^38); the next S-box input folds the 64-bit value
+// once: v = hi 2^32 + lo == lo + hi (2^32 mod P), a 33-bit value whose top bit
+// folds again, then one conditional subtraction leaves it below 1.02 P, which the
+// lazy cube accepts (x^2 + m P and x2 x + m P stay below 2^64).
+
+// x in [0, 1.02P): x^3 in Montgomery form, in [0, 1.78P)
+KB_HD uint32_t fcube_lazy(uint32_t x) {
+    uint32_t x2 = redc_lazy((uint64_t)x * x);
+    return redc_lazy((uint64_t)x2 * x);
+}
+// v < 2^38 plus rc (< P), reduced to [0, 1.02P)
+KB_HD uint32_t fold38(uint64_t v) {
+    uint64_t t = (uint64_t)(uint32_t)(v >> 32) * C32 + (uint32_t)v;  // < 1.5 * 2^32
+    uint32_t t2 = (uint32_t)t + ((t >> 32) ? C32 : 0u);              // t >= 2^32 leaves (uint32_t)t < 2^31
+    return umin(t2, t2 - P);
+}
+// the external layer on y_i < 2^32 into 64-bit sums
+template <int W>
+KB_HD void mds_light64(const uint32_t *y, uint64_t *o) {
+#pragma unroll
+    for (int c = 0; c < W; c += 4) {
+        const uint64_t x0 = y[c], x1 = y[c + 1], x2 = y[c + 2], x3 = y[c + 3];
+        const uint64_t t01 = x0 + x1, t23 = x2 + x3, t0123 = t01 + t23;
+        const uint64_t t01123 = t0123 + x1, t01233 = t0123 + x3;
+        o[c] = t01123 + t01;
+        o[c + 1] = t01123 + (x2 << 1);
+        o[c + 2] = t01233 + t23;
+        o[c + 3] = t01233 + (x0 << 1);
+    }
+    uint64_t sums[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint64_t acc = o[k];
+#pragma unroll
+        for (int j = k + 4; j < W; j += 4) acc += o[j];
+        sums[k] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < W; i++) o[i] += sums[i & 3];
+}
+
 // Round loops are kept rolled so that each round loads only its own constants
 // (uniform scalar loads); unrolling them hoists every constant of the
 // permutation into scalar registers and spills.
 template <int W, int RP, typename C>
 KB_HD void p2_permute(F *s, const C &k, const uint32_t *diag) {
-    mds_light<W>(s);
+    (void)diag;
+    uint32_t y[W];
+    uint64_t o[W];
+    mds_light64<W>(&s[0].v, o);
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
 #pragma unroll
-        for (int i = 0; i < W; i++) s[i] = fcube(fadd(s[i], k.ext_init[r][i]));
-        mds_light<W>(s);
+        for (int i = 0; i < W; i++) y[i] = fcube_lazy(fold38(o[i] + k.ext_init[r][i].v));
+        mds_light64<W>(y, o);
     }
-    (void)diag;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const uint32_t t = fold38(o[i]);
+        s[i] = F{umin(t, t - P)};
+    }
 #pragma unroll 1
     for (int r = 0; r < RP; r++) {
         s[0] = fcube(fadd(s[0], k.internal[r]));
         internal_layer_fast<W>(s);
     }
-#pragma unroll 1
-    for (int r = 0; r < 4; r++) {
 #pragma unroll
-        for (int i = 0; i < W; i++) s[i] = fcube(fadd(s[i], k.ext_term[r][i]));
-        mds_light<W>(s);
+    for (int i = 0; i < W; i++) y[i] = fcube_lazy(fadd(s[i], k.ext_term[0][i]).v);
+    mds_light64<W>(y, o);
+#pragma unroll 1
+    for (int r = 1; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < W; i++) y[i] = fcube_lazy(fold38(o[i] + k.ext_term[r][i].v));
+        mds_light64<W>(y, o);
+    }
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const uint32_t t = fold38(o[i]);
+        s[i] = F{umin(t, t - P)};
     }
 }
 
