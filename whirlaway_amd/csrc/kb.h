@@ -251,11 +251,14 @@ KB_HD uint32_t fcube_lazy(uint32_t x) {
     uint32_t x2 = redc_lazy((uint64_t)x * x);
     return redc_lazy((uint64_t)x2 * x);
 }
-// v < 2^38 plus rc (< P), reduced to [0, 1.02P)
+// v < 2^38, reduced to [0, 1.02P) in 32-bit steps: hi (2^32 mod P) < 2^31 is
+// added to lo; a carry out of 2^32 is worth another 2^32 mod P (the wrapped sum
+// is then below 2^31, so adding it cannot carry again)
 KB_HD uint32_t fold38(uint64_t v) {
-    uint64_t t = (uint64_t)(uint32_t)(v >> 32) * C32 + (uint32_t)v;  // < 1.5 * 2^32
-    uint32_t t2 = (uint32_t)t + ((t >> 32) ? C32 : 0u);              // t >= 2^32 leaves (uint32_t)t < 2^31
-    return umin(t2, t2 - P);
+    const uint32_t lo = (uint32_t)v, m = (uint32_t)(v >> 32) * C32;
+    uint32_t t = lo + m;
+    t += t < m ? C32 : 0u;
+    return umin(t, t - P);
 }
 // the external layer on y_i < 2^32 into 64-bit sums
 template <int W>
