@@ -25,10 +25,11 @@ sys.path.insert(0, ROOT)
 # work model: Poseidon2-24 982, Poseidon2-16 616 F-mul per permutation; one EF
 # AIR evaluation point 31168 F-mul; one skip-round point 5700 F-mul.
 # VALU peak: 256 CUs x 64 lanes x 2.4 GHz full-rate instructions (the 32-bit integer
-# multiplies v_mad_u64_u32 / v_mul_lo_u32 / v_mul_hi_u32 issue at full rate on gfx950,
-# as the live Montgomery-product probe below shows) over the 6 instructions a
-# Montgomery product needs at least (3 multiplies, subtract, add of p, unsigned min).
-VALU_PEAK_TFMUL = 256 * 64 * 2.4e9 / 6 / 1e12
+# multiplies v_mad_u64_u32 / v_mul_lo_u32 issue at full rate on gfx950, as the live
+# Montgomery-product probe below shows) over the 5 instructions a reduced Montgomery
+# product needs at least (product, m = lo (-1/p), the multiply-add m p + x, then a
+# subtraction of p and an unsigned minimum).
+VALU_PEAK_TFMUL = 256 * 64 * 2.4e9 / 5 / 1e12
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of each kernel class at 2^17 permutations, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE passes (profiles/prove_n17_hbm_traffic.md), fetch scaled x2.0
