@@ -211,17 +211,24 @@ KB_HD void internal_apply_all(F *s, F sum) {
     }
 }
 // sum of W reduced values: pairs add without overflow (each < P < 2^31), the
-// W/2 pair sums accumulate in 64 bits, and one fold + two conditional
-// subtractions bring the total below P (the total is < W/2 * 2^32).
+// W/2 pair sums accumulate in a 32-bit word plus a carry count (an add with
+// carry-out and an add of the carry per pair), then hi 2^32 + lo folds as
+// lo + hi (2^32 mod P) with the carry of that addition folded once more, and two
+// conditional subtractions bring the total below P.
 template <int W>
 KB_HD F fsum_lazy(const F *s) {
-    uint64_t acc = 0;
+    uint32_t lo = 0, hi = 0;
 #pragma unroll
-    for (int i = 0; i < W; i += 2) acc += (uint32_t)(s[i].v + s[i + 1].v);
-    uint64_t t = (uint64_t)(uint32_t)(acc >> 32) * C32 + (uint32_t)acc;  // < 2^32 + W/2 * 2^25 < 2.1 P
-    t = t >= P ? t - P : t;
-    t = t >= P ? t - P : t;
-    return F{(uint32_t)t};
+    for (int i = 0; i < W; i += 2) {
+        const uint32_t pr = s[i].v + s[i + 1].v;
+        lo += pr;
+        hi += lo < pr;
+    }
+    const uint32_t m = hi * C32;  // hi < W/2: m < 2^29
+    uint32_t t = lo + m;
+    t += t < m ? C32 : 0u;        // a carry: the wrapped sum is below 2^29
+    t = umin(t, t - P);
+    return F{umin(t, t - P)};
 }
 // s = (J + diag(d)) s with shifts and adds only
 template <int W>

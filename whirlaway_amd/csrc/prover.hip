@@ -539,13 +539,6 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
                      EqJob{std::vector<EF>(zc.begin() + 2, zc.end()), eone(), eq1.p}});
         int zmax = 3 * (K - 1);
         int nzp = zmax - K + 1;  // z = K .. zmax
-        std::vector<F> sel;
-        for (int z = K; z <= zmax; z++) {
-            std::vector<F> s = wh::selectors_f(skips, fsmall(z));
-            sel.insert(sel.end(), s.begin(), s.end());
-        }
-        DBuf<F> dsel(c->pool, sel.size());
-        c->h2d_async(dsel.p, sel.data(), sel.size());
         size_t nbx = (M + 255) / 256;
         DBuf<EF> parts(c->pool, (size_t)nzp * nbx);
         {
@@ -555,6 +548,14 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
                 DBuf<F> U0(c->pool, (size_t)nzp * AIR_NCOLS * M);
                 wk::zc_round0_lde(packed, n, skips, nzp, t->air_dev, dap, eq, U0, parts, c->stream);
             } else {
+                // selector rows of the nzp points (only this generic pass reads them)
+                std::vector<F> sel;
+                for (int z = K; z <= zmax; z++) {
+                    std::vector<F> s = wh::selectors_f(skips, fsmall(z));
+                    sel.insert(sel.end(), s.begin(), s.end());
+                }
+                DBuf<F> dsel(c->pool, sel.size());
+                c->h2d_async(dsel.p, sel.data(), sel.size());
                 wk::zc_round0(packed, n, skips, dsel, nzp, t->air_dev, dap, eq, parts, c->stream);
             }
         }
