@@ -135,6 +135,35 @@ def test_proof_bytes_identical(gpu_ctx, oracle, n):
     assert b.verify(got) == 0
 
 
+# Parameter variants (other security levels, folding factors, rates, skip counts
+# and initial domain reductions); skips above 4 take the generic round-0 pass.
+SETTINGS_VARIANTS = [
+    (10, dict(sec=100, f0=5, f=3, rate=2, skips=2, rs0=3)),
+    (11, dict(sec=128, f0=6, f=4, rate=1, skips=3, rs0=4)),
+    (9, dict(sec=128, f0=7, f=4, rate=2, skips=1, rs0=1)),
+    (12, dict(sec=128, f0=7, f=4, rate=1, skips=5, rs0=5)),
+    (13, dict(sec=90, f0=4, f=2, rate=3, skips=6, rs0=2)),
+]
+
+
+@pytest.mark.parametrize("n,kw", SETTINGS_VARIANTS)
+def test_proof_settings_variants(gpu_ctx, oracle, n, kw):
+    import whirlaway_amd as wa
+    _say(f"full proof n={n} settings {kw}")
+    b = oracle.Bench(n, seed=3)
+    want = b.prove(**kw)
+    st = wa.AirSettings(security_bits=kw["sec"], folding_factor_first=kw["f0"], folding_factor=kw["f"],
+                        log_inv_rate=kw["rate"], univariate_skips=kw["skips"],
+                        rs_domain_initial_reduction=kw["rs0"])
+    table = wa.AirTable(gpu_ctx, b.air_consts(), n, univariate_skips=kw["skips"])
+    got = table.prove(st, b.p2_consts(16), b.p2_consts(24), b.trace())
+    table.close()
+    assert got.size == want.size
+    diff = np.nonzero(got != want)[0]
+    assert diff.size == 0, f"first mismatch at {diff[:5]}"
+    assert b.verify(got, **kw) == 0
+
+
 def test_proof_benchmark_size_verifies(gpu_ctx, oracle):
     """n = 17 (the benchmark workload): the oracle verifier accepts the GPU proof,
     and a resident-witness proof is byte-identical to the host-witness proof."""
