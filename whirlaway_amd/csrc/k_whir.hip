@@ -387,9 +387,10 @@ static void launch_b(const F *Y, int n, int log_m, int LA, const F *tabs, F *cw,
 }
 
 int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int L, F *cw, F *tmp,
-              const F *tw_full, const F *tabs, hipStream_t s) {
+              const F *tw_full, const F *tabs, hipStream_t s, bool zero_fill) {
     size_t rows = (size_t)1 << L;
-    if (n_nonzero < n_cols) fill_zero(cw + (size_t)n_nonzero * rows, (size_t)(n_cols - n_nonzero) * rows * 4, s);
+    if (zero_fill && n_nonzero < n_cols)
+        fill_zero(cw + (size_t)n_nonzero * rows, (size_t)(n_cols - n_nonzero) * rows * 4, s);
     int LA, LB;
     const bool two = ntt_split(L, &LA, &LB);
     if (n_nonzero <= 0) return two ? LA : 0;
@@ -433,7 +434,8 @@ int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int L, F *cw,
 // ---------------------------------------------------------------- Merkle ---
 // Thread = storage position q of the codeword columns (coalesced reads); the row it
 // holds is r = (q >> lb) | ((q mod 2^lb) << la) (natural order when la = 0).
-__global__ void __launch_bounds__(256) k_merkle_leaves(const F *cw, int width, size_t rows, int la,
+// columns from nnz on are zero and are not read
+__global__ void __launch_bounds__(256) k_merkle_leaves(const F *cw, int width, int nnz, size_t rows, int la,
                                                       int lb, const P2DevT *pc, F *nodes) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= rows) return;
@@ -446,7 +448,7 @@ __global__ void __launch_bounds__(256) k_merkle_leaves(const F *cw, int width, s
         // compile-time state indices (a runtime index would place the state in scratch)
 #pragma unroll
         for (int j = 0; j < 16; j++)
-            if (j < n) s[j] = cw[(size_t)(off + j) * rows + q];
+            if (j < n) s[j] = off + j < nnz ? cw[(size_t)(off + j) * rows + q] : F{0};
         p24_permute(s, pc->p24);
     }
     F *o = nodes + (rows + r) * 8;
@@ -555,12 +557,12 @@ __global__ void __launch_bounds__(256) k_merkle_sub(F *nodes, size_t cnt, int le
     }
 }
 
-void merkle_leaves(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s) {
+void merkle_leaves(const F *cw, int width, int nnz, int log_rows, int la, const void *pc, F *nodes, hipStream_t s) {
     size_t rows = (size_t)1 << log_rows;
-    k_merkle_leaves<<<nblk(rows), 256, 0, s>>>(cw, width, rows, la, log_rows - la, (const P2DevT *)pc, nodes);
+    k_merkle_leaves<<<nblk(rows), 256, 0, s>>>(cw, width, nnz, rows, la, log_rows - la, (const P2DevT *)pc, nodes);
 }
 void merkle_build(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s) {
-    merkle_leaves(cw, width, log_rows, la, pc, nodes, s);
+    merkle_leaves(cw, width, width, log_rows, la, pc, nodes, s);
     merkle_nodes(log_rows, pc, nodes, s);
 }
 void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s) {
@@ -583,14 +585,15 @@ void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s) {
     }
 }
 
-__global__ void k_merkle_open(const F *cw, const F *nodes, int width, int log_rows, int la,
+__global__ void k_merkle_open(const F *cw, const F *nodes, int width, int nnz, int log_rows, int la,
                               const uint32_t *idx, F *rows_out, F *paths_out) {
     int q = blockIdx.x;
     size_t rows = (size_t)1 << log_rows;
     uint32_t r = idx[q];
     const int lb = log_rows - la;
     const size_t at = ((size_t)(r & ((1u << la) - 1)) << lb) | (r >> la);  // storage position of row r
-    for (int c = threadIdx.x; c < width; c += blockDim.x) rows_out[(size_t)q * width + c] = cw[(size_t)c * rows + at];
+    for (int c = threadIdx.x; c < width; c += blockDim.x)
+        rows_out[(size_t)q * width + c] = c < nnz ? cw[(size_t)c * rows + at] : F{0};
     for (int t = threadIdx.x; t < log_rows * 8; t += blockDim.x) {
         int l = t / 8, j = t % 8;
         size_t node = ((rows + r) >> l) ^ 1;
@@ -620,9 +623,9 @@ void fold_rows(const F *rows, int nq, int fold, bool base, const EF *eqr, EF *ou
     if (nq > 0) k_fold_rows<<<nq, 128, 0, s>>>(rows, fold, base, eqr, out);
 }
 
-void merkle_open(const F *cw, const F *nodes, int width, int log_rows, int la, const uint32_t *idx, int nq,
-                 F *rows_out, F *paths_out, hipStream_t s) {
-    if (nq > 0) k_merkle_open<<<nq, 128, 0, s>>>(cw, nodes, width, log_rows, la, idx, rows_out, paths_out);
+void merkle_open(const F *cw, const F *nodes, int width, int nnz, int log_rows, int la, const uint32_t *idx,
+                 int nq, F *rows_out, F *paths_out, hipStream_t s) {
+    if (nq > 0) k_merkle_open<<<nq, 128, 0, s>>>(cw, nodes, width, nnz, log_rows, la, idx, rows_out, paths_out);
 }
 
 // ----------------------------------------------------------------- grinding --

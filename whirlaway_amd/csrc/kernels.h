@@ -152,8 +152,10 @@ void ef_to_cols(const EF *in, int m, int k, F *out, hipStream_t s);
 // w^r).  Returns la: row r is stored at position ((r mod 2^la) << (log_rows - la))
 // | (r >> la) of every column (la = 0: natural order).  tmp: n_nonzero x
 // 2^log_rows words; tabs: ntt_tables(log_rows) (used when ntt_split holds).
+// Columns from n_nonzero on are zero: written as zeros when zero_fill is set, else
+// left unwritten (the Merkle leaf and opening kernels then take them as zero).
 int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int log_rows, F *cw,
-              F *tmp, const F *tw_full, const F *tabs, hipStream_t s);
+              F *tmp, const F *tw_full, const F *tabs, hipStream_t s, bool zero_fill = true);
 // lengths up to 2^12 are encoded in one block per column, 2^14 .. 2^20 by the
 // two-pass register kernels with L = LA + LB, LB = 10, others by LDS four-step kernels
 constexpr int NTT2_MIN_L = 14, NTT2_MAX_L = 20;
@@ -165,12 +167,14 @@ void twiddles(F *out, int L, hipStream_t s);
 // Merkle tree over rows of a column-major codeword (width F per leaf), heap layout nodes[2*rows][8]
 // (la: codeword storage order as returned by rs_encode)
 void merkle_build(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s);
-// the two halves of merkle_build: leaf hashes, then the compression levels
-void merkle_leaves(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s);
+// the two halves of merkle_build: leaf hashes (columns from nnz on taken as zero,
+// not read), then the compression levels
+void merkle_leaves(const F *cw, int width, int nnz, int log_rows, int la, const void *pc, F *nodes, hipStream_t s);
 void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s);
-// gather rows (width F each) and sibling paths for sorted query indices
-void merkle_open(const F *cw, const F *nodes, int width, int log_rows, int la, const uint32_t *idx, int nq,
-                 F *rows_out, F *paths_out, hipStream_t s);
+// gather rows (width F each; columns from nnz on are zero) and sibling paths for
+// sorted query indices
+void merkle_open(const F *cw, const F *nodes, int width, int nnz, int log_rows, int la, const uint32_t *idx,
+                 int nq, F *rows_out, F *paths_out, hipStream_t s);
 // folded query values: out[q] = sum_b eqr[b] * row_q[b], rows of 2^fold F (base)
 // or 2^fold EF (8 F each, coefficient-wise) values
 void fold_rows(const F *rows, int nq, int fold, bool base, const EF *eqr, EF *out, hipStream_t s);

@@ -34,6 +34,7 @@ struct Codeword {
     DBuf<F> cw;      // column-major [width][rows]
     DBuf<F> nodes;   // heap [2 rows][8]
     int width = 0;   // F per leaf
+    int nnz = 0;     // columns [nnz, width) are zero and not stored
     int log_rows = 0;
     int la = 0;      // storage order of the rows (rs_encode)
     bool base = true;
@@ -43,9 +44,10 @@ struct Codeword {
 void commit_codeword(wl_ctx *c, const P2Dev *pdev, Codeword &cwd, const F *cols, int n_cols,
                      int n_nonzero, int log_m, int log_rows) {
     size_t rows = (size_t)1 << log_rows;
-    cwd.cw = DBuf<F>(c->pool, (size_t)n_cols * rows);
+    cwd.cw = DBuf<F>(c->pool, (size_t)std::max(1, n_nonzero) * rows);  // zero columns are not stored
     cwd.nodes = DBuf<F>(c->pool, 2 * rows * 8);
     cwd.width = n_cols;
+    cwd.nnz = n_nonzero;
     cwd.log_rows = log_rows;
     DBuf<F> tmp(c->pool, (size_t)std::max(1, n_nonzero) * rows);
     F *tw = c->twiddle(log_rows);
@@ -53,13 +55,13 @@ void commit_codeword(wl_ctx *c, const P2Dev *pdev, Codeword &cwd, const F *cols,
         // algorithmic HBM bytes: read the evaluations, write the codeword columns
         KTimer kt(c, KT_NTT, 4.0 * (double)n_nonzero * (double)(rows + ((size_t)1 << log_m)));
         cwd.la = wk::rs_encode(cols, n_cols, n_nonzero, log_m, log_rows, cwd.cw, tmp, tw, c->ntt_tabs(log_rows),
-                               c->stream);
+                               c->stream, false);
         HIP_CHECK(hipGetLastError());
     }
     {
         // one Poseidon2-24 permutation per 16 leaf elements (PaddingFreeSponge rate 16)
         KTimer kt(c, KT_MERKLE_LEAVES, (double)rows * ((n_cols + 15) / 16));
-        wk::merkle_leaves(cwd.cw, n_cols, log_rows, cwd.la, pdev, cwd.nodes, c->stream);
+        wk::merkle_leaves(cwd.cw, n_cols, n_nonzero, log_rows, cwd.la, pdev, cwd.nodes, c->stream);
     }
     {
         KTimer kt(c, KT_MERKLE_NODES, (double)(rows - 1));
@@ -91,7 +93,7 @@ std::vector<EF> open_queries(wl_ctx *c, Transcript &ts, const Codeword &cwd, con
     Uploads up(c);
     const size_t o_idx = up.add(q), o_eq = eqr ? up.add(*eqr) : 0;
     up.send();
-    wk::merkle_open(cwd.cw, cwd.nodes, cwd.width, cwd.log_rows, cwd.la, up.at<uint32_t>(o_idx), nq, buf.p,
+    wk::merkle_open(cwd.cw, cwd.nodes, cwd.width, cwd.nnz, cwd.log_rows, cwd.la, up.at<uint32_t>(o_idx), nq, buf.p,
                     buf.p + nrow, c->stream);
     if (eqr)
         wk::fold_rows(buf.p, nq, cwd.fold, cwd.base, up.at<EF>(o_eq), (EF *)(buf.p + nrow + npath), c->stream);
