@@ -202,7 +202,7 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=64, help="batched leg: proofs (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=256, help="serving leg: requests (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=120.0, help="serving leg: Poisson arrivals per second")
-    ap.add_argument("--slots", type=int, default=4, help="engine slots (concurrent proofs)")
+    ap.add_argument("--slots", type=int, default=8, help="engine slots (concurrent proofs)")
     ap.add_argument("--no-micro", action="store_true", help="skip the kernel micro leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this size (HBM counter calibration)")
