@@ -235,7 +235,16 @@ def main():
     for _ in range(args.warmup):
         proof = table.prove(settings, p16, p24, wit)
 
+    # Untimed pass with HIP events around every kernel class: per-class device time
+    # and the dominant class.  Events are stream packets on the proof's critical path,
+    # so the timed steps below record them around the dominant class only.
     ctx.kernel_timing(list(MODELS))
+    proof = table.prove(settings, p16, p24, wit)
+    per_class = {k: ctx.kernel_timing_read(k) for k in MODELS}
+    ctx.kernel_timing([])
+    dom = max(per_class, key=lambda k: per_class[k][1])
+
+    ctx.kernel_timing([dom])
     ctx.phase_times(reset=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -243,10 +252,7 @@ def main():
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     value = (1 << n) / (ms / 1e3)
-    per_class = {}
-    for k, (name, bound, per_unit) in MODELS.items():
-        launches, kms, work = ctx.kernel_timing_read(k)
-        per_class[k] = (launches, kms, work)
+    dom_live = ctx.kernel_timing_read(dom)
     ctx.kernel_timing([])
     phases = ctx.phase_times(reset=True)
 
@@ -263,9 +269,8 @@ def main():
                 "steps": args.pcie_steps, "same_proof": bool((proof_h == proof).all())}
         del host_w
 
-    dom = max(per_class, key=lambda k: per_class[k][1])
     name, bound, per_unit = MODELS[dom]
-    launches, kms, work = per_class[dom]
+    launches, kms, work = dom_live
     avg_ms = kms / max(launches, 1)
     per_launch = work * per_unit / max(launches, 1)
     if bound == "hbm":
@@ -283,8 +288,8 @@ def main():
         "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": name,
         "fmul_probe_TFmul": round(ctx.fmul_rate() / 1e12, 4),
         "avg_launch_ms": round(avg_ms, 5), "launches_per_step": launches / args.steps,
-        "kernel_ms_per_step": {MODELS[k][0].split(" ")[0]: round(v[1] / args.steps, 4)
-                               for k, v in per_class.items()},
+        # from the untimed all-class pass (one proof)
+        "kernel_ms_per_step": {MODELS[k][0].split(" ")[0]: round(v[1], 4) for k, v in per_class.items()},
     }
 
     cpu = None
