@@ -88,20 +88,135 @@ inline std::vector<F> selectors_f(int k, F x) {
     }
     return out;
 }
-inline std::vector<EF> selectors_ef(int k, const EF &x) {
-    int N = 1 << k;
-    std::vector<EF> out(N);
-    for (int i = 0; i < N; i++) {
-        EF num = eone();
-        F den = fone();
-        for (int j = 0; j < N; j++) {
-            if (j == i) continue;
-            num = emul(num, esub(x, esmall(j)));
-            den = fmul(den, fsub(fsmall(i), fsmall(j)));
-        }
-        out[i] = emulf(num, finv(den));
+// ------------------------------------------ interpolation on 0, 1, .., n-1 --
+// Coefficient rows of the Lagrange basis on the integer nodes 0..n-1 (n <= 64):
+// row j is L_j(X) = prod_{i != j} (X - i) / (j - i), low degree first, from the
+// node polynomial M(X) = prod_i (X - i) divided by (X - j).  Built once per n and
+// thread (the batch engine proves on several threads).
+inline const std::vector<F> &int_basis(int n) {
+    static thread_local std::vector<std::vector<F>> cache(65);
+    if (n < 1 || n > 64) throw std::runtime_error("interpolation on more than 64 nodes");
+    std::vector<F> &B = cache[n];
+    if (!B.empty()) return B;
+    std::vector<F> m(n + 1, fzero());
+    m[0] = fone();
+    for (int i = 0; i < n; i++) {
+        const F xi = fsmall(i);
+        for (int d = i + 1; d > 0; d--) m[d] = fsub(m[d - 1], fmul(xi, m[d]));
+        m[0] = fneg(fmul(xi, m[0]));
     }
+    B.assign((size_t)n * n, fzero());
+    for (int j = 0; j < n; j++) {
+        F *row = &B[(size_t)j * n];
+        const F xj = fsmall(j);
+        F carry = m[n], den = fone();
+        for (int d = n - 1; d >= 0; d--) { row[d] = carry; carry = fadd(m[d], fmul(carry, xj)); }
+        for (int i = 0; i < n; i++)
+            if (i != j) den = fmul(den, fsub(xj, fsmall(i)));
+        const F inv = finv(den);
+        for (int d = 0; d < n; d++) row[d] = fmul(row[d], inv);
+    }
+    return B;
+}
+// sum_i w[i stride] v[i]: EF x F products summed per coefficient in 64 bits (folded
+// after every four, as in emul), one reduction per coefficient
+inline EF lin_ef(const F *w, size_t stride, const EF *v, int n) {
+    uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        const uint64_t wi = w[(size_t)i * stride].v;
+        for (int k = 0; k < 8; k++) acc[k] += wi * v[i].c[k].v;
+        if ((i & 3) == 3)
+            for (int k = 0; k < 8; k++) acc[k] = fold64(acc[k]);
+    }
+    EF r;
+    for (int k = 0; k < 8; k++) r.c[k] = F{redc64(acc[k])};
+    return r;
+}
+// coefficients of the polynomial of degree < n through (i, ys[i]), i = 0..n-1
+inline std::vector<EF> interp_int(const std::vector<EF> &ys) {
+    const int n = (int)ys.size();
+    const std::vector<F> &B = int_basis(n);
+    std::vector<EF> co(n);
+    for (int d = 0; d < n; d++) co[d] = lin_ef(&B[d], (size_t)n, ys.data(), n);
+    return co;
+}
+// value at the integer z of basis polynomial j on the nodes 0..n-1
+inline F int_basis_at(int n, int j, int z) {
+    const F *row = &int_basis(n)[(size_t)j * n];
+    const F x = fsmall((uint32_t)z);
+    F acc = fzero();
+    for (int d = n - 1; d >= 0; d--) acc = fadd(fmul(acc, x), row[d]);
+    return acc;
+}
+
+// univariate_selectors(k) at an extension point: L_i(x) on the nodes 0..2^k-1 as
+// (prefix product) (suffix product) / prod_{j != i} (i - j), the denominators'
+// inverses cached per k
+inline std::vector<EF> selectors_ef(int k, const EF &x) {
+    const int N = 1 << k;
+    static thread_local std::vector<std::vector<F>> dinv_cache(32);
+    std::vector<F> &dinv = dinv_cache[k];
+    if (dinv.empty()) {
+        dinv.resize(N);
+        for (int i = 0; i < N; i++) {
+            F den = fone();
+            for (int j = 0; j < N; j++)
+                if (j != i) den = fmul(den, fsub(fsmall(i), fsmall(j)));
+            dinv[i] = finv(den);
+        }
+    }
+    std::vector<EF> d(N), pre(N + 1), suf(N + 1), out(N);
+    for (int j = 0; j < N; j++) d[j] = esub(x, esmall(j));
+    pre[0] = eone();
+    for (int j = 0; j < N; j++) pre[j + 1] = emul(pre[j], d[j]);
+    suf[N] = eone();
+    for (int j = N - 1; j >= 0; j--) suf[j] = emul(suf[j + 1], d[j]);
+    for (int i = 0; i < N; i++) out[i] = emulf(emul(pre[i], suf[i + 1]), dinv[i]);
     return out;
+}
+
+// Round 0 of the zerocheck with k univariate skips, K = 2^k: the round polynomial
+// is f = p q, where p (degree 3(K-1)) is 0 on 0..K-1 and vals on K..3(K-1), and q
+// (degree K-1) takes sel[i] at i = 0..K-1.  f is formed from its values on
+// 0..4(K-1) (zero below K; p extrapolated past 3(K-1), q evaluated past K-1 through
+// cached basis values) and interpolated there: the same coefficients as
+// interpolating p and q and multiplying them.
+struct SkipTables {
+    std::vector<F> q_at;   // [z - K][i]: L_i on 0..K-1 at z = K..4(K-1)
+    std::vector<F> p_at;   // [z - D - 1][j - K]: L_j on 0..D at z = D+1..4(K-1), j = K..D
+    std::vector<F> coef;   // [d][z - K]: coefficient d of L_z on 0..4(K-1), z = K..4(K-1)
+};
+inline const SkipTables &skip_tables(int k) {
+    static thread_local std::vector<SkipTables> cache(8);
+    if (k < 1 || k > 4) throw std::runtime_error("univariate skips beyond 4 use the generic path");
+    SkipTables &t = cache[k];
+    if (!t.coef.empty()) return t;
+    const int K = 1 << k, D = 3 * (K - 1), Df = 4 * (K - 1), NZ = Df - K + 1;
+    t.q_at.resize((size_t)NZ * K);
+    for (int z = K; z <= Df; z++)
+        for (int i = 0; i < K; i++) t.q_at[(size_t)(z - K) * K + i] = int_basis_at(K, i, z);
+    t.p_at.resize((size_t)(Df - D) * (D - K + 1));
+    for (int z = D + 1; z <= Df; z++)
+        for (int j = K; j <= D; j++) t.p_at[(size_t)(z - D - 1) * (D - K + 1) + (j - K)] = int_basis_at(D + 1, j, z);
+    const std::vector<F> &B = int_basis(Df + 1);
+    t.coef.resize((size_t)(Df + 1) * NZ);
+    for (int d = 0; d <= Df; d++)
+        for (int z = K; z <= Df; z++) t.coef[(size_t)d * NZ + (z - K)] = B[(size_t)z * (Df + 1) + d];
+    return t;
+}
+inline std::vector<EF> skip_round_poly(int k, const std::vector<EF> &vals, const std::vector<EF> &sel) {
+    const SkipTables &t = skip_tables(k);
+    const int K = 1 << k, D = 3 * (K - 1), Df = 4 * (K - 1), NZ = Df - K + 1;
+    if ((int)vals.size() != D - K + 1 || (int)sel.size() != K) throw std::runtime_error("skip round sizes");
+    std::vector<EF> fz(NZ);
+    for (int z = K; z <= Df; z++) {
+        const EF q = lin_ef(&t.q_at[(size_t)(z - K) * K], 1, sel.data(), K);
+        const EF p = z <= D ? vals[z - K] : lin_ef(&t.p_at[(size_t)(z - D - 1) * (D - K + 1)], 1, vals.data(), D - K + 1);
+        fz[z - K] = emul(p, q);
+    }
+    std::vector<EF> co(Df + 1);
+    for (int d = 0; d <= Df; d++) co[d] = lin_ef(&t.coef[(size_t)d * NZ], 1, fz.data(), NZ);
+    return co;
 }
 // eq(point, .) big-endian (host, small tables only)
 inline std::vector<EF> eq_host(const std::vector<EF> &p) {

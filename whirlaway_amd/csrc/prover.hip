@@ -371,29 +371,38 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
     // initial f0 rounds: T_j(u) = sum_x f(u, x) eq(pt_j[f0..], x) on the device,
     // then the f0 round polynomials from the 2^f0-entry tables.
     // the OOD constraints come first in cs; their T tables were built at commit time
+    // Constraint j adds scal_j sum_u eq(p_j, u) T_j(u) over the 2^f0 entries u.  With
+    // E_j = eq(p_j[t+1..f0], .) and q = p_j[t], round t's values are a0 = (1 - q) d0,
+    // a1 = q d1, a2 = (3q - 1)(2 d1 - d0) for d0, d1 = sum_i E_j(i) T_j(i), T_j(i + h);
+    // folding by r multiplies the constraint by eq(q, r), halves T_j, and the next
+    // E_j is the sum of E_j's halves.
     const int X = v - f0, nC = (int)cs.size(), nood = (int)w.ood_T.size();
-    std::vector<std::vector<EF>> A(nC), sufs;
+    std::vector<std::vector<EF>> Q(nC), E(nC), sufs;
+    std::vector<EF> fac(nC);
     for (int j = 0; j < nC; j++) {
         std::vector<EF> pt = cons_point(cs[j], v);
-        A[j] = wh::eq_host(std::vector<EF>(pt.begin(), pt.begin() + f0));
+        Q[j].assign(pt.begin(), pt.begin() + f0);
+        E[j] = wh::eq_host(std::vector<EF>(pt.begin() + 1, pt.begin() + f0));
+        fac[j] = cs[j].scal;
         if (j >= nood) sufs.emplace_back(pt.begin() + f0, pt.end());
     }
     std::vector<std::vector<EF>> Tj = w.ood_T;
     for (auto &t : whir_T_tables(c, w.evals, v, f0, w.nz_len, sufs)) Tj.push_back(std::move(t));
     std::vector<EF> rf;
+    const EF one = eone(), three = wh::esmall(3);
     for (int t = 0; t < f0; t++) {
         size_t h = Tj[0].size() / 2;
         EF h0 = ezero(), h1 = ezero(), h2 = ezero();
         for (int j = 0; j < nC; j++) {
-            EF a0 = ezero(), a1 = ezero(), a2 = ezero();
+            EF d0 = ezero(), d1 = ezero();
             for (size_t i = 0; i < h; i++) {
-                a0 = eadd(a0, emul(A[j][i], Tj[j][i]));
-                a1 = eadd(a1, emul(A[j][i + h], Tj[j][i + h]));
-                a2 = eadd(a2, emul(esub(edbl(A[j][i + h]), A[j][i]), esub(edbl(Tj[j][i + h]), Tj[j][i])));
+                d0 = eadd(d0, emul(E[j][i], Tj[j][i]));
+                d1 = eadd(d1, emul(E[j][i], Tj[j][i + h]));
             }
-            h0 = eadd(h0, emul(cs[j].scal, a0));
-            h1 = eadd(h1, emul(cs[j].scal, a1));
-            h2 = eadd(h2, emul(cs[j].scal, a2));
+            const EF q = Q[j][t];
+            h0 = eadd(h0, emul(fac[j], emul(esub(one, q), d0)));
+            h1 = eadd(h1, emul(fac[j], emul(q, d1)));
+            h2 = eadd(h2, emul(fac[j], emul(esub(emul(three, q), one), esub(edbl(d1), d0))));
         }
         EF hv[3] = {h0, h1, h2};
         ts.add_ef(hv, 3);
@@ -402,12 +411,13 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
         rf.push_back(r);
         sum = wh::interp3(hv, r);
         for (int j = 0; j < nC; j++) {
-            for (size_t i = 0; i < h; i++) {
-                A[j][i] = eadd(A[j][i], emul(r, esub(A[j][i + h], A[j][i])));
-                Tj[j][i] = eadd(Tj[j][i], emul(r, esub(Tj[j][i + h], Tj[j][i])));
-            }
-            A[j].resize(h);
+            const EF q = Q[j][t];
+            fac[j] = emul(fac[j], eadd(emul(esub(one, q), esub(one, r)), emul(q, r)));
+            for (size_t i = 0; i < h; i++) Tj[j][i] = eadd(Tj[j][i], emul(r, esub(Tj[j][i + h], Tj[j][i])));
             Tj[j].resize(h);
+            const size_t hh = h / 2;
+            for (size_t i = 0; i < hh; i++) E[j][i] = eadd(E[j][i], E[j][i + hh]);
+            E[j].resize(hh ? hh : 1);
         }
     }
     for (auto &k : cs) fold_constraint(k, v, rf);
@@ -531,6 +541,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     const int K = 1 << skips;
     const size_t M = N >> skips;
     std::vector<EF> ch;
+    std::vector<EF> zc_inv(nz), sc0;  // 1 / zc[j] for the later rounds; the selectors at round 0's challenge
     EF sum = ezero(), missing = eone();
     DBuf<EF> U(c->pool, (size_t)AIR_NCOLS * M);
     // eq(zc[1..], .) for round 0 and eq(zc[2..], .) for round 1 in one launch
@@ -561,22 +572,30 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
                 wk::zc_round0(packed, n, skips, dsel, nzp, t->air_dev, dap, eq, parts, c->stream);
             }
         }
-        std::vector<EF> vals = reduce_many(c, parts, nzp, (int)nbx);
-        std::vector<F> xs;
-        std::vector<EF> ys;
-        for (int i = 0; i < K; i++) { xs.push_back(fsmall(i)); ys.push_back(ezero()); }
-        for (int i = 0; i < nzp; i++) { xs.push_back(fsmall(K + i)); ys.push_back(vals[i]); }
-        std::vector<EF> p = wh::lagrange(xs, ys);
+        // host values the round-0 reply needs, computed while the device works:
+        // the selectors at zc[0] and the inverses of the later round points
         std::vector<EF> sr = wh::selectors_ef(skips, zc[0]);
-        std::vector<F> qx;
-        for (int i = 0; i < K; i++) qx.push_back(fsmall(i));
-        p = wh::pmul(p, wh::lagrange(qx, sr));
+        if (skips <= 4) wh::skip_tables(skips);
+        for (int j = 1; j < nz; j++) zc_inv[j] = wh::einv(zc[j]);
+        std::vector<EF> vals = reduce_many(c, parts, nzp, (int)nbx);
+        std::vector<EF> p;
+        if (skips <= 4) {
+            p = wh::skip_round_poly(skips, vals, sr);
+        } else {
+            std::vector<F> xs;
+            std::vector<EF> ys;
+            for (int i = 0; i < K; i++) { xs.push_back(fsmall(i)); ys.push_back(ezero()); }
+            for (int i = 0; i < nzp; i++) { xs.push_back(fsmall(K + i)); ys.push_back(vals[i]); }
+            std::vector<F> qx;
+            for (int i = 0; i < K; i++) qx.push_back(fsmall(i));
+            p = wh::pmul(wh::lagrange(xs, ys), wh::lagrange(qx, sr));
+        }
         ts.add_ef(p.data(), p.size());
         EF c0 = ts.sample_ef();
         ch.push_back(c0);
         sum = wh::peval(p, c0);
         grind(c, ts, pdev, pow_bits_sumcheck(sec, (3 + 1) * (K - 1)));
-        std::vector<EF> sc0 = wh::selectors_ef(skips, c0);
+        sc0 = wh::selectors_ef(skips, c0);
         EF mm = ezero();
         for (int i = 0; i < K; i++) mm = eadd(mm, emul(sr[i], sc0[i]));
         missing = mm;
@@ -615,9 +634,8 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
         else wk::fold_cols_p(U, AIR_NCOLS, M, l, (const EF *)slot_cur.dev, eq, c->stream);
         std::vector<EF> v3 = collect(c, seq_cur, 3);
         EF e0 = emul(v3[0], missing), e2 = emul(v3[1], missing), e3 = emul(v3[2], missing);
-        EF e1 = emul(esub(sum, emul(e0, esub(eone(), r))), wh::einv(r));
-        std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2), fsmall(3)};
-        std::vector<EF> p = wh::lagrange(xs, {e0, e1, e2, e3});
+        EF e1 = emul(esub(sum, emul(e0, esub(eone(), r))), zc_inv[j]);
+        std::vector<EF> p = wh::interp_int({e0, e1, e2, e3});
         p = wh::pmul(p, {esub(eone(), r), esub(edbl(r), eone())});
         ts.add_ef(p.data(), p.size());
         EF cj = ts.sample_ef();
@@ -648,7 +666,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     {
         DBuf<EF> parts(c->pool, (size_t)AIR_NCOLS * K);
         wk::zc_down_parts(packed, n, skips, eqr, parts, c->stream);
-        DBuf<EF> dS = upload_ef(c, wh::selectors_ef(skips, ch[0]));
+        DBuf<EF> dS = upload_ef(c, sc0);
         DBuf<EF> out(c->pool, AIR_NCOLS);
         wk::zc_down_combine(parts, skips, dS, out, c->stream);
         s_dn = ++c->seq;
@@ -722,8 +740,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     const int gpu_rounds = n > HOST_TAIL ? n - HOST_TAIL : 0;
     auto inner_round = [&](const EF &h0, const EF &h2) {
         read_isum();
-        std::vector<F> xs = {fsmall(0), fsmall(1), fsmall(2)};
-        std::vector<EF> p = wh::lagrange(xs, {h0, esub(isum, h0), h2});
+        std::vector<EF> p = wh::interp_int({h0, esub(isum, h0), h2});
         ts.add_ef(p.data(), p.size());
         EF cj = ts.sample_ef();
         ich.push_back(cj);
