@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <chrono>
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -246,6 +247,23 @@ struct wl_ctx {
     }
     // copy `bytes` of device memory to dst through the mapped buffer (waits); the
     // first n_canon words are F values delivered in canonical form
+    // two device ranges in one round trip (two publishes, one wait)
+    void fetch2(void *dst0, const void *src0, size_t bytes0, void *dst1, const void *src1, size_t bytes1) {
+        ensure_mapped();
+        const size_t w0 = (bytes0 + 3) / 4, w1 = (bytes1 + 3) / 4;
+        if (!bytes0 || !bytes1 || w0 + w1 > mapped_words) {
+            fetch(dst0, src0, bytes0);
+            fetch(dst1, src1, bytes1);
+            return;
+        }
+        const uint32_t s0 = ++seq;
+        wk::publish((const uint32_t *)src0, w0, mapped_dev, flag_dev(), s0, publish_counter(), stream);
+        const uint32_t s1 = ++seq;
+        wk::publish((const uint32_t *)src1, w1, mapped_dev + w0, flag_dev(), s1, publish_counter(), stream);
+        wait_flag(s1);
+        memcpy(dst0, mapped, bytes0);
+        memcpy(dst1, mapped + w0, bytes1);
+    }
     void fetch(void *dst, const void *src, size_t bytes, size_t n_canon = 0) {
         if (!bytes) return;
         ensure_mapped();
@@ -342,19 +360,31 @@ struct KTimer {
 };
 
 // Sum block partials [item][part] and bring the item sums to the host.
-inline std::vector<EF> reduce_many(wl_ctx *c, const EF *parts, int n_items, int n_parts) {
+// `meanwhile` (optional) is host work run after the launches, before the wait.
+inline std::vector<EF> reduce_many(wl_ctx *c, const EF *parts, int n_items, int n_parts,
+                                   const std::function<void()> &meanwhile = nullptr) {
     std::vector<EF> r(n_items);
     c->ensure_mapped();
-    if (n_items <= 16) {  // one block, a wave per item
-        uint32_t s = ++c->seq;
-        wk::reduce_publish(parts, n_items, n_parts, (EF *)c->mapped_dev, c->flag_dev(), s, c->stream);
-        c->wait_flag(s);
-        memcpy(r.data(), c->mapped, (size_t)n_items * sizeof(EF));
+    const size_t words = (size_t)n_items * 8;
+    DBuf<EF> out;
+    if (words > c->mapped_words) {
+        out = DBuf<EF>(c->pool, n_items);
+        wk::reduce_parts(parts, n_items, n_parts, out, c->stream);
+        if (meanwhile) meanwhile();
+        c->fetch(r.data(), out.p, words * 4);
         return r;
     }
-    DBuf<EF> out(c->pool, n_items);
-    wk::reduce_parts(parts, n_items, n_parts, out, c->stream);
-    c->fetch(r.data(), out.p, (size_t)n_items * sizeof(EF));
+    const uint32_t s = ++c->seq;
+    if (n_items <= 16) {  // one block, a wave per item
+        wk::reduce_publish(parts, n_items, n_parts, (EF *)c->mapped_dev, c->flag_dev(), s, c->stream);
+    } else {
+        out = DBuf<EF>(c->pool, n_items);
+        wk::reduce_parts(parts, n_items, n_parts, out, c->stream);
+        wk::publish((const uint32_t *)out.p, words, c->mapped_dev, c->flag_dev(), s, c->publish_counter(), c->stream);
+    }
+    if (meanwhile) meanwhile();
+    c->wait_flag(s);
+    memcpy(r.data(), c->mapped, words * 4);
     return r;
 }
 inline EF reduce_one(wl_ctx *c, const EF *parts, int n_parts) { return reduce_many(c, parts, 1, n_parts)[0]; }

@@ -290,7 +290,8 @@ struct WhirWitness {
 // for several points in one pass (one eq launch, one pass over f, one reduction).
 // f(pre_j, suf_j) is then sum_u eq(pre_j, u) T_j(u).
 std::vector<std::vector<EF>> whir_T_tables(wl_ctx *c, const F *f, int v, int f0, size_t nz_len,
-                                           const std::vector<std::vector<EF>> &sufs) {
+                                           const std::vector<std::vector<EF>> &sufs,
+                                           const std::function<void()> &meanwhile = nullptr) {
     const int X = v - f0, U = 1 << f0, nC = (int)sufs.size();
     std::vector<std::vector<EF>> out(nC);
     if (nC == 0) return out;
@@ -301,7 +302,7 @@ std::vector<std::vector<EF>> whir_T_tables(wl_ctx *c, const F *f, int v, int f0,
     const int nch = wk::whir_T_chunks(v, f0);
     DBuf<EF> parts(c->pool, (size_t)nC * U * nch);
     wk::whir_T_f(f, v, f0, E, nC, parts, nz_len, c->stream);
-    std::vector<EF> T = reduce_many(c, parts, nC * U, nch);
+    std::vector<EF> T = reduce_many(c, parts, nC * U, nch, meanwhile);
     for (int j = 0; j < nC; j++) out[j].assign(T.begin() + (size_t)j * U, T.begin() + (size_t)(j + 1) * U);
     return out;
 }
@@ -317,8 +318,10 @@ EF eval_ef_univ(wl_ctx *c, const EF *f, int m, const EF &z) {
 
 // CommitmentWriter::commit: RS-encode the 2^f0 columns of the packed
 // polynomial, Merkle-commit, absorb the root, answer OOD samples.
+// `absorb_first` (optional): elements the transcript observes before the root,
+// absorbed while the device encodes and hashes.
 void whir_commit(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts, const F *evals,
-                 size_t nonzero_len, WhirWitness &w) {
+                 size_t nonzero_len, WhirWitness &w, const std::vector<uint32_t> *absorb_first = nullptr) {
     int v = cfg.num_variables, f0 = cfg.f0;
     int log_m = v - f0;
     int n_cols = 1 << f0;
@@ -327,6 +330,8 @@ void whir_commit(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &t
     w.evals = evals;
     w.nz_len = nonzero_len;
     commit_codeword(c, pdev, w.cw0, evals, n_cols, n_nonzero, log_m, v + cfg.starting_log_inv_rate - f0);
+    if (absorb_first)
+        for (uint32_t x : *absorb_first) ts.observe(to_mont(x));
     w.cw0.base = true;
     w.cw0.fold = f0;
     F root[8];
@@ -341,9 +346,12 @@ void whir_commit(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &t
         pres.emplace_back(pt.begin(), pt.begin() + cfg.f0);
         sufs.emplace_back(pt.begin() + cfg.f0, pt.end());
     }
-    w.ood_T = whir_T_tables(c, evals, v, cfg.f0, w.nz_len, sufs);
+    std::vector<std::vector<EF>> eqp(pres.size());
+    w.ood_T = whir_T_tables(c, evals, v, cfg.f0, w.nz_len, sufs, [&] {
+        for (size_t i = 0; i < pres.size(); i++) eqp[i] = wh::eq_host(pres[i]);
+    });
     for (size_t i = 0; i < w.ood_points.size(); i++) {
-        std::vector<EF> a = wh::eq_host(pres[i]);
+        const std::vector<EF> &a = eqp[i];
         EF ans = ezero();
         for (size_t u = 0; u < a.size(); u++) ans = eadd(ans, emul(a[u], w.ood_T[i][u]));
         w.ood_answers.push_back(ans);
@@ -382,12 +390,15 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
     for (int j = 0; j < nC; j++) {
         std::vector<EF> pt = cons_point(cs[j], v);
         Q[j].assign(pt.begin(), pt.begin() + f0);
-        E[j] = wh::eq_host(std::vector<EF>(pt.begin() + 1, pt.begin() + f0));
         fac[j] = cs[j].scal;
         if (j >= nood) sufs.emplace_back(pt.begin() + f0, pt.end());
     }
     std::vector<std::vector<EF>> Tj = w.ood_T;
-    for (auto &t : whir_T_tables(c, w.evals, v, f0, w.nz_len, sufs)) Tj.push_back(std::move(t));
+    auto eq_tables = [&] {
+        for (int j = 0; j < nC; j++) E[j] = wh::eq_host(std::vector<EF>(Q[j].begin() + 1, Q[j].end()));
+    };
+    for (auto &t : whir_T_tables(c, w.evals, v, f0, w.nz_len, sufs, eq_tables)) Tj.push_back(std::move(t));
+    if (sufs.empty()) eq_tables();
     std::vector<EF> rf;
     const EF one = eone(), three = wh::esmall(3);
     for (int t = 0; t < f0; t++) {
@@ -487,8 +498,7 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
     // final round
     size_t FN = (size_t)1 << s.m;
     std::vector<EF> ff(FN), Wf(FN);
-    c->d2h(ff.data(), s.f.p, FN);
-    c->d2h(Wf.data(), s.W.p, FN);
+    c->fetch2(ff.data(), s.f.p, FN * sizeof(EF), Wf.data(), s.W.p, FN * sizeof(EF));
     ts.add_ef(ff.data(), FN);
     grind(c, ts, pdev, cfg.final_pow_bits);
     {
@@ -511,7 +521,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     WhirCfg cfg = wh::whir_config(v, sec, 16, (int)st->folding_factor_first, (int)st->folding_factor,
                                   (int)st->log_inv_rate, (int)st->rs_domain_initial_reduction);
     Transcript ts(&ph.p16);
-    for (uint32_t x : wh::whir_domsep(cfg)) ts.observe(to_mont(x));
+    const std::vector<uint32_t> domsep = wh::whir_domsep(cfg);  // observed during the first commit
     c->ring_reset();
 
     auto tp = std::chrono::steady_clock::now();
@@ -524,7 +534,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     };
 
     WhirWitness w;
-    whir_commit(c, cfg, pdev, ts, packed, (size_t)AIR_NCOLS * N, w);
+    whir_commit(c, cfg, pdev, ts, packed, (size_t)AIR_NCOLS * N, w, &domsep);
     lap(0);
     grind(c, ts, pdev, pow_bits_air(sec, (int)wh::log2_up(AIR_NCONS)));
     EF alpha = ts.sample_ef();
@@ -767,8 +777,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     }
     const size_t tail = (size_t)1 << (n - gpu_rounds);
     std::vector<EF> hM(tail), hB(tail);
-    c->d2h(hM.data(), Mt.p, tail);
-    c->d2h(hB.data(), B.p, tail);
+    c->fetch2(hM.data(), Mt.p, tail * sizeof(EF), hB.data(), B.p, tail * sizeof(EF));
     for (size_t len = tail; len > 1; len >>= 1) {
         size_t h = len / 2;
         EF h0 = ezero(), h2 = ezero();
