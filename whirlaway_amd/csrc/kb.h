@@ -236,6 +236,79 @@ KB_HD void internal_layer_fast(F *s) {
     internal_apply_all<W>(s, fsum_lazy<W>(s));
 }
 
+// ---- internal layer of the permutation kernels (p2_permute) ----
+// v mod P for v < 2^37: with q = v >> 31 (one v_alignbit_b32) and
+// q P = q 2^31 - q (2^24 - 1), r = (v mod 2^31) + q (2^24 - 1) (one v_mad_u32_u24,
+// q < 2^6) is congruent to v and below 2^31 + 2^30 < 2P: one subtraction.
+KB_HD uint32_t fred37(uint64_t v) {
+    const uint32_t q = (uint32_t)(v >> 31);
+    const uint32_t r = ((uint32_t)v & 0x7fffffffu) + q * 0x00ffffffu;
+    return umin(r, r - P);
+}
+// acc + x as x * 1 + acc: one v_mad_u64_u32 instead of an add with carry pair
+KB_HD uint64_t acc64(uint64_t acc, uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t carry;
+    asm("v_mad_u64_u32 %0, %1, %2, 1, %0" : "+v"(acc), "=s"(carry) : "v"(x));
+    return acc;
+#else
+    return acc + x;
+#endif
+}
+// sum of W values below 2^31: pair sums (below 2^32) accumulated in 64 bits, then
+// one fred37 (the total is below W 2^31 < 2^37)
+template <int W>
+KB_HD uint32_t fsum_pairs(const F *s) {
+    uint64_t acc = (uint32_t)(s[0].v + s[1].v);
+#pragma unroll
+    for (int i = 2; i < W; i += 2) acc = acc64(acc, s[i].v + s[i + 1].v);
+    return fred37(acc);
+}
+constexpr uint32_t HALF_P1 = (P + 1) / 2;  // 1/2 mod P
+// S + d_I x, with S the canonical state sum and SP = S + P.  The entries 2^-k with
+// k >= 8 are left in [0, P + 2^23.01): x < 2^31 gives hi = x >> k < 2^23 and
+// SP + hi - lo 127 2^(24-k) in (S, 2P + 2^23), one subtraction from that bound;
+// the pair sums stay below 2^32 and the next full round's addition of its constant
+// and one subtraction give a cube input below 1.02 P.  Every other entry is fed its
+// own canonical output and returns a canonical value.
+template <int W, int I>
+KB_HD uint32_t diag_mix(uint32_t S, uint32_t SP, uint32_t x) {
+    constexpr DiagEntry e = W == 16 ? DIAG16_SPEC[I] : DIAG24_SPEC[I];
+    if constexpr (e.kind == 0 && e.val == 1) {
+        const uint32_t t = S + x;
+        return umin(t, t - P);
+    } else if constexpr (e.kind == 0 && e.val == 2) {
+        return e.neg ? fsub(F{S}, fdbl(F{x})).v : fadd(F{S}, fdbl(F{x})).v;
+    } else if constexpr (e.kind == 0) {  // 3, 4: one 64-bit multiply-add, then fred37
+        const uint32_t xx = e.neg ? P - x : x;
+        return fred37((uint64_t)xx * e.val + S);
+    } else if constexpr (e.val == 1) {  // 1/2: x/2 or (x + P)/2 = (x >> 1) + (P + 1)/2
+        const uint32_t h = (x >> 1) + ((x & 1u) ? HALF_P1 : 0u);
+        const uint32_t t = e.neg ? SP - h : S + h;
+        return umin(t, t - P);
+    } else if constexpr (!e.neg && e.val >= 8) {
+        constexpr uint32_t c = 127u << (24 - e.val);  // x 2^-k = hi - lo c mod P
+        const uint32_t lo = x & ((1u << e.val) - 1u), hi = x >> e.val;
+        const uint32_t t = SP + hi - lo * c;
+        return umin(t, t - P);
+    } else {
+        const F t = fdiv2k(F{x}, e.val);
+        return e.neg ? fsub(F{S}, t).v : fadd(F{S}, t).v;
+    }
+}
+template <int W, int I = 0>
+KB_HD void diag_mix_all(F *s, uint32_t S, uint32_t SP) {
+    if constexpr (I < W) {
+        s[I].v = diag_mix<W, I>(S, SP, s[I].v);
+        diag_mix_all<W, I + 1>(s, S, SP);
+    }
+}
+template <int W>
+KB_HD void internal_layer_mix(F *s) {
+    const uint32_t S = fsum_pairs<W>(s);
+    diag_mix_all<W>(s, S, S + P);
+}
+
 template <int W>
 KB_HD void internal_layer(F *s, const uint32_t *diag) {
     F sum = s[0];
@@ -249,23 +322,20 @@ KB_HD void internal_layer(F *s, const uint32_t *diag) {
 // The S-box output is left in [0, 1.78P) (no final correction) and the external
 // layer adds in 64 bits without reductions (rows of M4 weigh 7, the block sums
 // six blocks: outputs < 88 P < 2^38); the next S-box input folds the 64-bit value
-// once: v = hi 2^32 + lo == lo + hi (2^32 mod P), a 33-bit value whose top bit
-// folds again, then one conditional subtraction leaves it below 1.02 P, which the
-// lazy cube accepts (x^2 + m P and x2 x + m P stay below 2^64).
+// (plus its round constant) once with fold38, which leaves it below 1.02 P, what
+// the lazy cube accepts (x^2 + m P and x2 x + m P stay below 2^64).
 
 // x in [0, 1.02P): x^3 in Montgomery form, in [0, 1.78P)
 KB_HD uint32_t fcube_lazy(uint32_t x) {
     uint32_t x2 = redc_lazy((uint64_t)x * x);
     return redc_lazy((uint64_t)x2 * x);
 }
-// v < 2^38, reduced to [0, 1.02P) in 32-bit steps: hi (2^32 mod P) < 2^31 is
-// added to lo; a carry out of 2^32 is worth another 2^32 mod P (the wrapped sum
-// is then below 2^31, so adding it cannot carry again)
+// v < 2^38, reduced to [0, 2^32 - P) (below 1.016 P): as in fred37, q = v >> 31 <
+// 2^7 and r = (v mod 2^31) + q (2^24 - 1) < 2^32, then one subtraction
 KB_HD uint32_t fold38(uint64_t v) {
-    const uint32_t lo = (uint32_t)v, m = (uint32_t)(v >> 32) * C32;
-    uint32_t t = lo + m;
-    t += t < m ? C32 : 0u;
-    return umin(t, t - P);
+    const uint32_t q = (uint32_t)(v >> 31);
+    const uint32_t r = ((uint32_t)v & 0x7fffffffu) + q * 0x00ffffffu;
+    return umin(r, r - P);
 }
 // the external layer on y_i < 2^32 into 64-bit sums
 template <int W>
@@ -315,7 +385,7 @@ KB_HD void p2_permute(F *s, const C &k, const uint32_t *diag) {
 #pragma unroll 1
     for (int r = 0; r < RP; r++) {
         s[0] = fcube(fadd(s[0], k.internal[r]));
-        internal_layer_fast<W>(s);
+        internal_layer_mix<W>(s);
     }
 #pragma unroll
     for (int i = 0; i < W; i++) y[i] = fcube_lazy(fadd(s[i], k.ext_term[0][i]).v);
