@@ -5,6 +5,7 @@ crates/air/src/table.rs:32-56 / prove.rs:49-226 (AirTable::new / prove).  The
 reference prove() panics on failure; here a WhirlawayError is raised.
 """
 import ctypes
+import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -97,6 +98,10 @@ class AirTable:
         self.ctx = ctx
         self.log_length = log_length
         self._rc = rc
+        # proof output buffer kept across calls: a fresh multi-MiB array per proof
+        # costs its page faults on every call (the returned proof is a copy)
+        self._out = np.empty(1 << 20, dtype=np.uint32)
+        self._out_lock = threading.Lock()
         self.ptr = lib().wl_air_table_new(ctx.ptr, u32ptr(rc), log_length, univariate_skips)
         if not self.ptr:
             raise WhirlawayError(lib().wl_last_error().decode())
@@ -114,18 +119,23 @@ class AirTable:
         st = settings.c_struct()
         n = ctypes.c_size_t(0)
         L = lib()
-        cap = 1 << 20
-        out = np.zeros(cap, dtype=np.uint32)
-        if isinstance(witness, Witness):
-            check(L.wl_air_prove_resident(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24),
-                                          witness.ptr, u32ptr(out), cap, ctypes.byref(n)))
-        else:
-            w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1)
-            if w.size != NCOLS << self.log_length:
-                raise ValueError("witness must hold 165 columns of 2^log_length values")
-            check(L.wl_air_prove(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24), u32ptr(w),
-                                 u32ptr(out), cap, ctypes.byref(n)))
-        return out[: n.value].copy()
+        own = self._out_lock.acquire(blocking=False)
+        out = self._out if own else np.empty(self._out.size, dtype=np.uint32)
+        cap = out.size
+        try:
+            if isinstance(witness, Witness):
+                check(L.wl_air_prove_resident(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24),
+                                              witness.ptr, u32ptr(out), cap, ctypes.byref(n)))
+            else:
+                w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1)
+                if w.size != NCOLS << self.log_length:
+                    raise ValueError("witness must hold 165 columns of 2^log_length values")
+                check(L.wl_air_prove(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24), u32ptr(w),
+                                     u32ptr(out), cap, ctypes.byref(n)))
+            return out[: n.value].copy()
+        finally:
+            if own:
+                self._out_lock.release()
 
 
 class Witness:

@@ -160,6 +160,9 @@ struct wl_ctx {
     // witness, grinding blocks done} armed as {0xffffffff, 0}, then the block
     // counter of multi-block publishes (0)
     uint32_t *sync_dev = nullptr;
+    // Poseidon2 constants of the last proof on the device (re-sent only when they change)
+    wl::P2Dev *p2_dev = nullptr;
+    wl::P2Dev p2_host{};
     uint32_t *sync_words() {
         if (sync_dev) return sync_dev;
         HIP_CHECK(hipMalloc((void **)&sync_dev, 4 * sizeof(uint32_t)));
@@ -510,6 +513,18 @@ inline DBuf<P2Dev> upload_p2(wl_ctx *c, const P2Dev &host) {
     DBuf<P2Dev> dev(c->pool, 1);
     c->h2d(dev.p, &host, 1);
     return dev;
+}
+// the context's device copy of the constants, uploaded when they differ from the
+// previous call's (proofs in a row with the same permutation skip the copy)
+inline const P2Dev *context_p2(wl_ctx *c, const P2Dev &host) {
+    if (!c->p2_dev) {
+        HIP_CHECK(hipMalloc((void **)&c->p2_dev, sizeof(P2Dev)));
+    } else if (memcmp(&c->p2_host, &host, sizeof host) == 0) {
+        return c->p2_dev;
+    }
+    c->p2_host = host;
+    c->h2d(c->p2_dev, &host, 1);
+    return c->p2_dev;
 }
 
 inline int pow_bits_sumcheck(int sec, int degree) {  // crates/sumcheck/src/lib.rs:21-31

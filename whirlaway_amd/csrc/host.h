@@ -268,6 +268,7 @@ struct Transcript {
 
     explicit Transcript(const P2Consts16 *p) : p16(p) {
         for (int i = 0; i < 16; i++) state[i] = fzero();
+        data.reserve((size_t)1 << 16);  // a 2^17-row proof holds ~55K words
     }
     void duplex() {
         for (int i = 0; i < n_in; i++) state[i] = in_buf[i];

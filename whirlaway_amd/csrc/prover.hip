@@ -866,6 +866,7 @@ void wl_destroy(wl_ctx *c) {
     if (c->mbox) (void)hipHostFree(c->mbox);
     if (c->args_dev) (void)hipFree(c->args_dev);
     if (c->sync_dev) (void)hipFree(c->sync_dev);
+    if (c->p2_dev) (void)hipFree(c->p2_dev);
     for (auto &e : c->kt_free) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (auto &v : c->kt_pairs)
         for (auto &e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
@@ -921,10 +922,10 @@ static int prove_common(wl_table *t, const wl_air_settings *st, const uint32_t *
     wl_ctx *c = t->ctx;
     P2Dev ph;
     load_p2(p16c, p24c, ph);
-    DBuf<P2Dev> pd = upload_p2(c, ph);
+    const P2Dev *pd = context_p2(c, ph);
     std::vector<uint32_t> proof;
     try {
-        proof = air_prove(t, st, ph, pd.p, packed);
+        proof = air_prove(t, st, ph, pd, packed);
         HIP_CHECK(hipStreamSynchronize(c->stream));  // nothing of this proof left in flight
     } catch (...) {
         c->release_all();  // device waits for host values must not outlive the failure

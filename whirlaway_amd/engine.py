@@ -6,6 +6,7 @@ this module has no counterpart there.  Each job runs the single-proof path, so
 its proof is the one `AirTable.prove` returns for the same inputs.
 """
 import ctypes
+import threading
 
 import numpy as np
 
@@ -41,6 +42,7 @@ class Engine:
         if not self.ptr:
             raise WhirlawayError(lib().wl_last_error().decode())
         self._keep = {}
+        self._tls = threading.local()
 
     def submit(self, log_n, air_rc, inputs, p16, p24, settings=None):
         """Queue one proof; returns a ticket.  The arrays are kept alive until wait()."""
@@ -61,14 +63,17 @@ class Engine:
 
     def wait(self, ticket):
         """Returns (proof_data, {"queued_ms", "run_ms", "latency_ms"})."""
-        cap = 1 << 18
+        # per-thread output buffer kept across calls (a fresh array per proof costs
+        # its page faults every time); the returned proof is a copy
+        out = getattr(self._tls, "out", None)
+        if out is None:
+            out = self._tls.out = np.empty(1 << 18, dtype=np.uint32)
         n = ctypes.c_size_t(0)
         times = (ctypes.c_double * 3)()
         while True:
-            out = np.zeros(cap, dtype=np.uint32)
-            rc = lib().wl_engine_wait(self.ptr, ticket, u32ptr(out), cap, ctypes.byref(n), times)
-            if rc == 3 and n.value > cap:  # WL_ERR_BUFFER
-                cap = n.value
+            rc = lib().wl_engine_wait(self.ptr, ticket, u32ptr(out), out.size, ctypes.byref(n), times)
+            if rc == 3 and n.value > out.size:  # WL_ERR_BUFFER
+                out = self._tls.out = np.empty(n.value, dtype=np.uint32)
                 continue
             break
         self._keep.pop(ticket, None)
