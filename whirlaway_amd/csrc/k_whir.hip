@@ -456,6 +456,124 @@ __global__ void __launch_bounds__(256) k_merkle_leaves(const F *cw, int width, i
     for (int i = 0; i < 8; i++) o[i] = s[i];
 }
 
+// Poseidon2-24 on a lane pair (lanes 2p and 2p + 1 of a wave), for commits with
+// too few rows to give every SIMD two waves at one row per lane.  Lane h holds
+// state elements 12h .. 12h + 11, the external layer's blocks 3h .. 3h + 2: the
+// 4x4 blocks stay in the lane and only the four residue-class sums over the blocks
+// cross lanes (one DPP swap of the 64-bit partial sums).  A partial round sums the
+// lane's 12 elements, swaps the partial sum, and gives every element S + d x with
+// the lane's own diagonal entry (one Montgomery product); the S-box result is kept
+// by lane 0 only.  Same permutation as p2_permute<24, 23>.
+__device__ __forceinline__ uint32_t pair_swap(uint32_t x) {  // DPP quad_perm [1,0,3,2]
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint64_t pair_swap64(uint64_t x) {
+    return (uint64_t)pair_swap((uint32_t)x) | ((uint64_t)pair_swap((uint32_t)(x >> 32)) << 32);
+}
+// the external layer on the lane's 12 values y < 2^32 into 64-bit sums
+__device__ __forceinline__ void mds_pair(const uint32_t *y, uint64_t *o) {
+#pragma unroll
+    for (int c = 0; c < 12; c += 4) {
+        const uint64_t x0 = y[c], x1 = y[c + 1], x2 = y[c + 2], x3 = y[c + 3];
+        const uint64_t t01 = x0 + x1, t23 = x2 + x3, t0123 = t01 + t23;
+        const uint64_t t01123 = t0123 + x1, t01233 = t0123 + x3;
+        o[c] = t01123 + t01;
+        o[c + 1] = t01123 + (x2 << 1);
+        o[c + 2] = t01233 + t23;
+        o[c + 3] = t01233 + (x0 << 1);
+    }
+    uint64_t ps[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        ps[k] = o[k] + o[4 + k] + o[8 + k];
+        ps[k] += pair_swap64(ps[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < 12; i++) o[i] += ps[i & 3];
+}
+// every lane of the wave must call this (DPP reads the partner lane)
+__device__ __forceinline__ void p24_pair(F *s, bool hi, const P2Consts24 &k, const F *d) {
+    uint32_t y[12];
+    uint64_t o[12];
+    mds_pair(&s[0].v, o);
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < 12; i++) {
+            const uint32_t c = hi ? k.ext_init[r][12 + i].v : k.ext_init[r][i].v;
+            y[i] = fcube_lazy(fold38(o[i] + c));
+        }
+        mds_pair(y, o);
+    }
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        const uint32_t t = fold38(o[i]);
+        s[i] = F{umin(t, t - P)};
+    }
+#pragma unroll 1
+    for (int r = 0; r < 23; r++) {
+        const F y0 = fcube(fadd(s[0], k.internal[r]));
+        if (!hi) s[0] = y0;
+        uint64_t acc = (uint32_t)(s[0].v + s[1].v);
+#pragma unroll
+        for (int i = 2; i < 12; i += 2) acc = acc64(acc, s[i].v + s[i + 1].v);
+        const uint32_t part = fred37(acc);
+        const F S = fadd(F{part}, F{pair_swap(part)});
+#pragma unroll
+        for (int i = 0; i < 12; i++) s[i] = fadd(S, fmul(s[i], d[i]));
+    }
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        const uint32_t c = hi ? k.ext_term[0][12 + i].v : k.ext_term[0][i].v;
+        y[i] = fcube_lazy(fadd(s[i], F{c}).v);
+    }
+    mds_pair(y, o);
+#pragma unroll 1
+    for (int r = 1; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < 12; i++) {
+            const uint32_t c = hi ? k.ext_term[r][12 + i].v : k.ext_term[r][i].v;
+            y[i] = fcube_lazy(fold38(o[i] + c));
+        }
+        mds_pair(y, o);
+    }
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        const uint32_t t = fold38(o[i]);
+        s[i] = F{umin(t, t - P)};
+    }
+}
+// k_merkle_leaves with a lane pair per row: 2 rows threads (rows a multiple of
+// 32, so every wave is full and every lane has its partner)
+__global__ void __launch_bounds__(256) k_merkle_leaves_pair(const F *cw, int width, int nnz, size_t rows, int la,
+                                                           int lb, const P2DevT *pc, F *nodes) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t q = g >> 1;
+    const bool hi = (g & 1) != 0;
+    const size_t r = (q >> lb) | ((q & (((size_t)1 << lb) - 1)) << la);
+    F d[12], s[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        d[i] = F{hi ? DDIAG.d24[12 + i] : DDIAG.d24[i]};
+        s[i] = F{0};
+    }
+    const int base = hi ? 12 : 0;  // absorbed positions of the lane's slots
+    for (int off = 0; off < width; off += 16) {
+        const int n = width - off < 16 ? width - off : 16;
+#pragma unroll
+        for (int i = 0; i < 12; i++) {
+            const int j = base + i;
+            if (j < n) s[i] = off + j < nnz ? cw[(size_t)(off + j) * rows + q] : F{0};
+        }
+        p24_pair(s, hi, pc->p24, d);
+    }
+    if (!hi) {
+        F *o = nodes + (rows + r) * 8;
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = s[i];
+    }
+}
+
 __device__ __forceinline__ void compress_node(F *nodes, size_t i, const P2DevT *pc) {
     F s[16];
     const F *l = nodes + 2 * i * 8;
@@ -557,8 +675,19 @@ __global__ void __launch_bounds__(256) k_merkle_sub(F *nodes, size_t cnt, int le
     }
 }
 
+// commits of up to 2^MERKLE_PAIR_MAX_LOG rows hash with a lane pair per row
+#ifndef MERKLE_PAIR_MAX_LOG
+#define MERKLE_PAIR_MAX_LOG 15
+#endif
 void merkle_leaves(const F *cw, int width, int nnz, int log_rows, int la, const void *pc, F *nodes, hipStream_t s) {
     size_t rows = (size_t)1 << log_rows;
+    if (log_rows >= 5 && log_rows <= MERKLE_PAIR_MAX_LOG) {
+        const size_t threads = 2 * rows;  // a multiple of 64
+        const unsigned bs = threads < 256 ? (unsigned)threads : 256u;
+        k_merkle_leaves_pair<<<(unsigned)(threads / bs), bs, 0, s>>>(cw, width, nnz, rows, la, log_rows - la,
+                                                                     (const P2DevT *)pc, nodes);
+        return;
+    }
     k_merkle_leaves<<<nblk(rows), 256, 0, s>>>(cw, width, nnz, rows, la, log_rows - la, (const P2DevT *)pc, nodes);
 }
 void merkle_build(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s) {
