@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE passes (profiles/prove_n17_hbm_traffic.md), fetch scaled x2.0
 # and write x1.0 by the same counters on a 256 MiB device copy.  (read MiB, write MiB)
 MEASURED_TRAFFIC_N17 = {
-    1: (92.23, 5.75),     # k_merkle_leaves
+    1: (69.73, 5.75),     # k_merkle_leaves (3 launches) + k_merkle_leaves_pair (1), averaged
     3: (92.93, 139.00),   # k_ntt2_a + k_ntt2_b (one rs_encode, averaged over the 4 commits)
 }
 PHASES = ["commit", "zerocheck_round0", "zerocheck_rounds", "batching", "inner_sumcheck", "whir_open",
