@@ -255,25 +255,24 @@ KB_HD uint64_t acc64(uint64_t acc, uint32_t x) {
     return acc + x;
 #endif
 }
-// sum of W values below 2^31: pair sums (below 2^32) accumulated in 64 bits, then
-// one fred37 (the total is below W 2^31 < 2^37)
-template <int W>
-KB_HD uint32_t fsum_pairs(const F *s) {
-    uint64_t acc = (uint32_t)(s[0].v + s[1].v);
-#pragma unroll
-    for (int i = 2; i < W; i += 2) acc = acc64(acc, s[i].v + s[i + 1].v);
-    return fred37(acc);
-}
 constexpr uint32_t HALF_P1 = (P + 1) / 2;  // 1/2 mod P
-// S + d_I x, with S the canonical state sum and SP = S + P.  The entries 2^-k with
-// k >= 8 are left in [0, P + 2^23.01): x < 2^31 gives hi = x >> k < 2^23 and
-// SP + hi - lo 127 2^(24-k) in (S, 2P + 2^23), one subtraction from that bound;
-// the pair sums stay below 2^32 and the next full round's addition of its constant
-// and one subtraction give a cube input below 1.02 P.  Every other entry is fed its
-// own canonical output and returns a canonical value.
+template <int W, int I>
+constexpr DiagEntry diag_entry() { return W == 16 ? DIAG16_SPEC[I] : DIAG24_SPEC[I]; }
+// entries 2^-k and -2^-k run lazily in [0, 2P), except 2^-k with k >= 8 (below)
+template <int W, int I>
+constexpr bool diag_lazy() {
+    constexpr DiagEntry e = diag_entry<W, I>();
+    return e.kind == 1 && e.val >= 2 && (e.neg || e.val < 8);
+}
+// S + d_I x, with S the canonical state sum and SP = S + P.  Every entry is fed its
+// own previous output.  Lazy entries (x < 2P): fdiv2k is canonical for any
+// x < 2^32, and S + t or SP - t lies in [0, 2P) with no subtraction.  Entries 2^-k
+// with k >= 8 stay in [0, P + 2^23.01): x < 2^31 gives hi = x >> k < 2^23 and
+// SP + hi - lo 127 2^(24-k) in (S, 2P + 2^23), one subtraction from that bound.
+// The other entries take and return canonical values.
 template <int W, int I>
 KB_HD uint32_t diag_mix(uint32_t S, uint32_t SP, uint32_t x) {
-    constexpr DiagEntry e = W == 16 ? DIAG16_SPEC[I] : DIAG24_SPEC[I];
+    constexpr DiagEntry e = diag_entry<W, I>();
     if constexpr (e.kind == 0 && e.val == 1) {
         const uint32_t t = S + x;
         return umin(t, t - P);
@@ -286,14 +285,14 @@ KB_HD uint32_t diag_mix(uint32_t S, uint32_t SP, uint32_t x) {
         const uint32_t h = (x >> 1) + ((x & 1u) ? HALF_P1 : 0u);
         const uint32_t t = e.neg ? SP - h : S + h;
         return umin(t, t - P);
-    } else if constexpr (!e.neg && e.val >= 8) {
+    } else if constexpr (!diag_lazy<W, I>()) {
         constexpr uint32_t c = 127u << (24 - e.val);  // x 2^-k = hi - lo c mod P
         const uint32_t lo = x & ((1u << e.val) - 1u), hi = x >> e.val;
         const uint32_t t = SP + hi - lo * c;
         return umin(t, t - P);
     } else {
-        const F t = fdiv2k(F{x}, e.val);
-        return e.neg ? fsub(F{S}, t).v : fadd(F{S}, t).v;
+        const uint32_t t = fdiv2k(F{x}, e.val).v;
+        return e.neg ? SP - t : S + t;
     }
 }
 template <int W, int I = 0>
@@ -303,10 +302,35 @@ KB_HD void diag_mix_all(F *s, uint32_t S, uint32_t SP) {
         diag_mix_all<W, I + 1>(s, S, SP);
     }
 }
+// sum of the state: neighbours that are not lazy (each below 2^31) added in pairs,
+// lazy entries (below 2P) alone, all accumulated in 64 bits; below 37 * 2^31 < 2^37
+template <int W, int I>
+constexpr bool pair_ok() {
+    if constexpr (I + 1 < W) return !diag_lazy<W, I>() && !diag_lazy<W, I + 1>();
+    else return false;
+}
+template <int W, int I = 0>
+KB_HD uint64_t state_sum64(const F *s, uint64_t acc) {
+    if constexpr (I >= W) {
+        return acc;
+    } else if constexpr (pair_ok<W, I>()) {
+        return state_sum64<W, I + 2>(s, acc64(acc, s[I].v + s[I + 1].v));
+    } else {
+        return state_sum64<W, I + 1>(s, acc64(acc, s[I].v));
+    }
+}
 template <int W>
 KB_HD void internal_layer_mix(F *s) {
-    const uint32_t S = fsum_pairs<W>(s);
+    const uint32_t S = fred37(state_sum64<W>(s, 0));
     diag_mix_all<W>(s, S, S + P);
+}
+// lazy entries back below P (the state leaving the partial rounds)
+template <int W, int I = 0>
+KB_HD void canon_lazy(F *s) {
+    if constexpr (I < W) {
+        if constexpr (diag_lazy<W, I>()) s[I].v = umin(s[I].v, s[I].v - P);
+        canon_lazy<W, I + 1>(s);
+    }
 }
 
 template <int W>
@@ -387,6 +411,7 @@ KB_HD void p2_permute(F *s, const C &k, const uint32_t *diag) {
         s[0] = fcube(fadd(s[0], k.internal[r]));
         internal_layer_mix<W>(s);
     }
+    canon_lazy<W>(s);
 #pragma unroll
     for (int i = 0; i < W; i++) y[i] = fcube_lazy(fadd(s[i], k.ext_term[0][i]).v);
     mds_light64<W>(y, o);
