@@ -625,20 +625,37 @@ __device__ __forceinline__ F row_sum16(F x) {
     s = fadd(s, dpp<DPP_HMIRROR>(s));
     return fadd(s, dpp<DPP_MIRROR>(s));
 }
-// all 64 lanes must call this (DPP reads neighbours); i = lane & 15
-__device__ __forceinline__ F p16_row(F x, int i, const P2Consts16 &k) {
+// A lane's round constants and diagonal entry, loaded once per kernel: a load
+// inside the rounds would put a cache round trip on the permutation's serial chain
+// (one wave per permutation row, nothing else to hide it).
+struct Row16 {
+    F ei[4], et[4], d;
+    bool first;
+};
+__device__ __forceinline__ Row16 row16_consts(int i, const P2Consts16 &k) {
+    Row16 c;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        c.ei[r] = k.ext_init[r][i];
+        c.et[r] = k.ext_term[r][i];
+    }
+    c.d = F{DDIAG.d16[i]};
+    c.first = i == 0;
+    return c;
+}
+// all 64 lanes must call this (DPP reads neighbours); c from row16_consts(lane & 15)
+__device__ __forceinline__ F p16_row(F x, const Row16 &c, const P2Consts16 &k) {
     x = ext_row16(x);
-#pragma unroll 1
-    for (int r = 0; r < 4; r++) x = ext_row16(fcube(fadd(x, k.ext_init[r][i])));
-    const F d = F{DDIAG.d16[i]};
-#pragma unroll 1
+#pragma unroll
+    for (int r = 0; r < 4; r++) x = ext_row16(fcube(fadd(x, c.ei[r])));
+#pragma unroll
     for (int r = 0; r < 20; r++) {
         const F y = fcube(fadd(x, k.internal[r]));
-        if (i == 0) x = y;
-        x = fadd(row_sum16(x), fmul(x, d));
+        if (c.first) x = y;
+        x = fadd(row_sum16(x), fmul(x, c.d));
     }
-#pragma unroll 1
-    for (int r = 0; r < 4; r++) x = ext_row16(fcube(fadd(x, k.ext_term[r][i])));
+#pragma unroll
+    for (int r = 0; r < 4; r++) x = ext_row16(fcube(fadd(x, c.et[r])));
     return x;
 }
 
@@ -651,6 +668,7 @@ __global__ void __launch_bounds__(256) k_merkle_sub(F *nodes, size_t cnt, int le
     __shared__ F kid[32 * 8], par[16 * 8];
     const int g = threadIdx.x >> 4, i = threadIdx.x & 15;
     const size_t pb = (size_t)1 << (levels - 1);  // parents per block at the first level
+    const Row16 rc = row16_consts(i, pc->p16);
     for (int w = threadIdx.x; w < (int)(2 * pb * 8); w += blockDim.x)
         kid[w] = nodes[(2 * (cnt + blockIdx.x * pb)) * 8 + w];
     __syncthreads();
@@ -663,7 +681,7 @@ __global__ void __launch_bounds__(256) k_merkle_sub(F *nodes, size_t cnt, int le
         if ((g & ~3) < np) {
             const int gp = g < np ? g : 0;
             F x = kid[gp * 16 + i];
-            x = p16_row(x, i, pc->p16);
+            x = p16_row(x, rc, pc->p16);
             if (g < np && i < 8) {
                 par[g * 8 + i] = x;
                 nodes[(first + g) * 8 + i] = x;
