@@ -574,6 +574,103 @@ __global__ void __launch_bounds__(256) k_merkle_leaves_pair(const F *cw, int wid
     }
 }
 
+// Poseidon2-16 on a lane pair, as p24_pair: lane h holds state elements
+// 8h .. 8h + 7 (blocks 2h, 2h + 1); one DPP swap for the residue-class sums of the
+// external layer and one for the partial rounds' state sum.  Same permutation as
+// p2_permute<16, 20>; every lane of the wave must call it.
+__device__ __forceinline__ void mds_pair16(const uint32_t *y, uint64_t *o) {
+#pragma unroll
+    for (int c = 0; c < 8; c += 4) {
+        const uint64_t x0 = y[c], x1 = y[c + 1], x2 = y[c + 2], x3 = y[c + 3];
+        const uint64_t t01 = x0 + x1, t23 = x2 + x3, t0123 = t01 + t23;
+        const uint64_t t01123 = t0123 + x1, t01233 = t0123 + x3;
+        o[c] = t01123 + t01;
+        o[c + 1] = t01123 + (x2 << 1);
+        o[c + 2] = t01233 + t23;
+        o[c + 3] = t01233 + (x0 << 1);
+    }
+    uint64_t ps[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        ps[k] = o[k] + o[4 + k];
+        ps[k] += pair_swap64(ps[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] += ps[i & 3];
+}
+__device__ __forceinline__ void p16_pair(F *s, bool hi, const P2Consts16 &k, const F *d) {
+    uint32_t y[8];
+    uint64_t o[8];
+    mds_pair16(&s[0].v, o);
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t c = hi ? k.ext_init[r][8 + i].v : k.ext_init[r][i].v;
+            y[i] = fcube_lazy(fold38(o[i] + c));
+        }
+        mds_pair16(y, o);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t t = fold38(o[i]);
+        s[i] = F{umin(t, t - P)};
+    }
+#pragma unroll 1
+    for (int r = 0; r < 20; r++) {
+        const F y0 = fcube(fadd(s[0], k.internal[r]));
+        if (!hi) s[0] = y0;
+        uint64_t acc = (uint32_t)(s[0].v + s[1].v);
+#pragma unroll
+        for (int i = 2; i < 8; i += 2) acc = acc64(acc, s[i].v + s[i + 1].v);
+        const uint32_t part = fred37(acc);
+        const F S = fadd(F{part}, F{pair_swap(part)});
+#pragma unroll
+        for (int i = 0; i < 8; i++) s[i] = fadd(S, fmul(s[i], d[i]));
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t c = hi ? k.ext_term[0][8 + i].v : k.ext_term[0][i].v;
+        y[i] = fcube_lazy(fadd(s[i], F{c}).v);
+    }
+    mds_pair16(y, o);
+#pragma unroll 1
+    for (int r = 1; r < 4; r++) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t c = hi ? k.ext_term[r][8 + i].v : k.ext_term[r][i].v;
+            y[i] = fcube_lazy(fold38(o[i] + c));
+        }
+        mds_pair16(y, o);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t t = fold38(o[i]);
+        s[i] = F{umin(t, t - P)};
+    }
+}
+__device__ __forceinline__ void diag16_pair(bool hi, F *d) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) d[i] = F{hi ? DDIAG.d16[8 + i] : DDIAG.d16[i]};
+}
+// one tree level on lane pairs (2 cnt threads, cnt a multiple of 32): lane h of
+// pair i loads child 2(cnt + i) + h, lane 0 writes the parent
+__global__ void __launch_bounds__(256) k_merkle_level_pair(F *nodes, size_t cnt, const P2DevT *pc) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = cnt + (g >> 1);
+    const bool hi = (g & 1) != 0;
+    F d[8], s[8];
+    diag16_pair(hi, d);
+    const F *child = nodes + (2 * i + (hi ? 1 : 0)) * 8;
+#pragma unroll
+    for (int j = 0; j < 8; j++) s[j] = child[j];
+    p16_pair(s, hi, pc->p16, d);
+    if (!hi) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) nodes[i * 8 + j] = s[j];
+    }
+}
+
 __device__ __forceinline__ void compress_node(F *nodes, size_t i, const P2DevT *pc) {
     F s[16];
     const F *l = nodes + 2 * i * 8;
@@ -716,9 +813,13 @@ void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s) {
     const P2DevT *p = (const P2DevT *)pc;
     size_t rows = (size_t)1 << log_rows;
     size_t cnt = rows >> 1;
-    // wide levels: one thread per node
+    // wide levels: one thread per node; the 2^15-node level (512 waves at one thread
+    // per node, half the SIMDs) on lane pairs
     while (cnt >= ((size_t)1 << 15)) {
-        k_merkle_level<<<nblk(cnt), 256, 0, s>>>(nodes, cnt, p);
+        if (cnt == ((size_t)1 << 15))
+            k_merkle_level_pair<<<(unsigned)(2 * cnt / 256), 256, 0, s>>>(nodes, cnt, p);
+        else
+            k_merkle_level<<<nblk(cnt), 256, 0, s>>>(nodes, cnt, p);
         cnt >>= 1;
     }
     // narrow levels: subtrees of up to 5 levels per block, 16 lanes per compression
@@ -814,8 +915,50 @@ __global__ void __launch_bounds__(256) k_grind_pass(const P2DevT *pc, State16 st
         *flag = seq;
     }
 }
+// k_grind_pass on lane pairs (for passes of up to 2^16 candidates, which at one
+// candidate per lane leave SIMDs idle): pair i tries w = base + i; the witness
+// position (below 8) and the sampled element 7 are in lane 0.  Lanes past the
+// count run the permutation for their partner's sake and record nothing.
+__global__ void __launch_bounds__(256) k_grind_pass_pair(const P2DevT *pc, State16 st, int pos, uint32_t mask,
+                                                        uint32_t base, uint32_t count, uint32_t *state,
+                                                        uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = g >> 1, w = base + i;
+    const bool hi = (g & 1) != 0;
+    const bool valid = i < count && w < P;
+    F d[8], s[8];
+    diag16_pair(hi, d);
+#pragma unroll
+    for (int k = 0; k < 8; k++) s[k] = hi ? st.s[8 + k] : st.s[k];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (!hi && k == pos) s[k] = to_mont(valid ? w : 0u);
+    p16_pair(s, hi, pc->p16, d);
+    if (!hi && valid && (from_mont(s[7]) & mask) == 0) atomicMin(state, w);
+    __shared__ uint32_t last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(state + 1, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        host_dst[0] = __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(state, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(state + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence_system();
+        *flag = seq;
+    }
+}
 void grind_pass(const void *pc, State16 st, int pos, uint32_t mask, uint32_t base, uint32_t count,
                 uint32_t *state, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq, hipStream_t s) {
+    if (count <= (1u << 16) && pos < 8) {
+        const unsigned blocks = (unsigned)((2ull * count + 255) / 256);
+        k_grind_pass_pair<<<blocks, 256, 0, s>>>((const P2DevT *)pc, st, pos, mask, base, count, state, host_dst,
+                                                 flag, seq);
+        return;
+    }
     k_grind_pass<<<nblk(count), 256, 0, s>>>((const P2DevT *)pc, st, pos, mask, base, count, state, host_dst, flag,
                                             seq);
 }
