@@ -304,24 +304,26 @@ KB_HD void diag_mix_all(F *s, uint32_t S, uint32_t SP) {
 }
 // sum of the state: neighbours that are not lazy (each below 2^31) added in pairs,
 // lazy entries (below 2P) alone, all accumulated in 64 bits; below 37 * 2^31 < 2^37
+// in two interleaved accumulators:
 template <int W, int I>
 constexpr bool pair_ok() {
     if constexpr (I + 1 < W) return !diag_lazy<W, I>() && !diag_lazy<W, I + 1>();
     else return false;
 }
+// terms alternate between them, so the serial chain of 64-bit
+// multiply-adds is half as long (one wave per SIMD on the small commits waits on it)
 template <int W, int I = 0>
-KB_HD uint64_t state_sum64(const F *s, uint64_t acc) {
-    if constexpr (I >= W) {
-        return acc;
-    } else if constexpr (pair_ok<W, I>()) {
-        return state_sum64<W, I + 2>(s, acc64(acc, s[I].v + s[I + 1].v));
-    } else {
-        return state_sum64<W, I + 1>(s, acc64(acc, s[I].v));
+KB_HD void state_sum64_2(const F *s, uint64_t &a, uint64_t &b) {
+    if constexpr (I < W) {
+        if constexpr (pair_ok<W, I>()) { a = acc64(a, s[I].v + s[I + 1].v); state_sum64_2<W, I + 2>(s, b, a); }
+        else { a = acc64(a, s[I].v); state_sum64_2<W, I + 1>(s, b, a); }
     }
 }
 template <int W>
 KB_HD void internal_layer_mix(F *s) {
-    const uint32_t S = fred37(state_sum64<W>(s, 0));
+    uint64_t a = 0, b = 0;
+    state_sum64_2<W>(s, a, b);
+    const uint32_t S = fred37(a + b);
     diag_mix_all<W>(s, S, S + P);
 }
 // lazy entries back below P (the state leaving the partial rounds)
