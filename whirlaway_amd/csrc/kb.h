@@ -302,16 +302,16 @@ KB_HD void diag_mix_all(F *s, uint32_t S, uint32_t SP) {
         diag_mix_all<W, I + 1>(s, S, SP);
     }
 }
-// sum of the state: neighbours that are not lazy (each below 2^31) added in pairs,
-// lazy entries (below 2P) alone, all accumulated in 64 bits; below 37 * 2^31 < 2^37
-// in two interleaved accumulators:
 template <int W, int I>
 constexpr bool pair_ok() {
     if constexpr (I + 1 < W) return !diag_lazy<W, I>() && !diag_lazy<W, I + 1>();
     else return false;
 }
-// terms alternate between them, so the serial chain of 64-bit
-// multiply-adds is half as long (one wave per SIMD on the small commits waits on it)
+// sum of the state: neighbours that are not lazy (each below 2^31) added in pairs,
+// lazy entries (below 2P) alone, all accumulated in 64 bits (below 37 * 2^31 <
+// 2^37).  The terms alternate between two accumulators, so the serial chain of
+// 64-bit multiply-adds is half as long (one wave per SIMD on the small commits
+// waits on it).
 template <int W, int I = 0>
 KB_HD void state_sum64_2(const F *s, uint64_t &a, uint64_t &b) {
     if constexpr (I < W) {
