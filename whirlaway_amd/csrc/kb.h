@@ -319,11 +319,12 @@ KB_HD void state_sum64_2(const F *s, uint64_t &a, uint64_t &b) {
         else { a = acc64(a, s[I].v); state_sum64_2<W, I + 1>(s, b, a); }
     }
 }
+// s[0], the S-box output, joins the sum last: the other terms do not wait for the cube
 template <int W>
 KB_HD void internal_layer_mix(F *s) {
     uint64_t a = 0, b = 0;
-    state_sum64_2<W>(s, a, b);
-    const uint32_t S = fred37(a + b);
+    state_sum64_2<W, 1>(s, a, b);
+    const uint32_t S = fred37(a + b + s[0].v);
     diag_mix_all<W>(s, S, S + P);
 }
 // lazy entries back below P (the state leaving the partial rounds)
