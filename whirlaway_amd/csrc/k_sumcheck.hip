@@ -197,14 +197,15 @@ void reduce_publish(const EF *in, int n_items, int n_parts, EF *host_dst, volati
 // LDS while a product tree over 32 slots forms the block's factor from the high
 // variables (bits of b); entry = factor * low[t].  Latency ~max(lo, 5) EF products.
 __device__ __forceinline__ void eq_block(const EqPoint &pt, int lo, uint32_t b, EF *out) {
-    __shared__ EF ta[256], tb[256], ha[32], hb[32];
+    __shared__ EF ta[256], tb[256], ha[32], hb[32], pl[32];
     const int t = threadIdx.x, hi = pt.n - lo;
+    // every coordinate is loaded once, all in parallel, before the first barrier
+    // (a load per level would put one memory round trip per level on the chain)
+    const EF mine = t < pt.n ? pt.p[t] : ezero();
+    if (t >= hi && t < pt.n) pl[t - hi] = mine;
     if (t < 32) {
         EF f = eone();
-        if (t < hi) {
-            const EF p = pt.p[t];
-            f = ((b >> (hi - 1 - t)) & 1) ? p : esub(eone(), p);
-        }
+        if (t < hi) f = ((b >> (hi - 1 - t)) & 1) ? mine : esub(eone(), mine);
         if (t == 0) f = emul(f, pt.scale);
         ha[t] = f;
     }
@@ -214,7 +215,7 @@ __device__ __forceinline__ void eq_block(const EqPoint &pt, int lo, uint32_t b, 
     const int levels = lo > 5 ? lo : 5;
     for (int l = 0; l < levels; l++) {
         if (l < lo && t < (1 << l)) {
-            const EF p = pt.p[hi + l];
+            const EF p = pl[l];
             const EF v = src[t];
             const EF h = emul(v, p);
             dst[2 * t] = esub(v, h);
