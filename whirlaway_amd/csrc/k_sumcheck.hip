@@ -52,7 +52,13 @@ __global__ void k_reduce_parts(const EF *in, int n_parts, EF *out) {
     __shared__ EF lds[16];
     const EF *p = in + (size_t)blockIdx.x * n_parts;
     EF acc = ezero();
-    for (int i = threadIdx.x; i < n_parts; i += blockDim.x) acc = eadd(acc, p[i]);
+    const int st = blockDim.x;
+    int i = threadIdx.x;
+    for (; i + 3 * st < n_parts; i += 4 * st) {
+        const EF a = p[i], b = p[i + st], c = p[i + 2 * st], d = p[i + 3 * st];
+        acc = eadd(acc, eadd(eadd(a, b), eadd(c, d)));
+    }
+    for (; i < n_parts; i += st) acc = eadd(acc, p[i]);
     EF t = block_sum(acc, lds);
     if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
@@ -127,7 +133,13 @@ __device__ __forceinline__ void reduce_items(const EF *in, int n_items, int n_pa
     for (int it = threadIdx.x >> 6; it < n_items; it += nw) {
         const EF *p = in + (size_t)it * n_parts;
         EF acc = ezero();
-        for (int i = lane; i < n_parts; i += 64) acc = eadd(acc, p[i]);
+        int i = lane;
+        // four loads in flight per lane: a large round leaves 16 parts per lane
+        for (; i + 192 < n_parts; i += 256) {
+            const EF a = p[i], b = p[i + 64], c = p[i + 128], d = p[i + 192];
+            acc = eadd(acc, eadd(eadd(a, b), eadd(c, d)));
+        }
+        for (; i < n_parts; i += 64) acc = eadd(acc, p[i]);
         acc = wave_sum(acc);
         if (lane == 0) dst[it] = acc;
     }
