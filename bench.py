@@ -54,8 +54,10 @@ def run_micro(args):
     """Kernel micro configs at full size: (a) 2^24 base-field values folded 24 times
     by extension challenges (benches/folding_benchmarks.rs shape), (b) the eq-weighted
     a^2 b product sumcheck over n = 24 variables.  Device time with HIP events;
-    algorithmic bytes: (a) read 2^24 F + write/read each EF table once, (b) per round
-    read the two tables and the eq table and write the folded tables."""
+    algorithmic bytes: (a) read 2^24 F + write/read each EF table once, (b) rounds 0
+    and 1 read the two base-field tables, round 1 writes the two folded EF tables of
+    2^23 entries, each later round reads the two tables of the round before and writes
+    them folded (the eq weights come from tables of at most 2^14 entries)."""
     import numpy as np
     import whirlaway_amd as wa
     from whirlaway_amd import sumcheck, utils
@@ -75,8 +77,8 @@ def run_micro(args):
     _, _, p16, _ = bench_inputs(args.seed, 4, with_inputs=False)
     sumcheck.prove_a2b(ctx, p16, a, b, z)
     sc_ms = min(sumcheck.prove_a2b(ctx, p16, a, b, z)["ms"] for _ in range(2))
-    # round 0 reads a, b (F) and eq (EF, 2^23); later rounds read 2 EF tables and eq
-    sc_bytes = (8 + 32) * (1 << 23) + sum((64 + 32) * (1 << (23 - j)) + 64 * (1 << (23 - j)) for j in range(1, 24))
+    sc_bytes = 2 * (2 * 4 << 24) + 2 * 32 * (1 << 23) + sum(2 * 32 * ((1 << (25 - j)) + (1 << (24 - j)))
+                                                             for j in range(2, 24))
     ctx.close()
     return {"fold_2^24_x24": {"ms": round(fold_ms, 4), "GB_per_s": round(fold_bytes / fold_ms / 1e6, 1),
                                "hbm_frac": round(fold_bytes / fold_ms / 1e6 / HBM_PEAK_GBS, 4)},

@@ -13,6 +13,7 @@
 // the sequential definition.
 #include "common.h"
 #include "kernels.h"
+#include <stdexcept>
 
 namespace wk {
 using namespace kb;
@@ -813,6 +814,86 @@ int a2b_round_fused(const F *a0, const F *b0, EF *A, EF *B, int l, bool first, E
     unsigned nb = dot_blocks(h);
     if (first) k_a2b_round_fused<true><<<nb, 256, 0, s>>>(a0, b0, A, B, h, c, eq, parts);
     else k_a2b_round_fused<false><<<nb, 256, 0, s>>>(a0, b0, A, B, h, c, eq, parts);
+    return (int)nb;
+}
+
+// The same rounds with the eq weight split: eq(z[j+1..], x) = ehi[x >> L] elo[x mod
+// 2^L], elo over the last L variables (read from cache, never halved) and ehi
+// over the others.  Block b owns the h / gridDim.x consecutive x from b h /
+// gridDim.x: a power of two no larger than 2^L, so one ehi entry, by which the
+// block's sums are multiplied once.  No eq table of the round's length is built,
+// read or halved.
+template <bool FIRST>
+__global__ void __launch_bounds__(256) k_a2b_round_fused_split(const F *a0, const F *b0, EF *A, EF *B, size_t h,
+                                                               int L, EF c, const EF *elo, const EF *ehi,
+                                                               EF *parts) {
+    EF acc[3] = {ezero(), ezero(), ezero()};
+    const size_t len = h / gridDim.x, x0 = (size_t)blockIdx.x * len, mask = ((size_t)1 << L) - 1;
+    for (size_t i = threadIdx.x; i < len; i += blockDim.x) {
+        const size_t x = x0 + i;
+        EF al, ah, bl, bh;
+        if (FIRST) {
+            auto fl = [&](const F *t, size_t k) { return eadd(efrom(t[k]), emulf(c, fsub(t[k + 2 * h], t[k]))); };
+            al = fl(a0, x); ah = fl(a0, x + h); bl = fl(b0, x); bh = fl(b0, x + h);
+        } else {
+            auto fl = [&](const EF *t, size_t k) { return eadd(t[k], emul(c, esub(t[k + 2 * h], t[k]))); };
+            al = fl(A, x); ah = fl(A, x + h); bl = fl(B, x); bh = fl(B, x + h);
+        }
+        A[x] = al; A[x + h] = ah; B[x] = bl; B[x + h] = bh;
+        const EF e = elo[x & mask];
+        const EF da = esub(ah, al), db = esub(bh, bl);
+        acc[0] = eadd(acc[0], emul(e, emul(esqr(al), bl)));
+        EF a2 = eadd(al, edbl(da)), b2 = eadd(bl, edbl(db));
+        acc[1] = eadd(acc[1], emul(e, emul(esqr(a2), b2)));
+        EF a3 = eadd(a2, da), b3 = eadd(b2, db);
+        acc[2] = eadd(acc[2], emul(e, emul(esqr(a3), b3)));
+    }
+    __shared__ EF lds[16];
+    const EF w = ehi[x0 >> L];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        EF t = block_sum(acc[i], lds);
+        if (threadIdx.x == 0) parts[(size_t)i * gridDim.x + blockIdx.x] = emul(w, t);
+    }
+}
+// round 0 of the split form on the base-field tables, z = 0..3 (gridDim.y == 4) or
+// z = 0, 2, 3
+__global__ void __launch_bounds__(256) k_a2b_round_split(const F *a, const F *b, size_t h, int L, const EF *elo,
+                                                         const EF *ehi, EF *parts) {
+    const int zi = blockIdx.y;
+    const int z = gridDim.y == 4 ? zi : (zi == 0 ? 0 : zi + 1);
+    const size_t len = h / gridDim.x, x0 = (size_t)blockIdx.x * len, mask = ((size_t)1 << L) - 1;
+    EF acc = ezero();
+    for (size_t i = threadIdx.x; i < len; i += blockDim.x) {
+        const size_t x = x0 + i;
+        F az = zfold<F>(a[x], a[x + h], z), bz = zfold<F>(b[x], b[x + h], z);
+        acc = eadd(acc, emulf(elo[x & mask], fmul(fmul(az, az), bz)));
+    }
+    __shared__ EF lds[16];
+    EF t = block_sum(acc, lds);
+    if (threadIdx.x == 0) parts[(size_t)zi * gridDim.x + blockIdx.x] = emul(ehi[x0 >> L], t);
+}
+// block counts as dot_blocks gives them; each block's range must fit inside one
+// ehi entry (h / blocks <= 2^L) and h > 2^L
+static unsigned split_blocks(size_t h, int L) {
+    const unsigned nb = (unsigned)dot_blocks(h);
+    if (h <= ((size_t)1 << L) || h % nb != 0 || h / nb > ((size_t)1 << L) || (h / nb) % 256 != 0)
+        throw std::runtime_error("a2b split round: table length does not fit the block layout");
+    return nb;
+}
+int a2b_round_f_split(const F *a, const F *b, int l, int L, const EF *elo, const EF *ehi, EF *parts, hipStream_t s,
+                      bool all4) {
+    const size_t h = (size_t)1 << (l - 1);
+    const unsigned nb = split_blocks(h, L);
+    k_a2b_round_split<<<dim3(nb, all4 ? 4 : 3), 256, 0, s>>>(a, b, h, L, elo, ehi, parts);
+    return (int)nb;
+}
+int a2b_round_fused_split(const F *a0, const F *b0, EF *A, EF *B, int l, int L, bool first, EF c, const EF *elo,
+                          const EF *ehi, EF *parts, hipStream_t s) {
+    const size_t h = (size_t)1 << (l - 1);
+    const unsigned nb = split_blocks(h, L);
+    if (first) k_a2b_round_fused_split<true><<<nb, 256, 0, s>>>(a0, b0, A, B, h, L, c, elo, ehi, parts);
+    else k_a2b_round_fused_split<false><<<nb, 256, 0, s>>>(a0, b0, A, B, h, L, c, elo, ehi, parts);
     return (int)nb;
 }
 

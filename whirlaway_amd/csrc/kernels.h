@@ -127,6 +127,12 @@ int a2b_round_f(const F *a, const F *b, int l, const EF *eq, EF *parts, hipStrea
 // place in A, B otherwise), eq of length 2^l halved in place; parts [3][block].
 int a2b_round_fused(const F *a0, const F *b0, EF *A, EF *B, int l, bool first, EF c, EF *eq, EF *parts,
                     hipStream_t s);
+// the two above with eq(z[j+1..], x) = ehi[x >> L] elo[x mod 2^L] (no round-length
+// eq table; eq is not halved); need 2^(l-1) > 2^L
+int a2b_round_f_split(const F *a, const F *b, int l, int L, const EF *elo, const EF *ehi, EF *parts, hipStream_t s,
+                      bool all4);
+int a2b_round_fused_split(const F *a0, const F *b0, EF *A, EF *B, int l, int L, bool first, EF c, const EF *elo,
+                          const EF *ehi, EF *parts, hipStream_t s);
 // fold an F table by (1-c, c) into an EF table of half size
 void fold_f_to_ef(const F *in, int l, const EF *c, EF *out, hipStream_t s);
 

@@ -141,8 +141,32 @@ int wl_sumcheck_a2b(wl_ctx *c, const uint32_t *p16, const uint32_t *a, const uin
         upload_canon(c, (uint32_t *)da.p, a, N);
         upload_canon(c, (uint32_t *)db.p, b, N);
         HIP_CHECK(hipEventRecord(c->ev0, c->stream));
-        // eq(z[1..], .) once; every later round halves it inside its kernel
-        DBuf<EF> eq = eq_dev(c, std::vector<EF>(zz.begin() + 1, zz.end()), eone());
+        // Round j weighs x by eq(z[j+1..], x).  While that has more than L + 1
+        // variables it is split as ehi_j[x >> L] elo[x mod 2^L]: elo = eq over the
+        // last L variables, ehi_j over z[j+1 .. n-1-L] (all ehi_j built at once, 2^(n-L)
+        // entries in all), so no table of the round's length is built or read.  From
+        // the round with L + 1 variables on, one eq table of 2^(L+1) entries that
+        // every later round halves inside its kernel.  L >= n - 11 keeps a block's
+        // range (at most 2^(n-1) / 1024 values) inside one ehi entry.
+        const int L = std::max(13, (int)n - 11);
+        const bool split = (int)n - 1 > L;
+        std::vector<DBuf<EF>> ehi;
+        DBuf<EF> elo, eq;
+        if (split) {
+            elo = DBuf<EF>(c->pool, (size_t)1 << L);
+            eq = DBuf<EF>(c->pool, (size_t)1 << (L + 1));
+            std::vector<EqJob> jobs;
+            jobs.push_back(EqJob{std::vector<EF>(zz.end() - L, zz.end()), eone(), elo.p});
+            jobs.push_back(EqJob{std::vector<EF>(zz.end() - (L + 1), zz.end()), eone(), eq.p});
+            for (int j = 0; j + 1 + L < (int)n; j++) {  // ehi_j over z[j+1 .. n-1-L]
+                ehi.emplace_back(c->pool, (size_t)1 << (n - 1 - L - j));
+                jobs.push_back(EqJob{std::vector<EF>(zz.begin() + j + 1, zz.end() - L), eone(), ehi.back().p});
+            }
+            eq_batch(c, jobs);
+        } else {
+            // eq(z[1..], .) once; every later round halves it inside its kernel
+            eq = eq_dev(c, std::vector<EF>(zz.begin() + 1, zz.end()), eone());
+        }
         // round 0 on the base-field tables at z = 0..3: z = 0, 1 give the claimed sum
         // (1 - z0) S(0) + z0 S(1), z = 0, 2, 3 the round's evaluations
         EF sum;
@@ -150,7 +174,8 @@ int wl_sumcheck_a2b(wl_ctx *c, const uint32_t *p16, const uint32_t *a, const uin
         {
             int nb = wk::dot_blocks(N / 2);
             DBuf<EF> parts(c->pool, 4 * (size_t)nb);
-            wk::a2b_round_f(da, db, (int)n, eq, parts, c->stream, true);
+            if (split) wk::a2b_round_f_split(da, db, (int)n, L, elo, ehi[0], parts, c->stream, true);
+            else wk::a2b_round_f(da, db, (int)n, eq, parts, c->stream, true);
             std::vector<EF> s4 = reduce_many(c, parts, 4, nb);
             sum = eadd(emul(esub(eone(), zz[0]), s4[0]), emul(zz[0], s4[1]));
             v3 = {s4[0], s4[2], s4[3]};
@@ -166,7 +191,11 @@ int wl_sumcheck_a2b(wl_ctx *c, const uint32_t *p16, const uint32_t *a, const uin
                 // fold by the previous challenge, halve eq, evaluate: one pass
                 int nb = wk::dot_blocks((size_t)1 << (l - 1));
                 DBuf<EF> parts(c->pool, 3 * (size_t)nb);
-                wk::a2b_round_fused(da, db, ea, eb, l, j == 1, ch.back(), eq, parts, c->stream);
+                if (split && l - 1 > L)
+                    wk::a2b_round_fused_split(da, db, ea, eb, l, L, j == 1, ch.back(), elo, ehi[j], parts,
+                                              c->stream);
+                else
+                    wk::a2b_round_fused(da, db, ea, eb, l, j == 1, ch.back(), eq, parts, c->stream);
                 v3 = reduce_many(c, parts, 3, nb);
             }
             EF e0 = v3[0], e2 = v3[1], e3 = v3[2];
