@@ -79,12 +79,13 @@ def test_fold_chain_per_round(gpu_ctx, oracle):
     assert np.array_equal(final, cur)
 
 
-@pytest.mark.parametrize("n", [4, 9, 14, 15, 16])
+@pytest.mark.parametrize("n", [4, 9, 14, 15, 16, 17])
 def test_sumcheck_a2b(gpu_ctx, oracle, n):
     """Kernel micro (b) shape at reduced size: transcript, challenges and final values.
     From n = 15 the first rounds take the eq weight split over a 2^13-entry table of
     the last variables and a table of the others (n = 15: round 0 split, round 1 the
-    whole table; n = 16: rounds 0-1 split, then the whole table)."""
+    whole table; n = 16: rounds 0-1 split, then the whole table; n = 17: rounds 0-2
+    split)."""
     from whirlaway_amd import sumcheck
     _say(f"sumcheck a2b n={n}")
     b = oracle.Bench(5, seed=11)
