@@ -60,10 +60,13 @@ def test_fold_large(gpu_ctx, oracle, log_n, k):
                           oracle.fold_ef_large(me, sc, k))
 
 
-def test_fold_chain_per_round(gpu_ctx, oracle):
-    """Kernel micro (a) shape at reduced size: every intermediate table bit-exact."""
+@pytest.mark.parametrize("log_n,folds", [(14, 14), (13, 9), (12, 5), (2, 2)])
+def test_fold_chain_per_round(gpu_ctx, oracle, log_n, folds):
+    """Kernel micro (a) shape at reduced size: every intermediate table bit-exact, and
+    the final table of the timed chain, whose folds of tables of at most 2^12 entries
+    run in one launch (ending at one entry, at 16 and at 128 entries, and a chain of
+    two folds)."""
     from whirlaway_amd import utils
-    log_n, folds = 14, 14
     _say(f"fold_chain log_n={log_n}")
     rng = np.random.default_rng(7)
     m = rng.integers(0, P, 1 << log_n, dtype=np.uint32)

@@ -686,6 +686,40 @@ void fold_ef(EF *A, int l, const EF *c, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
     k_fold_ef<<<nblk(h), 256, 0, s>>>(A, h, c);
 }
+// nf in-place folds of an EF table of 2^l <= 2^12 entries in one block: the first
+// reads the table, the rest work from LDS; every fold's result is also written to
+// A[0 .. len), the memory state a chain of fold_ef launches leaves.  Saves a launch
+// (and its dispatch gap) per fold where each fold is a few microseconds of latency.
+constexpr int FOLD_TAIL_LOG = 12;
+__global__ void __launch_bounds__(1024) k_fold_ef_tail(EF *A, int l, int nf, const EF *c) {
+    __shared__ EF buf[1 << (FOLD_TAIL_LOG - 1)];
+    int h = 1 << (l - 1);
+    for (int x = threadIdx.x; x < h; x += blockDim.x) {
+        const EF a = A[x], r = eadd(a, emul(c[0], esub(A[x + h], a)));
+        buf[x] = r;
+        A[x] = r;
+    }
+    __syncthreads();
+    for (int i = 1; i < nf; i++) {
+        h >>= 1;
+        const int x = threadIdx.x;
+        EF r;
+        if (x < h) {
+            const EF a = buf[x];
+            r = eadd(a, emul(c[i], esub(buf[x + h], a)));
+        }
+        __syncthreads();
+        if (x < h) {
+            buf[x] = r;
+            A[x] = r;
+        }
+        __syncthreads();
+    }
+}
+void fold_ef_tail(EF *A, int l, int nf, const EF *c, hipStream_t s) {
+    if (l < 1 || l > FOLD_TAIL_LOG || nf < 1 || nf > l) throw std::runtime_error("fold_ef_tail: bad table size");
+    k_fold_ef_tail<<<1, 1024, 0, s>>>(A, l, nf, c);
+}
 
 // -------------------------------------------------------------------- micro --
 __global__ void k_fold_f_large(const F *in, size_t M, const EF *sc, int K, EF *out) {

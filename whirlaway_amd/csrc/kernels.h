@@ -114,6 +114,8 @@ int prod_round_fused(EF *A, EF *B, int l, bool fold, const EF *c, EF *parts, hip
 void fold_ef2_p(EF *A, EF *B, int l, const EF *c, hipStream_t s);
 // in-place fold of one EF table by (1-c, c)
 void fold_ef(EF *A, int l, const EF *c, hipStream_t s);
+// nf folds in place of a table of 2^l <= 2^12 entries by c[0 .. nf) in one launch
+void fold_ef_tail(EF *A, int l, int nf, const EF *c, hipStream_t s);
 
 // -------------------------------------------------------------- micro --
 // fold an F table (2^l) by 2^k EF scalars: out[i] = sum_j s_j in[i + j 2^(l-k)]

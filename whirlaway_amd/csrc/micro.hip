@@ -101,7 +101,13 @@ int wl_fold_chain(wl_ctx *c, const uint32_t *in, uint32_t log_n, const uint32_t 
         DBuf<EF> cur(c->pool, N / 2);
         HIP_CHECK(hipEventRecord(c->ev0, c->stream));
         wk::fold_f_to_ef(din, (int)log_n, dch.p, cur, c->stream);
-        for (uint32_t i = 1; i < n_folds; i++) wk::fold_ef(cur, (int)(log_n - i), dch.p + i, c->stream);
+        for (uint32_t i = 1; i < n_folds; i++) {
+            if (log_n - i <= 12) {  // the rest in one launch
+                wk::fold_ef_tail(cur, (int)(log_n - i), (int)(n_folds - i), dch.p + i, c->stream);
+                break;
+            }
+            wk::fold_ef(cur, (int)(log_n - i), dch.p + i, c->stream);
+        }
         HIP_CHECK(hipEventRecord(c->ev1, c->stream));
         HIP_CHECK(hipEventSynchronize(c->ev1));
         float t = 0;
