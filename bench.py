@@ -157,15 +157,14 @@ def _replay(eng, reqs, arrivals):
 def run_serving(args):
     """Poisson arrivals of proof requests (log_n ~ U{10..18}) replayed open-loop against
     the engine (several slots, continuous admission) and against the plain path (one
-    slot: the single-proof path behind a FIFO).  Latency = submit to proof ready."""
+    slot: the single-proof path behind a FIFO), at each arrival rate of the sweep; the
+    last rate is above what the plain path sustains.  Latency = submit to proof ready."""
     import numpy as np
     from whirlaway_amd.engine import Engine
     reqs = _requests(args.seed + 1, args.serve_requests, 10, 18)
-    rng = np.random.default_rng(args.seed + 2)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, len(reqs)))
-    arrivals -= arrivals[0]
-    out = {}
-    proofs = {}
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    points = []
+    same = True
     for name, slots in (("plain", 1), ("engine", args.slots)):
         with Engine(0, n_slots=slots) as eng:
             # warm-up: every slot proves each size once (pools, twiddle tables)
@@ -176,18 +175,28 @@ def run_serving(args):
                 ts = [eng.submit(n, *job) for n, job in sorted(by_n.items())]
                 for t in ts:
                     eng.wait(t)
-            res, wall = _replay(eng, reqs, arrivals)
-        lat = np.array([r[1]["latency_ms"] for r in res])
-        proofs[name] = [r[0] for r in res]
-        out[name] = {"proofs_per_s": round(len(reqs) / wall, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                     "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                     "mean_queue_ms": round(float(np.mean([r[1]["queued_ms"] for r in res])), 3)}
-    same = all(np.array_equal(a, b) for a, b in zip(proofs["plain"], proofs["engine"]))
-    e = out["engine"]
-    return {"requests": len(reqs), "rate_per_s": args.serve_rate, "log_n": "U{10..18}",
-            "slots": args.slots, "proofs_per_s": e["proofs_per_s"], "p50_ms": e["p50_ms"],
-            "p99_ms": e["p99_ms"], "mean_queue_ms": e["mean_queue_ms"], "plain": out["plain"],
-            "identical_to_plain": bool(same)}
+            for i, rate in enumerate(rates):
+                rng = np.random.default_rng(args.seed + 2 + i)
+                arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+                arrivals -= arrivals[0]
+                res, wall = _replay(eng, reqs, arrivals)
+                lat = np.array([r[1]["latency_ms"] for r in res])
+                if len(points) <= i:
+                    points.append({"rate_per_s": rate})
+                points[i][name] = {"proofs_per_s": round(len(reqs) / wall, 1),
+                                   "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                                   "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                                   "mean_queue_ms": round(float(np.mean([r[1]["queued_ms"] for r in res])), 3)}
+                if name == "plain":
+                    points[i]["_proofs"] = [r[0] for r in res]
+                else:
+                    same = same and all(np.array_equal(a, r[0]) for a, r in zip(points[i].pop("_proofs"), res))
+    sat = points[-1]
+    e = sat["engine"]
+    return {"requests": len(reqs), "log_n": "U{10..18}", "slots": args.slots,
+            "rate_per_s": sat["rate_per_s"], "proofs_per_s": e["proofs_per_s"], "p50_ms": e["p50_ms"],
+            "p99_ms": e["p99_ms"], "mean_queue_ms": e["mean_queue_ms"], "plain": sat["plain"],
+            "sweep": points, "identical_to_plain": bool(same)}
 
 
 def main():
@@ -203,7 +212,9 @@ def main():
                     help="extra proofs timed from a host-memory witness (PCIe-inclusive rate)")
     ap.add_argument("--batch-jobs", type=int, default=64, help="batched leg: proofs (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=256, help="serving leg: requests (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=120.0, help="serving leg: Poisson arrivals per second")
+    ap.add_argument("--serve-rates", default="120,400",
+                    help="serving leg: Poisson arrival rates per second, comma-separated; the last one "
+                         "saturates the plain path and gives the headline serving figures")
     ap.add_argument("--slots", type=int, default=8, help="engine slots (concurrent proofs)")
     ap.add_argument("--no-micro", action="store_true", help="skip the kernel micro leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -295,6 +306,7 @@ def main():
     }
 
     cpu = None
+    identical = None  # GPU proof == oracle proof of the same workload (None: not compared)
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py  # the checker, timed here only as the reported CPU baseline
@@ -307,8 +319,8 @@ def main():
                "kind": "port",
                "sample": f"one oracle proof of 2^{args.cpu_log_n} permutations (same seed and "
                          f"settings), {tcpu:.2f} s wall, OpenMP"}
-        if args.cpu_log_n == n and not (cpu_proof.size == proof.size and (cpu_proof == proof).all()):
-            cpu["note"] = "CPU and GPU proofs differ"
+        if args.cpu_log_n == n:
+            identical = bool(cpu_proof.size == proof.size and (cpu_proof == proof).all())
         b.close()
 
     micro = None if args.no_micro else run_micro(args)
@@ -329,9 +341,10 @@ def main():
         "dtype": "u32 (KoalaBear F_p, Montgomery) / EF8",
         "data": "synthetic (StdRng seed 0 inputs, GPU-generated Poseidon2 trace)",
         "config": {"workload": f"Poseidon2-16 KoalaBear AIR, 2^{n} permutations, one proof per step",
-                   "log_n_rows": n, "security_bits": 128, "folding": [7, 4], "log_inv_rate": 1,
-                   "univariate_skips": 4, "rs_domain_initial_reduction": 5,
+                   "log_n_rows": n, "security_bits": 128, "soundness": "CapacityBound", "folding": [7, 4],
+                   "log_inv_rate": 1, "univariate_skips": 4, "rs_domain_initial_reduction": 5,
                    "proof_words": int(proof.size)},
+        "identical_to_oracle": identical,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "phase_ms_per_step": dict(zip(PHASES, [round(x / args.steps, 4) for x in phases])),
@@ -344,6 +357,12 @@ def main():
     table.close()
     ctx.close()
     print(json.dumps(line))
+    mismatch = [k for k, v in (("oracle", identical), ("pcie", pcie and pcie["same_proof"]),
+                               ("batched", batched and batched["identical_to_plain"]),
+                               ("serving", serving and serving["identical_to_plain"])) if v is False]
+    if mismatch:
+        print(f"bench.py: proofs differ ({', '.join(mismatch)})", file=sys.stderr)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -28,9 +28,12 @@ typedef struct wl_ctx wl_ctx;
 typedef struct wl_table wl_table;
 typedef struct wl_witness wl_witness;
 
+/* SecurityAssumption of whir-p3 (crates/air/src/lib.rs:3,15), in its order. */
+enum { WL_UNIQUE_DECODING = 0, WL_JOHNSON_BOUND = 1, WL_CAPACITY_BOUND = 2 };
+
 /* AirSettings (crates/air/src/lib.rs:13-41) with
  * FoldingFactor::ConstantFromSecondRound(folding_factor_first, folding_factor)
- * and SecurityAssumption::CapacityBound (src/main.rs:16-23). */
+ * (src/main.rs:16-23 passes 128, CapacityBound, (7, 4), 1, 4, 5). */
 typedef struct {
     uint32_t security_bits;               /* 128 */
     uint32_t folding_factor_first;        /* 7 */
@@ -38,10 +41,19 @@ typedef struct {
     uint32_t log_inv_rate;                /* 1 */
     uint32_t univariate_skips;            /* 4 (must equal the table's) */
     uint32_t rs_domain_initial_reduction; /* 5 */
+    uint32_t soundness;                   /* whir_soudness_type: WL_CAPACITY_BOUND */
 } wl_air_settings;
 
 /* Message of the last failed call on this thread. */
 const char *wl_last_error(void);
+
+/* Checks settings against a table of 2^log_length rows whose committed
+ * polynomial has n_witness_columns columns, without a device: the WHIR schedule
+ * of AirTable::build_whir_params (crates/air/src/table.rs:72-98) must exist and
+ * every codeword domain must fit the field's 2^24 two-adic subgroup (the
+ * reference panics inside WhirConfig::new / the DFT instead).  WL_OK or
+ * WL_ERR_ARG with the reason in wl_last_error(). */
+int wl_air_settings_validate(uint32_t log_length, uint32_t n_witness_columns, const wl_air_settings *settings);
 
 /* Number of visible HIP devices (0 when none). */
 int wl_device_count(void);
@@ -127,8 +139,8 @@ void wl_mem_stats(wl_ctx *ctx, uint64_t *held, uint64_t *peak_live, int reset_pe
  * these entry points have no counterpart there.  An engine runs independent
  * proof jobs concurrently on one GPU: n_slots worker threads, each with its own
  * context (stream + pool).  Jobs are admitted in FIFO order when a slot is free
- * and wl_job_footprint(log_n) fits the memory budget beside the running jobs'
- * footprints (budget 0: 80% of the device's free memory).  Each job runs the
+ * and wl_job_footprint(log_n, settings) fits the memory budget beside the
+ * running jobs' footprints (budget 0: 80% of the device's free memory).  Each job runs the
  * single-proof path (wl_air_table_new, wl_air_generate_witness,
  * wl_air_prove_resident), so its proof equals the one proven alone.  Pools keep
  * freed blocks for reuse, up to about one footprint per slot, outside the budget. */
@@ -142,9 +154,14 @@ typedef struct {
     wl_air_settings settings;
 } wl_job;                      /* arrays must stay valid until the job has completed */
 
-uint64_t wl_job_footprint(uint32_t log_n);
+/* Device bytes reserved for a job of 2^log_n rows under its settings (NULL: the
+ * defaults of src/main.rs): witness, first codeword with its encoding scratch and
+ * Merkle tree, zerocheck, inner-sumcheck and WHIR-round buffers, plus a margin. */
+uint64_t wl_job_footprint(uint32_t log_n, const wl_air_settings *settings);
 wl_engine *wl_engine_create(int device, int n_slots, uint64_t mem_budget_bytes);
-/* Drops queued jobs, waits for running ones, frees the slots' contexts. */
+/* Completes queued jobs with an "engine stopped" error (their waiters return it),
+ * waits for running jobs and for every thread inside wl_engine_wait to leave,
+ * then frees the slots' contexts. */
 void wl_engine_destroy(wl_engine *e);
 int wl_engine_submit(wl_engine *e, const wl_job *job, uint64_t *ticket);
 /* 1 done, 0 pending, -1 unknown ticket */

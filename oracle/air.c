@@ -47,7 +47,7 @@ void or_p2_bench_free(or_p2_bench_t *b) {
 int or_air_whir_config(or_whir_cfg_t *c, const or_air_settings_t *s, int log_n) {
     int v = log_n + (int)or_log2_up(NC);
     return or_whir_config(c, v, s->security_bits, WHIR_POW_BITS, s->folding_f0, s->folding_f,
-                          s->log_inv_rate, s->rs_reduction0);
+                          s->log_inv_rate, s->rs_reduction0, s->soundness);
 }
 
 /* crates/air/src/uni_skip_utils.rs:5-13 */

@@ -41,7 +41,8 @@ void or_api_compress(void *h, const f_t *l, const f_t *r, f_t *out) {
     or_compress(&((or_p2_bench_t *)h)->p16, l, r, out);
 }
 
-static void settings(or_air_settings_t *s, int sec, int f0, int f, int rate, int skips, int rs0) {
+static void settings(or_air_settings_t *s, int sec, int f0, int f, int rate, int skips, int rs0, int sa) {
+    s->soundness = sa;
     s->security_bits = sec;
     s->folding_f0 = f0;
     s->folding_f = f;
@@ -50,15 +51,15 @@ static void settings(or_air_settings_t *s, int sec, int f0, int f, int rate, int
     s->rs_reduction0 = rs0;
 }
 
-size_t or_api_prove(void *h, int sec, int f0, int f, int rate, int skips, int rs0, f_t **out) {
+size_t or_api_prove(void *h, int sec, int f0, int f, int rate, int skips, int rs0, int sa, f_t **out) {
     or_air_settings_t s;
-    settings(&s, sec, f0, f, rate, skips, rs0);
+    settings(&s, sec, f0, f, rate, skips, rs0, sa);
     return or_air_prove(&s, (or_p2_bench_t *)h, out);
 }
-int or_api_verify(void *h, int sec, int f0, int f, int rate, int skips, int rs0, const f_t *proof,
+int or_api_verify(void *h, int sec, int f0, int f, int rate, int skips, int rs0, int sa, const f_t *proof,
                   size_t len) {
     or_air_settings_t s;
-    settings(&s, sec, f0, f, rate, skips, rs0);
+    settings(&s, sec, f0, f, rate, skips, rs0, sa);
     return or_air_verify(&s, (or_p2_bench_t *)h, proof, len);
 }
 void or_api_free(void *p) { free(p); }
@@ -66,9 +67,9 @@ void or_api_free(void *p) { free(p); }
 /* WHIR schedule as a flat int array: [n_rounds, final_sumcheck_rounds,
  * commit_ood, starting_folding_pow, final_queries, final_pow, final_folding_pow,
  * final_log_domain, then per round 8 ints] */
-int or_api_whir_config(int v, int sec, int max_pow, int f0, int f, int rate, int rs0, int *out) {
+int or_api_whir_config(int v, int sec, int max_pow, int f0, int f, int rate, int rs0, int sa, int *out) {
     or_whir_cfg_t c;
-    if (or_whir_config(&c, v, sec, max_pow, f0, f, rate, rs0)) return -1;
+    if (or_whir_config(&c, v, sec, max_pow, f0, f, rate, rs0, sa)) return -1;
     int k = 0;
     out[k++] = c.n_rounds; out[k++] = c.final_sumcheck_rounds; out[k++] = c.commit_ood_samples;
     out[k++] = c.starting_folding_pow_bits; out[k++] = c.final_queries; out[k++] = c.final_pow_bits;

@@ -10,6 +10,7 @@
  * inverse, irreducibility of X^8 - 3 (sympy), the two-adic generator table
  * of order 2^24.
  */
+#include <stdlib.h>
 #include "oracle.h"
 
 f_t f_pow(f_t a, uint64_t e) {
@@ -34,6 +35,8 @@ f_t f_from_monty(uint32_t m) {
 /* Plonky3 KoalaBear: multiplicative generator 3, TWO_ADICITY 24; the 2^bits
  * generator is 3^((p-1)/2^bits). */
 f_t f_two_adic_generator(int bits) {
+    /* Plonky3 asserts bits <= TWO_ADICITY: no larger power-of-two subgroup exists */
+    if (bits < 0 || bits > 24) abort();
     f_t g = f_pow(3, (OR_P - 1) >> 24);
     for (int i = bits; i < 24; i++) g = f_mul(g, g);
     return g;

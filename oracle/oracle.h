@@ -198,8 +198,10 @@ typedef struct {
     int num_variables, folding_factor, log_domain_size;
 } or_whir_round_t;
 
+/* SecurityAssumption (whir-p3 parameters/errors.rs), in the reference's order */
+enum { OR_SA_UNIQUE_DECODING = 0, OR_SA_JOHNSON_BOUND = 1, OR_SA_CAPACITY_BOUND = 2 };
 typedef struct {
-    int num_variables, security_level, max_pow_bits;
+    int num_variables, security_level, max_pow_bits, soundness;
     int f0, f;                     /* FoldingFactor::ConstantFromSecondRound(f0, f) */
     int starting_log_inv_rate, rs_reduction0;
     int n_rounds, final_sumcheck_rounds;
@@ -210,7 +212,7 @@ typedef struct {
 } or_whir_cfg_t;
 
 int or_whir_config(or_whir_cfg_t *c, int num_variables, int security_level, int max_pow_bits,
-                   int f0, int f, int log_inv_rate, int rs_reduction0);
+                   int f0, int f, int log_inv_rate, int rs_reduction0, int soundness);
 int or_whir_folding_at(const or_whir_cfg_t *c, int round);
 size_t or_whir_domsep(const or_whir_cfg_t *c, f_t *out, size_t cap);
 
@@ -254,6 +256,7 @@ typedef struct {
     int log_inv_rate;
     int univariate_skips;
     int rs_reduction0;
+    int soundness;  /* OR_SA_* */
 } or_air_settings_t;
 
 typedef struct {

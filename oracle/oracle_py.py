@@ -44,12 +44,12 @@ def lib():
         L.or_api_hash_leaf.argtypes = [vp, u32p, ctypes.c_size_t, u32p]
         L.or_api_compress.argtypes = [vp, u32p, u32p, u32p]
         L.or_api_prove.restype = ctypes.c_size_t
-        L.or_api_prove.argtypes = [vp] + [ctypes.c_int] * 6 + [ctypes.POINTER(u32p)]
+        L.or_api_prove.argtypes = [vp] + [ctypes.c_int] * 7 + [ctypes.POINTER(u32p)]
         L.or_api_verify.restype = ctypes.c_int
-        L.or_api_verify.argtypes = [vp] + [ctypes.c_int] * 6 + [u32p, ctypes.c_size_t]
+        L.or_api_verify.argtypes = [vp] + [ctypes.c_int] * 7 + [u32p, ctypes.c_size_t]
         L.or_api_free.argtypes = [vp]
         L.or_api_whir_config.restype = ctypes.c_int
-        L.or_api_whir_config.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_int)]
+        L.or_api_whir_config.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(ctypes.c_int)]
         L.or_api_ef_mul.argtypes = [u32p, u32p, u32p]
         L.or_api_ef_inv.argtypes = [u32p, u32p]
         L.or_api_two_adic_generator.restype = ctypes.c_uint32
@@ -85,7 +85,9 @@ def ptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
 
 
-DEFAULT = dict(sec=128, f0=7, f=4, rate=1, skips=4, rs0=5)
+# SecurityAssumption codes in the reference's order
+UNIQUE_DECODING, JOHNSON_BOUND, CAPACITY_BOUND = 0, 1, 2
+DEFAULT = dict(sec=128, f0=7, f=4, rate=1, skips=4, rs0=5, sa=CAPACITY_BOUND)
 
 
 class Bench:
@@ -132,7 +134,7 @@ class Bench:
         s = dict(DEFAULT, **kw)
         p = ctypes.POINTER(ctypes.c_uint32)()
         n = lib().or_api_prove(self.h, s["sec"], s["f0"], s["f"], s["rate"], s["skips"], s["rs0"],
-                               ctypes.byref(p))
+                               s["sa"], ctypes.byref(p))
         arr = np.ctypeslib.as_array(p, shape=(n,)).copy()
         lib().or_api_free(p)
         return arr
@@ -141,7 +143,7 @@ class Bench:
         s = dict(DEFAULT, **kw)
         pr = np.ascontiguousarray(proof, dtype=np.uint32)
         return lib().or_api_verify(self.h, s["sec"], s["f0"], s["f"], s["rate"], s["skips"],
-                                   s["rs0"], ptr(pr), pr.size)
+                                   s["rs0"], s["sa"], ptr(pr), pr.size)
 
     def sc_a2b(self, a, b, z):
         n = a.size.bit_length() - 1
@@ -157,9 +159,11 @@ class Bench:
                 "final_sum": fs, "proof": proof}
 
 
-def whir_config(v, sec=128, max_pow=16, f0=7, f=4, rate=1, rs0=5):
+def whir_config(v, sec=128, max_pow=16, f0=7, f=4, rate=1, rs0=5, sa=CAPACITY_BOUND):
     out = (ctypes.c_int * 256)()
-    k = lib().or_api_whir_config(v, sec, max_pow, f0, f, rate, rs0, out)
+    k = lib().or_api_whir_config(v, sec, max_pow, f0, f, rate, rs0, sa, out)
+    if k < 0:
+        raise ValueError("no WHIR configuration for these parameters")
     vals = list(out[:k])
     head = dict(zip(["n_rounds", "final_sumcheck_rounds", "commit_ood", "starting_folding_pow",
                      "final_queries", "final_pow", "final_folding_pow", "final_log_domain"],

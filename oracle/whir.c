@@ -34,52 +34,78 @@
 #include "oracle.h"
 
 /* ------------------------------------------------------------ config -- */
+/* SecurityAssumption (whir-p3 @ 9caef91 parameters/errors.rs, restated from its
+ * published form; parity unpinned): the error terms of the WHIR round-by-round
+ * soundness analysis for the three decoding regimes.  eta is the proximity slack:
+ * JohnsonBound eta = sqrt(rho)/20 (log2 = -(r/2 + log2(10) + 1)), CapacityBound
+ * eta = rho/2 (log2 = -(r + 1)), none in unique decoding. */
 static const int FIELD_BITS = 248;
 static const int MAX_VARS_SEND_COEFFS = 6;
+static const double LOG2_10 = 3.321928094887362;
 
-static double cb_log_eta(int log_inv_rate) { return -((double)log_inv_rate + 1.0); }
-static double cb_list_size_bits(int log_degree, int log_inv_rate) {
-    return (double)(log_degree + log_inv_rate) - cb_log_eta(log_inv_rate);
+static double sa_log_eta(int sa, int r) {
+    if (sa == OR_SA_UNIQUE_DECODING) return 0.0;
+    if (sa == OR_SA_JOHNSON_BOUND) return -(0.5 * r + LOG2_10 + 1.0);
+    return -((double)r + 1.0);
 }
-static double cb_prox_gaps_error(int log_degree, int log_inv_rate, int num_functions) {
-    double err = (double)(log_degree + 2 * log_inv_rate) - cb_log_eta(log_inv_rate);
+static double sa_list_size_bits(int sa, int log_degree, int r) {
+    double le = sa_log_eta(sa, r);
+    if (sa == OR_SA_UNIQUE_DECODING) return 0.0;
+    if (sa == OR_SA_JOHNSON_BOUND) return 0.5 * r - (1.0 + le);
+    return (double)(log_degree + r) - le;
+}
+static double sa_prox_gaps_error(int sa, int log_degree, int r, int num_functions) {
+    double le = sa_log_eta(sa, r), err;
+    if (sa == OR_SA_UNIQUE_DECODING) err = (double)(log_degree + r);
+    else if (sa == OR_SA_JOHNSON_BOUND) {
+        double sqrt_rho_20 = 1.0 + LOG2_10 + 0.5 * r;
+        err = 2.0 * log_degree + 7.0 * ((sqrt_rho_20 < -le ? sqrt_rho_20 : -le) - 1.0);
+    } else err = (double)(log_degree + 2 * r) - le;
     return (double)FIELD_BITS - (err + log2((double)num_functions - 1.0));
 }
-static double cb_ood_error(int log_degree, int log_inv_rate, int samples) {
-    double lsb = cb_list_size_bits(log_degree, log_inv_rate);
+static double sa_ood_error(int sa, int log_degree, int r, int samples) {
+    double lsb = sa_list_size_bits(sa, log_degree, r);
     double err = 2.0 * lsb + (double)(log_degree * samples);
     return (double)(samples * FIELD_BITS) + 1.0 - err;
 }
-static int cb_ood_samples(int sec, int log_degree, int log_inv_rate) {
+static int sa_ood_samples(int sa, int sec, int log_degree, int r) {
+    if (sa == OR_SA_UNIQUE_DECODING) return 0;
     for (int s = 1; s < 64; s++)
-        if (cb_ood_error(log_degree, log_inv_rate, s) >= (double)sec) return s;
+        if (sa_ood_error(sa, log_degree, r, s) >= (double)sec) return s;
     return -1;
 }
-static int cb_queries(int protocol_sec, int log_inv_rate) {
-    return (int)ceil((double)protocol_sec / (double)log_inv_rate);
+/* -log2 of the per-query survival probability (1 - delta) */
+static double sa_query_bits(int sa, int r) {
+    if (sa == OR_SA_UNIQUE_DECODING) return -log2(0.5 * (1.0 + 1.0 / (double)(1 << r)));
+    if (sa == OR_SA_JOHNSON_BOUND) return 0.5 * r;
+    return (double)r;
 }
-static double cb_queries_error(int log_inv_rate, int q) { return (double)q * log_inv_rate; }
-static double fold_sumcheck_error(int nv, int log_inv_rate) {
-    return (double)FIELD_BITS - (cb_list_size_bits(nv, log_inv_rate) + 1.0);
+static int sa_queries(int sa, int protocol_sec, int r) {
+    return (int)ceil((double)protocol_sec / sa_query_bits(sa, r));
 }
-static int folding_pow_bits(int sec, int nv, int log_inv_rate) {
-    double e1 = cb_prox_gaps_error(nv, log_inv_rate, 2);
-    double e2 = fold_sumcheck_error(nv, log_inv_rate);
+static double sa_queries_error(int sa, int r, int q) { return (double)q * sa_query_bits(sa, r); }
+static double fold_sumcheck_error(int sa, int nv, int r) {
+    return (double)FIELD_BITS - (sa_list_size_bits(sa, nv, r) + 1.0);
+}
+static int folding_pow_bits(int sa, int sec, int nv, int r) {
+    double e1 = sa_prox_gaps_error(sa, nv, r, 2);
+    double e2 = fold_sumcheck_error(sa, nv, r);
     double e = e1 < e2 ? e1 : e2;
-    double r = (double)sec - e;
-    return r > 0 ? (int)ceil(r) : 0;
+    double x = (double)sec - e;
+    return x > 0 ? (int)ceil(x) : 0;
 }
-static double queries_combination_error(int nv, int log_inv_rate, int ood, int q) {
-    double lsb = cb_list_size_bits(nv, log_inv_rate);
+static double queries_combination_error(int sa, int nv, int r, int ood, int q) {
+    double lsb = sa_list_size_bits(sa, nv, r);
     return (double)FIELD_BITS - (log2((double)(ood + q)) + lsb + 1.0);
 }
 
 int or_whir_folding_at(const or_whir_cfg_t *c, int round) { return round == 0 ? c->f0 : c->f; }
 
 int or_whir_config(or_whir_cfg_t *c, int v, int sec, int max_pow, int f0, int f, int rate,
-                   int rs0) {
+                   int rs0, int sa) {
     memset(c, 0, sizeof *c);
-    if (f0 > v || f <= 0 || rs0 > f0) return -1;
+    if (f0 > v || f <= 0 || rs0 > f0 || sa < 0 || sa > 2) return -1;
+    c->soundness = sa;
     c->num_variables = v;
     c->security_level = sec;
     c->max_pow_bits = max_pow;
@@ -98,21 +124,23 @@ int or_whir_config(or_whir_cfg_t *c, int v, int sec, int max_pow, int f0, int f,
     if (c->n_rounds > OR_WHIR_MAX_ROUNDS) return -1;
     int psec = sec - max_pow;
     if (psec < 0) psec = 0;
-    c->commit_ood_samples = cb_ood_samples(sec, v, rate);
-    c->starting_folding_pow_bits = folding_pow_bits(sec, v, rate);
+    c->commit_ood_samples = sa_ood_samples(sa, sec, v, rate);
+    c->starting_folding_pow_bits = folding_pow_bits(sa, sec, v, rate);
+    if (c->commit_ood_samples < 0) return -1;
     int nv = v - f0, lir = rate, logd = v + rate;
     for (int r = 0; r < c->n_rounds; r++) {
         int rs = r == 0 ? rs0 : 1;
         int fold = or_whir_folding_at(c, r);
         int next_rate = lir + (fold - rs);
-        int q = cb_queries(psec, lir);
-        int ood = cb_ood_samples(sec, nv, next_rate);
-        double qe = cb_queries_error(lir, q);
-        double ce = queries_combination_error(nv, next_rate, ood, q);
+        int q = sa_queries(sa, psec, lir);
+        int ood = sa_ood_samples(sa, sec, nv, next_rate);
+        if (ood < 0) return -1;
+        double qe = sa_queries_error(sa, lir, q);
+        double ce = queries_combination_error(sa, nv, next_rate, ood, q);
         double pw = (double)sec - (qe < ce ? qe : ce);
         or_whir_round_t *rp = &c->rounds[r];
         rp->pow_bits = pw > 0 ? (int)ceil(pw) : 0;
-        rp->folding_pow_bits = folding_pow_bits(sec, nv, next_rate);
+        rp->folding_pow_bits = folding_pow_bits(sa, sec, nv, next_rate);
         rp->num_queries = q;
         rp->ood_samples = ood;
         rp->log_inv_rate = lir;
@@ -123,13 +151,19 @@ int or_whir_config(or_whir_cfg_t *c, int v, int sec, int max_pow, int f0, int f,
         lir = next_rate;
         logd -= rs;
     }
-    c->final_queries = cb_queries(psec, lir);
-    double fp = (double)sec - cb_queries_error(lir, c->final_queries);
+    c->final_queries = sa_queries(sa, psec, lir);
+    double fp = (double)sec - sa_queries_error(sa, lir, c->final_queries);
     c->final_pow_bits = fp > 0 ? (int)ceil(fp) : 0;
     double ffp = (double)sec - (double)(FIELD_BITS - 1);
     c->final_folding_pow_bits = ffp > 0 ? (int)ceil(ffp) : 0;
     c->final_log_inv_rate = lir;
     c->final_log_domain_size = logd;
+    /* every committed codeword lives on a power-of-two subgroup of F*: at most 2^24 */
+    if (v + rate - f0 > 24) return -1;
+    for (int r = 0; r < c->n_rounds; r++) {
+        int rs = r == 0 ? rs0 : 1;
+        if (c->rounds[r].log_domain_size - rs - or_whir_folding_at(c, r + 1) > 24) return -1;
+    }
     return 0;
 }
 

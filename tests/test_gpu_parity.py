@@ -185,9 +185,10 @@ def test_proof_settings_variants(gpu_ctx, oracle, n, kw):
     assert b.verify(got, **kw) == 0
 
 
-def test_proof_benchmark_size_verifies(gpu_ctx, oracle):
-    """n = 17 (the benchmark workload): the oracle verifier accepts the GPU proof,
-    and a resident-witness proof is byte-identical to the host-witness proof."""
+def test_proof_benchmark_size_identical(gpu_ctx, oracle):
+    """n = 17 (the benchmark workload): the GPU proof is byte-identical to the oracle's
+    proof of the same inputs, the oracle verifier accepts it and rejects a changed
+    word, and a resident-witness proof equals the host-witness proof."""
     import whirlaway_amd as wa
     from whirlaway_amd.air import Witness
     n = 17
@@ -196,6 +197,11 @@ def test_proof_benchmark_size_verifies(gpu_ctx, oracle):
     table = wa.AirTable(gpu_ctx, b.air_consts(), n)
     tr = b.trace()
     got = table.prove(wa.AirSettings(), b.p2_consts(16), b.p2_consts(24), tr)
+    _say("  oracle proof n=17")
+    want = b.prove()
+    assert got.size == want.size
+    diff = np.nonzero(got != want)[0]
+    assert diff.size == 0, f"first mismatch at {diff[:5]}"
     assert b.verify(got) == 0
     w = Witness(gpu_ctx, tr, n)
     again = table.prove(wa.AirSettings(), b.p2_consts(16), b.p2_consts(24), w)
@@ -205,6 +211,36 @@ def test_proof_benchmark_size_verifies(gpu_ctx, oracle):
     bad = got.copy()
     bad[len(bad) // 2] = (int(bad[len(bad) // 2]) + 1) % P
     assert b.verify(bad) != 0
+
+
+@pytest.mark.parametrize("sa", [0, 1])
+def test_proof_soundness_variants(gpu_ctx, oracle, sa):
+    """UniqueDecoding (no OOD samples, more queries) and JohnsonBound schedules:
+    byte-identical to the oracle, which verifies them."""
+    import whirlaway_amd as wa
+    n = 10
+    _say(f"full proof n={n} soundness {sa}")
+    b = oracle.Bench(n, seed=5)
+    want = b.prove(sa=sa)
+    table = wa.AirTable(gpu_ctx, b.air_consts(), n)
+    got = table.prove(wa.AirSettings(soundness=sa), b.p2_consts(16), b.p2_consts(24), b.trace())
+    table.close()
+    assert np.array_equal(got, want)
+    assert b.verify(got, sa=sa) == 0
+
+
+def test_settings_beyond_two_adicity_rejected(gpu_ctx):
+    """A shape whose first codeword would need a 2^25 domain is refused before any
+    device work (rate 1/8, first folding 4, n = 18)."""
+    import whirlaway_amd as wa
+    n = 18
+    rc = np.zeros(148, dtype=np.uint32)
+    table = wa.AirTable(gpu_ctx, rc, n, univariate_skips=4)
+    st = wa.AirSettings(log_inv_rate=3, folding_factor_first=4, folding_factor=2, rs_domain_initial_reduction=2,
+                        security_bits=90)
+    with pytest.raises(wa.WhirlawayError, match="two-adicity"):
+        table.prove(st, np.zeros(148, np.uint32), np.zeros(215, np.uint32), np.zeros(165 << n, np.uint32))
+    table.close()
 
 
 @pytest.mark.parametrize("n", [5, 11, 17])

@@ -65,7 +65,8 @@ def lib():
         L.wl_mem_stats.argtypes = [vp, u64p, u64p, ctypes.c_int]
         L.wl_fmul_rate.argtypes = [vp, dp]
         L.wl_job_footprint.restype = u64
-        L.wl_job_footprint.argtypes = [ctypes.c_uint32]
+        L.wl_job_footprint.argtypes = [ctypes.c_uint32, vp]
+        L.wl_air_settings_validate.argtypes = [ctypes.c_uint32, ctypes.c_uint32, vp]
         L.wl_engine_create.restype = vp
         L.wl_engine_create.argtypes = [ctypes.c_int, ctypes.c_int, u64]
         L.wl_engine_destroy.argtypes = [vp]

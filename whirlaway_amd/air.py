@@ -14,22 +14,34 @@ from . import check, lib, u32ptr, WhirlawayError
 
 NCOLS = 165  # Poseidon2Air<16, 3, 0, 4, 20> width
 
+# SecurityAssumption (whir-p3), in the reference's order
+UNIQUE_DECODING, JOHNSON_BOUND, CAPACITY_BOUND = 0, 1, 2
+
 
 @dataclass
 class AirSettings:
-    """AirSettings::new(security_bits, CapacityBound, ConstantFromSecondRound(f0, f),
-    log_inv_rate, univariate_skips, initial_domain_reduction) -- src/main.rs:16-23."""
+    """AirSettings::new(security_bits, soundness, ConstantFromSecondRound(f0, f),
+    log_inv_rate, univariate_skips, initial_domain_reduction) -- crates/air/src/lib.rs:13-41,
+    defaults from src/main.rs:16-23."""
     security_bits: int = 128
     folding_factor_first: int = 7
     folding_factor: int = 4
     log_inv_rate: int = 1
     univariate_skips: int = 4
     rs_domain_initial_reduction: int = 5
+    soundness: int = CAPACITY_BOUND
+
+    def fields(self):
+        return (self.security_bits, self.folding_factor_first, self.folding_factor, self.log_inv_rate,
+                self.univariate_skips, self.rs_domain_initial_reduction, self.soundness)
 
     def c_struct(self):
-        return (ctypes.c_uint32 * 6)(self.security_bits, self.folding_factor_first,
-                                      self.folding_factor, self.log_inv_rate,
-                                      self.univariate_skips, self.rs_domain_initial_reduction)
+        return (ctypes.c_uint32 * 7)(*self.fields())
+
+    def validate(self, log_length, n_witness_columns=NCOLS):
+        """wl_air_settings_validate: raises WhirlawayError when no valid schedule exists."""
+        st = self.c_struct()
+        check(lib().wl_air_settings_validate(log_length, n_witness_columns, ctypes.byref(st)))
 
 
 class Context:

@@ -197,8 +197,14 @@ struct wl_ctx {
             }
         }
         __atomic_thread_fence(__ATOMIC_ACQUIRE);
-        if (mbox && __atomic_load_n(&mbox[1], __ATOMIC_RELAXED) != 0)
+        if (mbox && __atomic_load_n(&mbox[1], __ATOMIC_RELAXED) != 0) {
+            // the failed proof's remaining device waits are released and drained, and
+            // the report cleared, so that the context serves later calls
+            release_all();
+            (void)hipStreamSynchronize(stream);
+            __atomic_store_n(&mbox[1], 0u, __ATOMIC_RELEASE);
             throw std::runtime_error("a device wait for a host value timed out");
+        }
         phase_ms[6] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     // ---- host values for pre-enqueued device work ----

@@ -12,6 +12,7 @@
 // depend on what else runs.
 #include <hip/hip_runtime.h>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -60,6 +61,7 @@ struct wl_engine {
     int running = 0, peak_running = 0;
     uint64_t completed = 0, next_id = 1;
     bool stop = false;
+    int waiters = 0;  // threads inside wl_engine_wait (destroy waits for them to leave)
 
     void worker(int slot);
 };
@@ -129,13 +131,33 @@ void wl_engine::worker(int slot) {
 
 extern "C" {
 
-uint64_t wl_job_footprint(uint32_t log_n) {
-    // Device bytes a proof of 2^log_n rows holds at its peak, from the buffers of
-    // the prover (packed witness 2^(n+10) B, first codeword + encoding scratch +
-    // Merkle tree ~2^(n+11.6) B, zerocheck round-0 rows 2^(n+10.3) B and columns
-    // 2^(n+8.4) B, WHIR rounds), with margin; checked against measured pool peaks
-    // by the test suite.
-    return ((uint64_t)8192 << log_n) + ((uint64_t)64 << 20);
+uint64_t wl_job_footprint(uint32_t log_n, const wl_air_settings *settings) {
+    // Device bytes a proof of 2^log_n rows can hold: the sum of the prover's buffers
+    // over all phases (an upper bound of the live peak, since the phases' buffers
+    // do not all coexist), plus a margin.  C = 165 committed columns, v = n + 8.
+    wl_air_settings d{128, 7, 4, 1, 4, 5, WL_CAPACITY_BOUND};
+    const wl_air_settings &s = settings ? *settings : d;
+    const double C = 165.0, N = std::ldexp(1.0, (int)log_n);
+    const int v = (int)log_n + 8, f0 = (int)s.folding_factor_first, f = (int)s.folding_factor;
+    const int rate = (int)s.log_inv_rate, rs0 = (int)s.rs_domain_initial_reduction;
+    const int k = (int)s.univariate_skips;
+    const double K = std::ldexp(1.0, k), M = N / K;
+    const double packed = std::ldexp(4.0, v);
+    // first codeword: the non-zero columns (of 2^f0) times 2^(v + rate - f0) rows, its
+    // encoding scratch of the same size, and the Merkle heap (64 B per node)
+    const double nnz = std::ceil(C * N / std::ldexp(1.0, v - f0));
+    const double rows0 = std::ldexp(1.0, v + rate - f0);
+    const double commit = 2.0 * nnz * rows0 * 4.0 + rows0 * 64.0;
+    // zerocheck: round-0 extended rows (skips <= 4), folded EF columns, eq tables
+    const double zc = (k <= 4 ? (2.0 * K - 2.0) * C * M * 4.0 : 0.0) + C * M * 32.0 + 2.0 * M * 32.0;
+    // inner sums and inner sumcheck: batched, mixed and matrix tables (EF)
+    const double inner = 4.0 * N * 32.0;
+    // WHIR opening: f, W and the column split of f (2^(v - f0) EF each), the first
+    // round's codeword (8 << f columns of 2^(v + rate - rs0 - f) rows) with scratch
+    // and tree, and the eq tables of the initial rounds' constraints
+    const double m1 = std::ldexp(1.0, v - f0), rows1 = std::ldexp(1.0, v + rate - rs0 - f);
+    const double whir = 3.0 * m1 * 32.0 + 2.0 * std::ldexp(8.0, f) * rows1 * 4.0 + rows1 * 64.0 + 4.0 * m1 * 32.0;
+    return (uint64_t)(packed + commit + zc + inner + whir) + ((uint64_t)64 << 20);
 }
 
 wl_engine *wl_engine_create(int device, int n_slots, uint64_t mem_budget_bytes) {
@@ -171,10 +193,24 @@ void wl_engine_destroy(wl_engine *e) {
     {
         std::lock_guard<std::mutex> lk(e->mu);
         e->stop = true;
-        e->queue.clear();  // jobs not yet admitted are dropped
+        // jobs not yet admitted complete with an error, so their waiters return
+        for (auto &j : e->queue) {
+            j->status = WL_ERR_ARG;
+            j->err = "engine stopped before the job was admitted";
+            j->t_start = j->t_done = clk::now();
+            j->done = true;
+        }
+        e->queue.clear();
     }
     e->cv_work.notify_all();
+    e->cv_done.notify_all();
     for (auto &t : e->workers) t.join();
+    {
+        // running jobs have completed and notified; wait until every waiter has left
+        std::unique_lock<std::mutex> lk(e->mu);
+        e->cv_done.notify_all();
+        e->cv_done.wait(lk, [&] { return e->waiters == 0; });
+    }
     for (wl_ctx *c : e->ctxs) wl_destroy(c);
     delete e;
 }
@@ -184,13 +220,14 @@ int wl_engine_submit(wl_engine *e, const wl_job *job, uint64_t *ticket) {
         wl_set_error("null argument");
         return WL_ERR_ARG;
     }
-    if (job->log_n < 2 || job->log_n > 24 || job->settings.univariate_skips >= job->log_n) {
+    if (job->log_n > 24) {
         wl_set_error("unsupported log_n");
         return WL_ERR_ARG;
     }
+    if (wl_air_settings_validate(job->log_n, 165, &job->settings) != WL_OK) return WL_ERR_ARG;
     auto j = std::make_shared<Job>();
     j->spec = *job;
-    j->footprint = wl_job_footprint(job->log_n);
+    j->footprint = wl_job_footprint(job->log_n, &job->settings);
     if (j->footprint > e->budget) {
         wl_set_error("job footprint exceeds the engine's memory budget");
         return WL_ERR_ARG;
@@ -220,31 +257,29 @@ int wl_engine_wait(wl_engine *e, uint64_t ticket, uint32_t *proof_out, size_t pr
                    double *times_ms) {
     if (!e) { wl_set_error("null engine"); return WL_ERR_ARG; }
     std::shared_ptr<Job> j;
-    {
-        std::unique_lock<std::mutex> lk(e->mu);
-        auto it = e->jobs.find(ticket);
-        if (it == e->jobs.end()) { wl_set_error("unknown ticket"); return WL_ERR_ARG; }
-        j = it->second;
-        e->cv_done.wait(lk, [&] { return j->done; });
-    }
+    // every access to the engine happens under its lock, before `waiters` drops:
+    // wl_engine_destroy frees the engine once no waiter is left
+    std::unique_lock<std::mutex> lk(e->mu);
+    auto it = e->jobs.find(ticket);
+    if (it == e->jobs.end()) { wl_set_error("unknown ticket"); return WL_ERR_ARG; }
+    j = it->second;
+    e->waiters++;
+    e->cv_done.wait(lk, [&] { return j->done; });
+    int rc = j->status;
+    if (rc == WL_OK && proof_out && proof_cap < j->proof.size()) rc = WL_ERR_BUFFER;  // the ticket stays
+    else e->jobs.erase(ticket);
+    e->waiters--;
+    if (e->waiters == 0 && e->stop) e->cv_done.notify_all();  // destroy may be waiting for this
+    lk.unlock();
     if (proof_len) *proof_len = j->proof.size();
     if (times_ms) {
         times_ms[0] = ms_between(j->t_submit, j->t_start);  // queued
         times_ms[1] = ms_between(j->t_start, j->t_done);    // running
         times_ms[2] = ms_between(j->t_submit, j->t_done);   // latency
     }
-    if (j->status != WL_OK) {
-        wl_set_error(j->err.c_str());
-        std::lock_guard<std::mutex> lk(e->mu);
-        e->jobs.erase(ticket);
-        return j->status;
-    }
-    if (proof_out) {
-        if (proof_cap < j->proof.size()) { wl_set_error("proof buffer too small"); return WL_ERR_BUFFER; }
-        memcpy(proof_out, j->proof.data(), j->proof.size() * 4);
-    }
-    std::lock_guard<std::mutex> lk(e->mu);
-    e->jobs.erase(ticket);
+    if (rc == WL_ERR_BUFFER) { wl_set_error("proof buffer too small"); return rc; }
+    if (rc != WL_OK) { wl_set_error(j->err.c_str()); return rc; }
+    if (proof_out) memcpy(proof_out, j->proof.data(), j->proof.size() * 4);
     return WL_OK;
 }
 

@@ -301,69 +301,95 @@ struct Transcript {
 };
 
 // ----------------------------------------------------------- WHIR schedule --
-// WhirConfig::new for ConstantFromSecondRound(f0, f), CapacityBound soundness,
-// 248-bit extension field (whir-p3 @ 9caef91; restated, parity unpinned).
+// WhirConfig::new for ConstantFromSecondRound(f0, f) and a SecurityAssumption,
+// 248-bit extension field (whir-p3 @ 9caef91 parameters/errors.rs; restated,
+// parity unpinned).
+enum Soundness { SA_UNIQUE_DECODING = 0, SA_JOHNSON_BOUND = 1, SA_CAPACITY_BOUND = 2 };
 struct WhirRound {
     int pow_bits, folding_pow_bits, num_queries, ood_samples, log_inv_rate, num_variables,
         folding_factor, log_domain_size;
 };
 struct WhirCfg {
-    int num_variables, security_level, max_pow_bits, f0, f, starting_log_inv_rate, rs_reduction0;
+    int num_variables, security_level, max_pow_bits, f0, f, starting_log_inv_rate, rs_reduction0, soundness;
     int n_rounds, final_sumcheck_rounds, commit_ood_samples, starting_folding_pow_bits;
     std::vector<WhirRound> rounds;
     int final_queries, final_pow_bits, final_folding_pow_bits, final_log_inv_rate, final_log_domain_size;
     int folding_at(int r) const { return r == 0 ? f0 : f; }
 };
 
+// Error terms of the round-by-round soundness analysis per decoding regime, in
+// bits.  eta, the proximity slack: sqrt(rho)/20 under the Johnson bound, rho/2
+// at capacity, none for unique decoding; r = log2(1/rho).
 namespace cfgmath {
-constexpr double FIELD_BITS = 248.0;
-inline double log_eta(int r) { return -((double)r + 1.0); }
-inline double list_size_bits(int d, int r) { return (double)(d + r) - log_eta(r); }
-inline double prox_gaps_error(int d, int r, int nf) {
-    return FIELD_BITS - ((double)(d + 2 * r) - log_eta(r) + log2((double)nf - 1.0));
+constexpr double FIELD_BITS = 248.0, LOG2_10 = 3.321928094887362;
+inline double log_eta(int sa, int r) {
+    if (sa == SA_UNIQUE_DECODING) return 0.0;
+    if (sa == SA_JOHNSON_BOUND) return -(0.5 * r + LOG2_10 + 1.0);
+    return -((double)r + 1.0);
 }
-inline int ood_samples(int sec, int d, int r) {
+inline double list_size_bits(int sa, int d, int r) {
+    if (sa == SA_UNIQUE_DECODING) return 0.0;
+    if (sa == SA_JOHNSON_BOUND) return 0.5 * r - (1.0 + log_eta(sa, r));
+    return (double)(d + r) - log_eta(sa, r);
+}
+inline double prox_gaps_error(int sa, int d, int r, int nf) {
+    double err;
+    if (sa == SA_UNIQUE_DECODING) err = (double)(d + r);
+    else if (sa == SA_JOHNSON_BOUND) err = 2.0 * d + 7.0 * (std::min(1.0 + LOG2_10 + 0.5 * r, -log_eta(sa, r)) - 1.0);
+    else err = (double)(d + 2 * r) - log_eta(sa, r);
+    return FIELD_BITS - (err + log2((double)nf - 1.0));
+}
+inline int ood_samples(int sa, int sec, int d, int r) {
+    if (sa == SA_UNIQUE_DECODING) return 0;
     for (int s = 1; s < 64; s++) {
-        double err = 2.0 * list_size_bits(d, r) + (double)(d * s);
+        double err = 2.0 * list_size_bits(sa, d, r) + (double)(d * s);
         if ((double)s * FIELD_BITS + 1.0 - err >= (double)sec) return s;
     }
     throw std::runtime_error("no OOD sample count reaches the security level");
 }
-inline int queries(int psec, int r) { return (int)ceil((double)psec / (double)r); }
-inline int folding_pow(int sec, int nv, int r) {
-    double e1 = prox_gaps_error(nv, r, 2);
-    double e2 = FIELD_BITS - (list_size_bits(nv, r) + 1.0);
+// bits of soundness per query: -log2(1 - delta)
+inline double query_bits(int sa, int r) {
+    if (sa == SA_UNIQUE_DECODING) return -log2(0.5 * (1.0 + 1.0 / (double)(1 << r)));
+    if (sa == SA_JOHNSON_BOUND) return 0.5 * r;
+    return (double)r;
+}
+inline int queries(int sa, int psec, int r) { return (int)ceil((double)psec / query_bits(sa, r)); }
+inline int folding_pow(int sa, int sec, int nv, int r) {
+    double e1 = prox_gaps_error(sa, nv, r, 2);
+    double e2 = FIELD_BITS - (list_size_bits(sa, nv, r) + 1.0);
     double v = (double)sec - std::min(e1, e2);
     return v > 0 ? (int)ceil(v) : 0;
 }
 }  // namespace cfgmath
 
-inline WhirCfg whir_config(int v, int sec, int max_pow, int f0, int f, int rate, int rs0) {
+inline WhirCfg whir_config(int v, int sec, int max_pow, int f0, int f, int rate, int rs0,
+                           int sa = SA_CAPACITY_BOUND) {
     using namespace cfgmath;
-    if (f0 > v || f <= 0 || rs0 > f0) throw std::runtime_error("invalid WHIR folding parameters");
+    if (f0 > v || f <= 0 || rs0 > f0 || rate < 1) throw std::runtime_error("invalid WHIR folding parameters");
+    if (sa < SA_UNIQUE_DECODING || sa > SA_CAPACITY_BOUND) throw std::runtime_error("unknown soundness assumption");
     WhirCfg c{};
     c.num_variables = v; c.security_level = sec; c.max_pow_bits = max_pow; c.f0 = f0; c.f = f;
-    c.starting_log_inv_rate = rate; c.rs_reduction0 = rs0;
+    c.starting_log_inv_rate = rate; c.rs_reduction0 = rs0; c.soundness = sa;
     int nv1 = v - f0;
     const int MAXC = 6;
     if (nv1 < MAXC) { c.n_rounds = 0; c.final_sumcheck_rounds = nv1; }
     else { c.n_rounds = (nv1 - MAXC + f - 1) / f; c.final_sumcheck_rounds = nv1 - c.n_rounds * f; }
     int psec = std::max(0, sec - max_pow);
-    c.commit_ood_samples = ood_samples(sec, v, rate);
-    c.starting_folding_pow_bits = folding_pow(sec, v, rate);
+    c.commit_ood_samples = ood_samples(sa, sec, v, rate);
+    c.starting_folding_pow_bits = folding_pow(sa, sec, v, rate);
     int nv = v - f0, lir = rate, logd = v + rate;
     for (int r = 0; r < c.n_rounds; r++) {
         int rs = r == 0 ? rs0 : 1;
         int fold = c.folding_at(r);
         int next = lir + (fold - rs);
-        int q = queries(psec, lir);
-        int ood = ood_samples(sec, nv, next);
-        double qe = (double)q * lir;
-        double ce = FIELD_BITS - (log2((double)(ood + q)) + list_size_bits(nv, next) + 1.0);
+        int q = queries(sa, psec, lir);
+        int ood = ood_samples(sa, sec, nv, next);
+        double qe = (double)q * query_bits(sa, lir);
+        double ce = FIELD_BITS - (log2((double)(ood + q)) + list_size_bits(sa, nv, next) + 1.0);
         double pw = (double)sec - std::min(qe, ce);
         WhirRound rp{};
         rp.pow_bits = pw > 0 ? (int)ceil(pw) : 0;
-        rp.folding_pow_bits = folding_pow(sec, nv, next);
+        rp.folding_pow_bits = folding_pow(sa, sec, nv, next);
         rp.num_queries = q; rp.ood_samples = ood; rp.log_inv_rate = lir; rp.num_variables = nv;
         rp.folding_factor = fold; rp.log_domain_size = logd;
         c.rounds.push_back(rp);
@@ -371,13 +397,18 @@ inline WhirCfg whir_config(int v, int sec, int max_pow, int f0, int f, int rate,
         lir = next;
         logd -= rs;
     }
-    c.final_queries = queries(psec, lir);
-    double fp = (double)sec - (double)c.final_queries * lir;
+    c.final_queries = queries(sa, psec, lir);
+    double fp = (double)sec - (double)c.final_queries * query_bits(sa, lir);
     c.final_pow_bits = fp > 0 ? (int)ceil(fp) : 0;
     double ffp = (double)sec - (FIELD_BITS - 1.0);
     c.final_folding_pow_bits = ffp > 0 ? (int)ceil(ffp) : 0;
     c.final_log_inv_rate = lir;
     c.final_log_domain_size = logd;
+    // every codeword is evaluated on a power-of-two subgroup of F*, at most 2^24
+    bool fits = v + rate - f0 <= TWO_ADICITY;
+    for (int r = 0; r < c.n_rounds; r++)
+        fits = fits && c.rounds[r].log_domain_size - (r == 0 ? rs0 : 1) - c.folding_at(r + 1) <= TWO_ADICITY;
+    if (!fits) throw std::runtime_error("a codeword domain exceeds 2^24, the field's two-adicity");
     return c;
 }
 

@@ -10,6 +10,7 @@
 //                            w_N^(n2*k1), DFT along n2, store X[k1 + N1*k2]
 // Mobius along one axis commutes with the DFT along the other, so the fused
 // pipeline equals Mobius-then-NTT on the whole column.
+#include <stdexcept>
 #include "common.h"
 #include "kernels.h"
 
@@ -38,7 +39,9 @@ __global__ void k_twiddles(F *out, int L, F w) {
     for (size_t i = start; i < start + chunk && i < N; i++) { out[i] = acc; acc = fmul(acc, w); }
 }
 void twiddles(F *out, int L, hipStream_t s) {
-    // generator of order 2^L: 3^((p-1)/2^L)
+    // generator of order 2^L: 3^((p-1)/2^L); KoalaBear's two-adicity is 24, so no
+    // larger power-of-two domain exists
+    if (L < 0 || L > TWO_ADICITY) throw std::runtime_error("NTT domain beyond the field's two-adicity (2^24)");
     F g = fpow(fsmall(3), (uint64_t)(P - 1) >> L);
     size_t N = (size_t)1 << L;
     k_twiddles<<<nblk((N + 63) / 64), 256, 0, s>>>(out, L, g);

@@ -23,6 +23,7 @@ constexpr uint32_t MU = 0x81000001u;          // P^-1 mod 2^32
 constexpr uint32_t R2 = 0x17f7efe4u;          // 2^64 mod P (to Montgomery)
 constexpr uint32_t ONE_M = 0x01fffffeu;       // 2^32 mod P (Montgomery 1)
 constexpr uint32_t W_EXT = 3u;                // X^8 = W
+constexpr int TWO_ADICITY = 24;               // 2^24 divides P - 1 exactly
 
 struct F { uint32_t v; };
 

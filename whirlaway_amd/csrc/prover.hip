@@ -519,7 +519,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     const int lc = (int)wh::log2_up(AIR_NCOLS);
     const int v = n + lc;
     WhirCfg cfg = wh::whir_config(v, sec, 16, (int)st->folding_factor_first, (int)st->folding_factor,
-                                  (int)st->log_inv_rate, (int)st->rs_domain_initial_reduction);
+                                  (int)st->log_inv_rate, (int)st->rs_domain_initial_reduction, (int)st->soundness);
     Transcript ts(&ph.p16);
     const std::vector<uint32_t> domsep = wh::whir_domsep(cfg);  // observed during the first commit
     c->ring_reset();
@@ -836,6 +836,26 @@ static thread_local std::string g_err;
 
 const char *wl_last_error(void) { return g_err.c_str(); }
 
+int wl_air_settings_validate(uint32_t log_length, uint32_t n_witness_columns, const wl_air_settings *st) {
+    if (!st) { g_err = "null argument"; return WL_ERR_ARG; }
+    if (st->folding_factor_first < 1 || st->folding_factor < 1 || log_length < 2 || log_length > 30 ||
+        n_witness_columns < 1 || st->univariate_skips < 1 || st->univariate_skips >= log_length ||
+        st->univariate_skips > 16 || st->log_inv_rate < 1 || st->log_inv_rate > 16 ||
+        st->security_bits < 1 || st->security_bits > 512) {
+        g_err = "unsupported settings";
+        return WL_ERR_ARG;
+    }
+    try {
+        const int v = (int)log_length + (int)wh::log2_up(n_witness_columns);
+        (void)wh::whir_config(v, (int)st->security_bits, 16, (int)st->folding_factor_first, (int)st->folding_factor,
+                              (int)st->log_inv_rate, (int)st->rs_domain_initial_reduction, (int)st->soundness);
+    } catch (const std::exception &e) {
+        g_err = std::string("unsupported settings: ") + e.what();
+        return WL_ERR_ARG;
+    }
+    return WL_OK;
+}
+
 wl_ctx *wl_create(int device) {
     try {
         HIP_CHECK(hipSetDevice(device));
@@ -941,14 +961,11 @@ static int prove_common(wl_table *t, const wl_air_settings *st, const uint32_t *
 
 static int check_settings(const wl_table *t, const wl_air_settings *st) {
     if (!t || !st) { g_err = "null argument"; return WL_ERR_ARG; }
-    if (st->folding_factor_first < 1 || st->folding_factor < 1 ||
-        st->rs_domain_initial_reduction > st->folding_factor_first ||
-        t->log_length + 8 < st->folding_factor_first || t->log_length < 2 ||
-        t->univariate_skips < 1 || t->univariate_skips >= t->log_length || st->log_inv_rate < 1) {
-        g_err = "unsupported settings";
+    if (st->univariate_skips != t->univariate_skips) {
+        g_err = "settings.univariate_skips differs from the table's";
         return WL_ERR_ARG;
     }
-    return WL_OK;
+    return wl_air_settings_validate(t->log_length, AIR_NCOLS, st);
 }
 
 int wl_air_prove(wl_table *t, const wl_air_settings *st, const uint32_t *p16c, const uint32_t *p24c,

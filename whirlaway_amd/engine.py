@@ -17,7 +17,7 @@ from .air import AirSettings
 class _Settings(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("security_bits", "folding_factor_first", "folding_factor",
                                                "log_inv_rate", "univariate_skips",
-                                               "rs_domain_initial_reduction")]
+                                               "rs_domain_initial_reduction", "soundness")]
 
 
 class _Job(ctypes.Structure):
@@ -29,9 +29,10 @@ class _Job(ctypes.Structure):
                 ("settings", _Settings)]
 
 
-def job_footprint(log_n):
-    """Device bytes reserved for a proof of 2^log_n permutations."""
-    return int(lib().wl_job_footprint(log_n))
+def job_footprint(log_n, settings=None):
+    """Device bytes reserved for a proof of 2^log_n permutations under `settings`."""
+    st = (settings or AirSettings()).c_struct()
+    return int(lib().wl_job_footprint(log_n, ctypes.byref(st)))
 
 
 class Engine:
@@ -51,8 +52,7 @@ class Engine:
         if arrs[1].size != 16 << log_n:
             raise ValueError("inputs must hold 16 x 2^log_n values")
         job = _Job(log_n, u32ptr(arrs[0]), u32ptr(arrs[1]), u32ptr(arrs[2]), u32ptr(arrs[3]),
-                   _Settings(s.security_bits, s.folding_factor_first, s.folding_factor, s.log_inv_rate,
-                             s.univariate_skips, s.rs_domain_initial_reduction))
+                   _Settings(*s.fields()))
         t = ctypes.c_uint64(0)
         check(lib().wl_engine_submit(self.ptr, ctypes.byref(job), ctypes.byref(t)))
         self._keep[t.value] = arrs
