@@ -50,6 +50,11 @@ MODELS = {
 }
 
 
+def _progress(msg):
+    """One line on stderr per finished leg (the JSON line alone goes to stdout)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def run_micro(args):
     """Kernel micro configs at full size: (a) 2^24 base-field values folded 24 times
     by extension challenges (benches/folding_benchmarks.rs shape), (b) the eq-weighted
@@ -268,6 +273,7 @@ def main():
     dom_live = ctx.kernel_timing_read(dom)
     ctx.kernel_timing([])
     phases = ctx.phase_times(reset=True)
+    _progress(f"{args.steps} timed proofs: {ms:.3f} ms each")
 
     pcie = None
     if args.pcie_steps > 0:
@@ -281,6 +287,7 @@ def main():
         pcie = {"value": round((1 << n) / tp, 1), "ms_per_step": round(tp * 1e3, 4),
                 "steps": args.pcie_steps, "same_proof": bool((proof_h == proof).all())}
         del host_w
+        _progress(f"host-witness proofs: {tp * 1e3:.3f} ms each")
 
     name, bound, per_unit = MODELS[dom]
     launches, kms, work = dom_live
@@ -322,10 +329,14 @@ def main():
         if args.cpu_log_n == n:
             identical = bool(cpu_proof.size == proof.size and (cpu_proof == proof).all())
         b.close()
+        _progress(f"CPU baseline: {tcpu:.2f} s")
 
     micro = None if args.no_micro else run_micro(args)
+    _progress("micro done")
     batched = run_batched(args) if args.batch_jobs > 0 else None
+    _progress("batched done")
     serving = run_serving(args) if args.serve_requests > 0 else None
+    _progress("serving done")
 
     line = {
         "metric": "Poseidon2 perms proven/s (2^17 perms, 1 GPU); batched proofs/s + p50/p99 latency",

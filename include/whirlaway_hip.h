@@ -22,11 +22,13 @@
 extern "C" {
 #endif
 
-enum { WL_OK = 0, WL_ERR_ARG = 1, WL_ERR_DEVICE = 2, WL_ERR_BUFFER = 3 };
+/* WL_ERR_VERIFY: a proof was rejected (verifier entry points) */
+enum { WL_OK = 0, WL_ERR_ARG = 1, WL_ERR_DEVICE = 2, WL_ERR_BUFFER = 3, WL_ERR_VERIFY = 4 };
 
 typedef struct wl_ctx wl_ctx;
 typedef struct wl_table wl_table;
 typedef struct wl_witness wl_witness;
+typedef struct wl_transcript wl_transcript;
 
 /* SecurityAssumption of whir-p3 (crates/air/src/lib.rs:3,15), in its order. */
 enum { WL_UNIQUE_DECODING = 0, WL_JOHNSON_BOUND = 1, WL_CAPACITY_BOUND = 2 };
@@ -83,6 +85,26 @@ int wl_air_prove(wl_table *table, const wl_air_settings *settings, const uint32_
                  const uint32_t *p24, const uint32_t *witness, uint32_t *proof_out,
                  size_t proof_cap, size_t *proof_len);
 
+/* The same proof appended to a caller's transcript, as AirTable::prove takes
+ * `prover_state: &mut ProverState` (crates/air/src/prove.rs:54) that the caller
+ * built from its DomainSeparator (src/examples/poseidon2.rs:155-163): the
+ * transcript continues from its current state, and proof_data grows by the
+ * proof (wl_transcript_proof_data).  The transcript's Poseidon2-16 must equal p16,
+ * as the reference hands the same permutation to the challenger and to the
+ * Merkle compression (src/examples/poseidon2.rs:148-163). */
+int wl_air_prove_into(wl_table *table, const wl_air_settings *settings, const uint32_t *p16,
+                      const uint32_t *p24, const uint32_t *witness, wl_transcript *ts);
+
+/* The domain-separator words this build observes first in wl_air_prove (the
+ * counterpart of DomainSeparator::commit_statement + add_whir_proof,
+ * src/examples/poseidon2.rs:155-157): one (tag, count) pair per transcript
+ * operation of the WHIR commitment and opening, tags 1 observe, 2 sample, 3 hint,
+ * 4 proof of work.  A caller whose whir-p3 version encodes the pattern otherwise
+ * passes its own words to wl_transcript_new and proves with wl_air_prove_into.
+ * *len receives the length; out may be NULL. */
+int wl_air_domain_separator(uint32_t log_length, uint32_t n_witness_columns, const wl_air_settings *settings,
+                            uint32_t *out, size_t cap, size_t *len);
+
 /* Witness resident in HBM (the commitment's packed polynomial,
  * crates/utils/src/multilinear.rs:169-181, built on the device). */
 wl_witness *wl_witness_upload(wl_ctx *ctx, const uint32_t *witness, uint32_t log_length);
@@ -91,6 +113,52 @@ void wl_witness_free(wl_witness *w);
 int wl_air_prove_resident(wl_table *table, const wl_air_settings *settings, const uint32_t *p16,
                           const uint32_t *p24, const wl_witness *w, uint32_t *proof_out,
                           size_t proof_cap, size_t *proof_len);
+/* wl_air_prove_into on a resident witness. */
+int wl_air_prove_resident_into(wl_table *table, const wl_air_settings *settings, const uint32_t *p16,
+                               const uint32_t *p24, const wl_witness *w, wl_transcript *ts);
+
+/* ------------------------------------------------------------- transcript --
+ * ProverState / VerifierState of whir-p3 over DuplexChallenger<F, Poseidon2KoalaBear<16>,
+ * 16, 8> (src/examples/poseidon2.rs:27,157-163,183-184), host code.  Created
+ * from the Poseidon2-16 constants (as wl_air_prove's p16) and the domain
+ * separator's field elements, which the challenger observes first
+ * (DomainSeparator::to_prover_state / to_verifier_state).  The sponge: observe
+ * overwrites the rate (8 words) and permutes when it is full; sample permutes
+ * when input is pending or the output is used up and pops the output's last word. */
+wl_transcript *wl_transcript_new(const uint32_t *p16, const uint32_t *domsep, size_t domsep_len);
+wl_transcript *wl_transcript_new_verifier(const uint32_t *p16, const uint32_t *domsep, size_t domsep_len,
+                                          const uint32_t *proof, size_t proof_len);
+void wl_transcript_free(wl_transcript *ts);
+/* FieldChallenger::observe (nothing appended) */
+int wl_transcript_observe(wl_transcript *ts, const uint32_t *v, size_t n);
+/* ProverState::add_base_scalars / add_extension_scalars: append and observe */
+int wl_transcript_add_base(wl_transcript *ts, const uint32_t *v, size_t n);
+int wl_transcript_add_ext(wl_transcript *ts, const uint32_t *v, size_t n_ef);
+/* ProverState hint: append without observing (Merkle openings) */
+int wl_transcript_hint(wl_transcript *ts, const uint32_t *v, size_t n);
+/* ProverState/VerifierState::sample (EF) and sample_bits */
+int wl_transcript_sample_ext(wl_transcript *ts, uint32_t *out, size_t n_ef);
+int wl_transcript_sample_bits(wl_transcript *ts, uint32_t bits, uint32_t *out);
+/* pow_grinding(bits) (crates/air/src/utils.rs:162-212): the smallest witness w with
+ * sample_bits(bits) == 0 after observe(w), appended; bits 0 does nothing.  On ctx's
+ * device when ctx is not NULL, else on the host.  *witness (may be NULL). */
+int wl_transcript_grind(wl_transcript *ts, wl_ctx *ctx, uint32_t bits, uint32_t *witness);
+/* proof_data (src/examples/poseidon2.rs:178): length and copy */
+size_t wl_transcript_proof_len(const wl_transcript *ts);
+int wl_transcript_proof_data(const wl_transcript *ts, uint32_t *out, size_t cap);
+/* The sponge as 34 canonical words: state[16], input[8], input count, output[8],
+ * output count; import restores it (a transcript continued elsewhere). */
+#define WL_TRANSCRIPT_STATE_WORDS 34
+int wl_transcript_export(const wl_transcript *ts, uint32_t *state);
+int wl_transcript_import(wl_transcript *ts, const uint32_t *state);
+/* Verifier: next_base_scalars / next_extension_scalars (read and observe), hints
+ * (read only), check_pow_grinding; WL_ERR_VERIFY when the proof is exhausted, holds
+ * a non-canonical word or the witness fails. */
+int wl_transcript_next_base(wl_transcript *ts, uint32_t *out, size_t n);
+int wl_transcript_next_ext(wl_transcript *ts, uint32_t *out, size_t n_ef);
+int wl_transcript_next_hint(wl_transcript *ts, uint32_t *out, size_t n);
+int wl_transcript_check_pow(wl_transcript *ts, uint32_t bits);
+size_t wl_transcript_remaining(const wl_transcript *ts);
 
 /* Synthetic benchmark inputs exactly as src/examples/poseidon2.rs:104-151 draws
  * them from StdRng::seed_from_u64(seed): air_rc (148, RoundConstants::from_rng),
@@ -180,6 +248,25 @@ void wl_engine_stats(wl_engine *e, uint64_t *out4);
  * 2^log_k EF scalars, EF output of 2^(log_n - log_k) elements. */
 int wl_fold_large(wl_ctx *ctx, const uint32_t *in, int in_is_ef, uint32_t log_n,
                   const uint32_t *scalars, uint32_t log_k, uint32_t *out);
+
+/* crates/utils/src/multilinear.rs:10-52 fold_multilinear_in_small_field:
+ * out[i] = sum_j scalars[j] * in[i + j * 2^(log_n - log_k)] with 2^log_k base-field
+ * scalars; in and out both F (in_is_ef = 0) or both EF. */
+int wl_fold_small(wl_ctx *ctx, const uint32_t *in, int in_is_ef, uint32_t log_n, const uint32_t *scalars,
+                  uint32_t log_k, uint32_t *out);
+
+/* crates/utils/src/multilinear.rs:125-147 multilinears_linear_combination:
+ * out[x] = sum_c scalars[c] * cols[c * 2^log_n + x]; n_cols base-field columns,
+ * n_cols EF scalars, 2^log_n EF outputs. */
+int wl_multilinears_linear_combination(wl_ctx *ctx, const uint32_t *cols, uint32_t n_cols, uint32_t log_n,
+                                       const uint32_t *scalars, uint32_t *out);
+
+/* EvaluationsList::evaluate (whir-p3; called at crates/air/src/prove.rs:190 and
+ * crates/air/src/verify.rs:114-136): the multilinear extension of 2^n_vars
+ * evaluations (F, or EF when is_ef) at an EF point of n_vars coordinates
+ * (coordinate 0 binds the most significant index bit), n_vars <= 28; out: 1 EF. */
+int wl_mle_evaluate(wl_ctx *ctx, const uint32_t *evals, int is_ef, uint32_t n_vars, const uint32_t *point,
+                    uint32_t *out);
 
 /* Kernel micro (a), benches/folding_benchmarks.rs: fold 2^log_n base-field
  * evaluations by n_folds successive EF challenges (1 - c, c), most-significant

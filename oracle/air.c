@@ -117,35 +117,45 @@ static ef_t matrix_down_lde(const ef_t *p, int len) {
     return ef_add(next_mle(p, len), prod);
 }
 
-/* crates/air/src/prove.rs:49-226 */
+/* crates/air/src/prove.rs:49-226 with the default transcript (the domain
+ * separator of src/examples/poseidon2.rs:155-163); returns the proof length */
 size_t or_air_prove(const or_air_settings_t *s, const or_p2_bench_t *b, f_t **proof_out) {
+    or_whir_cfg_t cfg;
+    or_air_whir_config(&cfg, s, b->log_n);
+    f_t domsep[4096];
+    size_t dn = or_whir_domsep(&cfg, domsep, 4096);
+    or_ts_t ts;
+    or_ts_init(&ts, &b->p16, domsep, dn);
+    or_air_prove_ts(s, b, &ts);
+    *proof_out = ts.data;
+    return ts.len;
+}
+
+/* crates/air/src/prove.rs:49-226: appends the proof to a caller's transcript */
+void or_air_prove_ts(const or_air_settings_t *s, const or_p2_bench_t *b, or_ts_t *ts) {
     int n = b->log_n, skips = s->univariate_skips, sec = s->security_bits;
     size_t N = (size_t)1 << n;
     int lc = (int)or_log2_up(NC);
     or_whir_cfg_t cfg;
     or_air_whir_config(&cfg, s, n);
-    f_t domsep[4096];
-    size_t dn = or_whir_domsep(&cfg, domsep, 4096);
-    or_ts_t ts;
-    or_ts_init(&ts, &b->p16, domsep, dn);
 
     /* 1) commit: packed_multilinear (crates/utils/src/multilinear.rs:169-181) */
     int v = n + lc;
     f_t *packed = (f_t *)calloc((size_t)1 << v, sizeof(f_t));
     memcpy(packed, b->trace, N * NC * sizeof(f_t));
     or_whir_witness_t w;
-    or_whir_commit(&w, &cfg, &b->p16, &b->p24, &ts, packed);
+    or_whir_commit(&w, &cfg, &b->p16, &b->p24, ts, packed);
     free(packed);
 
-    or_ts_pow(&ts, pow_bits_for(sec, (int)or_log2_up(NCONS)));
-    ef_t alpha = or_ts_sample_ef(&ts);
+    or_ts_pow(ts, pow_bits_for(sec, (int)or_log2_up(NCONS)));
+    ef_t alpha = or_ts_sample_ef(ts);
     ef_t ap[NCONS];
     ap[0] = ef_one();
     for (int i = 1; i < NCONS; i++) ap[i] = ef_mul(ap[i - 1], alpha);
-    or_ts_pow(&ts, pow_bits_for(sec, n));
+    or_ts_pow(ts, pow_bits_for(sec, n));
     int nz = n + 1 - skips;
     ef_t *zc = (ef_t *)malloc((size_t)nz * sizeof(ef_t));
-    for (int i = 0; i < nz; i++) zc[i] = or_ts_sample_ef(&ts);
+    for (int i = 0; i < nz; i++) zc[i] = or_ts_sample_ef(ts);
 
     /* columns_up_and_down (crates/air/src/utils.rs:140-160) */
     f_t *ud = (f_t *)malloc(2 * NC * N * sizeof(f_t));
@@ -163,15 +173,15 @@ size_t or_air_prove(const or_air_settings_t *s, const or_p2_bench_t *b, f_t **pr
     or_sc_comp_t comp = {OR_SC_AIR, &b->air, ap, NC};
     ef_t *ch = (ef_t *)malloc((size_t)nz * sizeof(ef_t));
     ef_t fe[2 * NC], fsum;
-    or_sc_prove(skips, 2 * NC, 1, mles, n, &comp, 3, zc, 1, &ts, ef_zero(), sec, ch, fe, &fsum);
+    or_sc_prove(skips, 2 * NC, 1, mles, n, &comp, 3, zc, 1, ts, ef_zero(), sec, ch, fe, &fsum);
     free(ud);
     free(mles);
 
-    or_ts_add_ef(&ts, fe, NC);
-    or_ts_add_ef(&ts, fe + NC, NC);
-    or_ts_pow(&ts, pow_bits_for(sec, lc));
+    or_ts_add_ef(ts, fe, NC);
+    or_ts_add_ef(ts, fe + NC, NC);
+    or_ts_pow(ts, pow_bits_for(sec, lc));
     ef_t cbs[16];
-    for (int i = 0; i < lc; i++) cbs[i] = or_ts_sample_ef(&ts);
+    for (int i = 0; i < lc; i++) cbs[i] = or_ts_sample_ef(ts);
     ef_t eqc[256];
     or_eval_eq(cbs, lc, eqc);
     /* multilinears_linear_combination (crates/utils/src/multilinear.rs:125-147) */
@@ -182,7 +192,7 @@ size_t or_air_prove(const or_air_settings_t *s, const or_p2_bench_t *b, f_t **pr
         for (int c = 0; c < NC; c++) acc = ef_add(acc, ef_mul_f(eqc[c], b->trace[(size_t)c * N + x]));
         batched[x] = acc;
     }
-    ef_t alpha2 = or_ts_sample_ef(&ts);
+    ef_t alpha2 = or_ts_sample_ef(ts);
     ef_t *mixed = (ef_t *)malloc(N * sizeof(ef_t));
     for (size_t x = 0; x < N; x++) {
         ef_t up = x == N - 1 ? batched[N - 2] : batched[x];
@@ -201,9 +211,9 @@ size_t or_air_prove(const or_air_settings_t *s, const or_p2_bench_t *b, f_t **pr
         sub[j] = acc;
     }
     free(eqr);
-    or_ts_add_ef(&ts, sub, (size_t)K);
+    or_ts_add_ef(ts, sub, (size_t)K);
     ef_t *pt = (ef_t *)malloc((size_t)n * sizeof(ef_t));
-    for (int i = 0; i < skips; i++) pt[i] = or_ts_sample_ef(&ts);
+    for (int i = 0; i < skips; i++) pt[i] = or_ts_sample_ef(ts);
     for (int i = 0; i < n - skips; i++) pt[skips + i] = ch[1 + i];
     ef_t *mu = (ef_t *)malloc(N * sizeof(ef_t));
     ef_t *md = (ef_t *)malloc(N * sizeof(ef_t));
@@ -217,21 +227,19 @@ size_t or_air_prove(const or_air_settings_t *s, const or_p2_bench_t *b, f_t **pr
     or_sc_comp_t icomp = {OR_SC_PRODUCT, NULL, NULL, 0};
     ef_t *ich = (ef_t *)malloc((size_t)n * sizeof(ef_t));
     ef_t ie[2], isum;
-    or_sc_prove(1, 2, 0, imles, n, &icomp, 2, NULL, 0, &ts, inner_sum, sec, ich, ie, &isum);
+    or_sc_prove(1, 2, 0, imles, n, &icomp, 2, NULL, 0, ts, inner_sum, sec, ich, ie, &isum);
     free(mu);
     free(batched);
     ef_t *fp = (ef_t *)malloc((size_t)v * sizeof(ef_t));
     for (int i = 0; i < lc; i++) fp[i] = cbs[i];
     for (int i = 0; i < n; i++) fp[lc + i] = ich[i];
-    or_whir_prove(&w, &ts, fp, ie[1]);
+    or_whir_prove(&w, ts, fp, ie[1]);
     or_whir_witness_free(&w);
     free(fp);
     free(ich);
     free(pt);
     free(ch);
     free(zc);
-    *proof_out = ts.data;
-    return ts.len;
 }
 
 /* crates/air/src/verify.rs:55-249.  Error codes follow AirVerifError order:

@@ -160,6 +160,34 @@ void or_api_fold_ef_large(const f_t *in, int log_n, const f_t *scalars, int k, f
     }
 }
 
+/* crates/utils/src/multilinear.rs:10-52 fold_multilinear_in_small_field:
+ * out[i] = sum_j s_j m[i + j 2^(log_n - k)], F scalars; m and out in F (width 1)
+ * or EF (width 8) */
+void or_api_fold_small(const f_t *in, int width, int log_n, const f_t *scalars, int k, f_t *out) {
+    size_t M = (size_t)1 << (log_n - k);
+    int K = 1 << k;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < M; i++)
+        for (int c = 0; c < width; c++) {
+            f_t acc = 0;
+            for (int j = 0; j < K; j++) acc = f_add(acc, f_mul(scalars[j], in[(i + (size_t)j * M) * width + c]));
+            out[i * width + c] = acc;
+        }
+}
+/* crates/utils/src/multilinear.rs:125-147 multilinears_linear_combination:
+ * out[x] = sum_c scalars[c] col_c[x], F columns, EF scalars and output */
+void or_api_lincomb(const f_t *cols, int n_cols, int log_n, const f_t *scalars, f_t *out) {
+    size_t N = (size_t)1 << log_n;
+    const ef_t *s = (const ef_t *)scalars;
+    ef_t *o = (ef_t *)out;
+#pragma omp parallel for schedule(static)
+    for (size_t x = 0; x < N; x++) {
+        ef_t acc = ef_zero();
+        for (int c = 0; c < n_cols; c++) acc = ef_add(acc, ef_mul_f(s[c], cols[(size_t)c * N + x]));
+        o[x] = acc;
+    }
+}
+
 /* ---- product sumcheck micro: sumcheck::prove(1, [a, b], a^2 b, 3, [1], Some(z), false, ...)
  * over a transcript whose challenger uses the Poseidon16 of bench handle h with an
  * empty domain separator.  Returns the proof length; proof copied to out. ---- */
@@ -183,4 +211,61 @@ size_t or_api_sc_a2b(void *h, const f_t *a, const f_t *b, int n, const f_t *z, f
     memcpy(final_sum, fs.c, 32);
     *proof_out = ts.data;
     return ts.len;
+}
+
+/* ---- transcript handles (whir-p3 ProverState over DuplexChallenger) ---- */
+typedef struct {
+    or_p2_t p16;
+    or_ts_t ts;
+} or_api_ts_t;
+
+static void p16_from_packed(or_p2_t *p, const f_t *c) {
+    memset(p, 0, sizeof *p);
+    p->width = 16;
+    p->rounds_f = 8;
+    p->rounds_p = 20;
+    int k = 0;
+    for (int r = 0; r < 4; r++) for (int i = 0; i < 16; i++) p->ext_init[r][i] = c[k++];
+    for (int r = 0; r < 4; r++) for (int i = 0; i < 16; i++) p->ext_term[r][i] = c[k++];
+    for (int r = 0; r < 20; r++) p->internal[r] = c[k++];
+}
+void *or_api_ts_new(const f_t *p16c, const f_t *domsep, size_t n) {
+    or_api_ts_t *h = (or_api_ts_t *)malloc(sizeof *h);
+    p16_from_packed(&h->p16, p16c);
+    or_ts_init(&h->ts, &h->p16, domsep, n);
+    return h;
+}
+void or_api_ts_free(void *h) {
+    or_ts_free(&((or_api_ts_t *)h)->ts);
+    free(h);
+}
+void or_api_ts_observe(void *h, const f_t *v, size_t n) {
+    for (size_t i = 0; i < n; i++) or_chal_observe(&((or_api_ts_t *)h)->ts.ch, v[i]);
+}
+void or_api_ts_add_base(void *h, const f_t *v, size_t n) { or_ts_add_base(&((or_api_ts_t *)h)->ts, v, n); }
+void or_api_ts_hint(void *h, const f_t *v, size_t n) { or_ts_hint_base(&((or_api_ts_t *)h)->ts, v, n); }
+void or_api_ts_sample_ef(void *h, f_t *out, size_t n_ef) {
+    for (size_t i = 0; i < n_ef; i++) {
+        ef_t e = or_ts_sample_ef(&((or_api_ts_t *)h)->ts);
+        memcpy(out + 8 * i, e.c, 32);
+    }
+}
+uint32_t or_api_ts_sample_bits(void *h, int bits) { return or_chal_sample_bits(&((or_api_ts_t *)h)->ts.ch, bits); }
+void or_api_ts_pow(void *h, int bits) { or_ts_pow(&((or_api_ts_t *)h)->ts, bits); }
+size_t or_api_ts_len(void *h) { return ((or_api_ts_t *)h)->ts.len; }
+const f_t *or_api_ts_data(void *h) { return ((or_api_ts_t *)h)->ts.data; }
+/* AirTable::prove appended to the handle's transcript (bench h for the inputs) */
+void or_api_prove_ts(void *bench, int sec, int f0, int f, int rate, int skips, int rs0, int sa, void *h) {
+    or_air_settings_t s;
+    settings(&s, sec, f0, f, rate, skips, rs0, sa);
+    or_air_prove_ts(&s, (or_p2_bench_t *)bench, &((or_api_ts_t *)h)->ts);
+}
+/* the domain-separator words of the default transcript */
+size_t or_api_domsep(int log_n, int sec, int f0, int f, int rate, int skips, int rs0, int sa, f_t *out,
+                     size_t cap) {
+    or_air_settings_t s;
+    settings(&s, sec, f0, f, rate, skips, rs0, sa);
+    or_whir_cfg_t cfg;
+    if (or_air_whir_config(&cfg, &s, log_n)) return 0;
+    return or_whir_domsep(&cfg, out, cap);
 }

@@ -272,6 +272,8 @@ void or_p2_bench_setup(or_p2_bench_t *b, int log_n, uint64_t seed);
 void or_p2_bench_free(or_p2_bench_t *b);
 /* crates/air/src/prove.rs:49-226 ; returns proof length in F */
 size_t or_air_prove(const or_air_settings_t *s, const or_p2_bench_t *b, f_t **proof_out);
+/* the same, appended to a caller's transcript */
+void or_air_prove_ts(const or_air_settings_t *s, const or_p2_bench_t *b, or_ts_t *ts);
 /* crates/air/src/verify.rs:55-249 ; returns 0 on accept, error code otherwise */
 int or_air_verify(const or_air_settings_t *s, const or_p2_bench_t *b, const f_t *proof,
                   size_t len);

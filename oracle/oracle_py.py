@@ -68,6 +68,25 @@ def lib():
         L.or_api_sc_a2b.restype = ctypes.c_size_t
         L.or_api_sc_a2b.argtypes = [vp, u32p, u32p, ctypes.c_int, u32p, u32p, u32p, u32p,
                                     ctypes.POINTER(u32p)]
+        L.or_api_fold_small.argtypes = [u32p, ctypes.c_int, ctypes.c_int, u32p, ctypes.c_int, u32p]
+        L.or_api_lincomb.argtypes = [u32p, ctypes.c_int, ctypes.c_int, u32p, u32p]
+        szt = ctypes.c_size_t
+        L.or_api_ts_new.restype = vp
+        L.or_api_ts_new.argtypes = [u32p, u32p, szt]
+        L.or_api_ts_free.argtypes = [vp]
+        for fn in ("observe", "add_base", "hint"):
+            getattr(L, "or_api_ts_" + fn).argtypes = [vp, u32p, szt]
+        L.or_api_ts_sample_ef.argtypes = [vp, u32p, szt]
+        L.or_api_ts_sample_bits.restype = ctypes.c_uint32
+        L.or_api_ts_sample_bits.argtypes = [vp, ctypes.c_int]
+        L.or_api_ts_pow.argtypes = [vp, ctypes.c_int]
+        L.or_api_ts_len.restype = szt
+        L.or_api_ts_len.argtypes = [vp]
+        L.or_api_ts_data.restype = u32p
+        L.or_api_ts_data.argtypes = [vp]
+        L.or_api_prove_ts.argtypes = [vp] + [ctypes.c_int] * 7 + [vp]
+        L.or_api_domsep.restype = szt
+        L.or_api_domsep.argtypes = [ctypes.c_int] * 8 + [u32p, szt]
         L.or_ntt.argtypes = [u32p, ctypes.c_int]
         L.or_mobius.argtypes = [u32p, ctypes.c_int]
         L.or_chacha_block.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, u32p]
@@ -145,6 +164,12 @@ class Bench:
         return lib().or_api_verify(self.h, s["sec"], s["f0"], s["f"], s["rate"], s["skips"],
                                    s["rs0"], s["sa"], ptr(pr), pr.size)
 
+    def prove_into(self, ts, **kw):
+        """AirTable::prove appended to the Transcript ts (its Poseidon2-16 must be this
+        bench's)."""
+        s = dict(DEFAULT, **kw)
+        lib().or_api_prove_ts(self.h, s["sec"], s["f0"], s["f"], s["rate"], s["skips"], s["rs0"], s["sa"], ts.h)
+
     def sc_a2b(self, a, b, z):
         n = a.size.bit_length() - 1
         ch = np.zeros(8 * n, dtype=np.uint32)
@@ -201,3 +226,84 @@ def fold_ef_large(m, scalars, k):
     out = np.zeros(((m.size // 8) >> k) * 8, dtype=np.uint32)
     lib().or_api_fold_ef_large(ptr(m), log_n, ptr(np.ascontiguousarray(scalars)), k, ptr(out))
     return out
+
+
+def fold_small(m, scalars, k, is_ef=False):
+    """fold_multilinear_in_small_field: F scalars, F or EF table."""
+    w = 8 if is_ef else 1
+    log_n = (m.size // w).bit_length() - 1
+    out = np.zeros(((m.size // w) >> k) * w, dtype=np.uint32)
+    lib().or_api_fold_small(ptr(m), w, log_n, ptr(np.ascontiguousarray(scalars)), k, ptr(out))
+    return out
+
+
+def lincomb(cols, scalars):
+    """multilinears_linear_combination: cols (n_cols x 2^n F), scalars (n_cols x 8)."""
+    n_cols, N = cols.shape
+    out = np.zeros(N * 8, dtype=np.uint32)
+    lib().or_api_lincomb(ptr(np.ascontiguousarray(cols).reshape(-1)), n_cols, N.bit_length() - 1,
+                         ptr(np.ascontiguousarray(scalars).reshape(-1)), ptr(out))
+    return out
+
+
+def mle_eval(evals, point, is_ef=False):
+    """EvaluationsList::evaluate at an EF point (big-endian variables)."""
+    n = point.size // 8
+    out = np.zeros(8, dtype=np.uint32)
+    fn = lib().or_api_mle_eval_ef if is_ef else lib().or_api_mle_eval_f
+    fn(ptr(np.ascontiguousarray(evals)), n, ptr(np.ascontiguousarray(point).reshape(-1)), ptr(out))
+    return out
+
+
+def domain_separator(log_n, **kw):
+    s = dict(DEFAULT, **kw)
+    out = np.zeros(4096, dtype=np.uint32)
+    k = lib().or_api_domsep(log_n, s["sec"], s["f0"], s["f"], s["rate"], s["skips"], s["rs0"], s["sa"], ptr(out),
+                            out.size)
+    return out[:k].copy()
+
+
+class Transcript:
+    """The oracle's ProverState over DuplexChallenger<F, Poseidon2-16, 16, 8>."""
+
+    def __init__(self, p16, domsep=()):
+        d = np.ascontiguousarray(domsep, dtype=np.uint32)
+        self.h = lib().or_api_ts_new(ptr(np.ascontiguousarray(p16, dtype=np.uint32)), ptr(d), d.size)
+
+    def close(self):
+        if self.h:
+            lib().or_api_ts_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _call(self, fn, v):
+        a = np.ascontiguousarray(v, dtype=np.uint32).reshape(-1)
+        getattr(lib(), "or_api_ts_" + fn)(self.h, ptr(a), a.size)
+
+    def observe(self, v):
+        self._call("observe", v)
+
+    def add_base(self, v):
+        self._call("add_base", v)
+
+    def hint(self, v):
+        self._call("hint", v)
+
+    def sample_ext(self, n=1):
+        out = np.zeros(8 * n, dtype=np.uint32)
+        lib().or_api_ts_sample_ef(self.h, ptr(out), n)
+        return out.reshape(n, 8)
+
+    def sample_bits(self, bits):
+        return int(lib().or_api_ts_sample_bits(self.h, bits))
+
+    def grind(self, bits):
+        lib().or_api_ts_pow(self.h, bits)
+
+    def proof_data(self):
+        n = lib().or_api_ts_len(self.h)
+        if n == 0:
+            return np.zeros(0, dtype=np.uint32)
+        return np.ctypeslib.as_array(lib().or_api_ts_data(self.h), shape=(n,)).copy()

@@ -294,3 +294,40 @@ def test_sumcheck_a2b_full_size(gpu_ctx, oracle):
     for k in ("proof", "challenges", "final_evals", "final_sum"):
         assert np.array_equal(got[k], want[k]), k
     b.close()
+
+
+@pytest.mark.parametrize("n", [8, 11])
+def test_prove_into_caller_transcript(gpu_ctx, oracle, n):
+    """AirTable::prove on a caller's ProverState (crates/air/src/prove.rs:54): the
+    transcript already holds a domain separator and earlier messages, and the proof
+    continues it.  proof_data equals the oracle's for the same prefix; the device
+    grinding of a transcript equals the host grinding."""
+    import whirlaway_amd as wa
+    from whirlaway_amd.transcript import ProverState
+    _say(f"prove_into n={n}")
+    b = oracle.Bench(n, seed=21)
+    p16, p24 = b.p2_consts(16), b.p2_consts(24)
+    rng = np.random.default_rng(n)
+    dom = rng.integers(0, P, 50, dtype=np.uint32)
+    prefix = rng.integers(0, P, 13, dtype=np.uint32)
+    ours = ProverState(p16, dom)
+    ours.add_base_scalars(prefix)
+    ours.pow_grinding(6, ctx=gpu_ctx)
+    ref = oracle.Transcript(p16, dom)
+    ref.add_base(prefix)
+    ref.grind(6)
+    table = wa.AirTable(gpu_ctx, b.air_consts(), n)
+    table.prove_into(wa.AirSettings(), p16, p24, b.trace(), ours)
+    table.close()
+    b.prove_into(ref)
+    got, want = ours.proof_data(), ref.proof_data()
+    assert got.size == want.size and np.array_equal(got, want)
+    # the transcripts end in the same state
+    assert np.array_equal(ours.sample(2), ref.sample_ext(2))
+    ours.close()
+    # a transcript whose Poseidon2-16 differs from the Merkle compression's is refused
+    other = ProverState(oracle.Bench(n, seed=22).p2_consts(16))
+    table = wa.AirTable(gpu_ctx, b.air_consts(), n)
+    with pytest.raises(wa.WhirlawayError):
+        table.prove_into(wa.AirSettings(), p16, p24, b.trace(), other)
+    table.close()

@@ -252,6 +252,34 @@ def test_mle_eval_and_folds_agree_with_brute_force(oracle):
         assert [int(x) for x in step[8 * i: 8 * i + 8]] == want_i
 
 
+def test_small_field_fold_and_linear_combination(oracle):
+    """fold_multilinear_in_small_field against Python integers, and against the
+    large-field fold with the same scalars embedded in EF (another oracle path);
+    multilinears_linear_combination against Python integers."""
+    rng = np.random.default_rng(21)
+    n, k = 6, 2
+    m = _rand_f(rng, 1 << n)
+    sc = _rand_f(rng, 1 << k)
+    got = oracle.fold_small(_u32(m), _u32(sc), k)
+    M = 1 << (n - k)
+    want = [sum(sc[j] * m[i + j * M] for j in range(1 << k)) % P for i in range(M)]
+    assert [int(x) for x in got] == want
+    emb = _u32(pf.to_u32([pf.ef(v) for v in sc]))
+    large = oracle.fold_f_large(_u32(m), emb, k)
+    assert [int(x) for x in large[0::8]] == want and not large.reshape(-1, 8)[:, 1:].any()
+    me = _rand_f(rng, (1 << n) * 8)
+    gote = oracle.fold_small(_u32(me), _u32(sc), k, is_ef=True)
+    for i in (0, 5, M - 1):
+        for c in range(8):
+            assert int(gote[8 * i + c]) == sum(sc[j] * me[8 * (i + j * M) + c] for j in range(1 << k)) % P
+    cols = [_rand_f(rng, 1 << 4) for _ in range(3)]
+    scal = [_rand_ef(rng) for _ in range(3)]
+    lc = oracle.lincomb(_u32(cols), _u32(scal))
+    for x in (0, 7, 15):
+        want_x = [sum(scal[c][d] * cols[c][x] for c in range(3)) % P for d in range(8)]
+        assert [int(v) for v in lc[8 * x: 8 * x + 8]] == want_x
+
+
 def test_selectors_closed_form(oracle):
     rng = np.random.default_rng(6)
     for k in (1, 2, 4):

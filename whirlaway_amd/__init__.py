@@ -50,6 +50,9 @@ def lib():
         L.wl_air_prove_resident.argtypes = [vp, vp, u32p, u32p, vp, u32p, ctypes.c_size_t, szp]
         L.wl_phase_times.argtypes = [vp, dp, ctypes.c_int]
         L.wl_fold_large.argtypes = [vp, u32p, ctypes.c_int, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p]
+        L.wl_fold_small.argtypes = [vp, u32p, ctypes.c_int, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p]
+        L.wl_multilinears_linear_combination.argtypes = [vp, u32p, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p]
+        L.wl_mle_evaluate.argtypes = [vp, u32p, ctypes.c_int, ctypes.c_uint32, u32p, u32p]
         L.wl_fold_chain.argtypes = [vp, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p, dp]
         L.wl_sumcheck_a2b.argtypes = [vp, u32p, u32p, u32p, ctypes.c_uint32, u32p, u32p, u32p, u32p,
                                       u32p, ctypes.c_size_t, szp, dp]
@@ -74,6 +77,26 @@ def lib():
         L.wl_engine_poll.argtypes = [vp, u64]
         L.wl_engine_wait.argtypes = [vp, u64, u32p, ctypes.c_size_t, szp, dp]
         L.wl_engine_stats.argtypes = [vp, u64p]
+        L.wl_air_prove_into.argtypes = [vp, vp, u32p, u32p, u32p, vp]
+        L.wl_air_prove_resident_into.argtypes = [vp, vp, u32p, u32p, vp, vp]
+        L.wl_air_domain_separator.argtypes = [ctypes.c_uint32, ctypes.c_uint32, vp, u32p, ctypes.c_size_t, szp]
+        L.wl_transcript_new.restype = vp
+        L.wl_transcript_new.argtypes = [u32p, u32p, ctypes.c_size_t]
+        L.wl_transcript_new_verifier.restype = vp
+        L.wl_transcript_new_verifier.argtypes = [u32p, u32p, ctypes.c_size_t, u32p, ctypes.c_size_t]
+        L.wl_transcript_free.argtypes = [vp]
+        for fn in ("observe", "add_base", "add_ext", "hint", "sample_ext", "next_base", "next_ext", "next_hint"):
+            getattr(L, "wl_transcript_" + fn).argtypes = [vp, u32p, ctypes.c_size_t]
+        L.wl_transcript_sample_bits.argtypes = [vp, ctypes.c_uint32, u32p]
+        L.wl_transcript_grind.argtypes = [vp, vp, ctypes.c_uint32, u32p]
+        L.wl_transcript_proof_len.restype = ctypes.c_size_t
+        L.wl_transcript_proof_len.argtypes = [vp]
+        L.wl_transcript_proof_data.argtypes = [vp, u32p, ctypes.c_size_t]
+        L.wl_transcript_export.argtypes = [vp, u32p]
+        L.wl_transcript_import.argtypes = [vp, u32p]
+        L.wl_transcript_check_pow.argtypes = [vp, ctypes.c_uint32]
+        L.wl_transcript_remaining.restype = ctypes.c_size_t
+        L.wl_transcript_remaining.argtypes = [vp]
         L.wl_kernel_timing.argtypes = [vp, ctypes.c_uint32]
         L.wl_kernel_timing_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), dp, dp]
         _lib = L
@@ -94,4 +117,4 @@ def check(rc):
 
 
 from .air import AirSettings, AirTable, Context  # noqa: E402
-from . import engine, sumcheck, utils  # noqa: E402,F401
+from . import engine, sumcheck, transcript, utils  # noqa: E402,F401

@@ -123,6 +123,24 @@ class AirTable:
             lib().wl_air_table_free(self.ptr)
             self.ptr = None
 
+    def prove_into(self, settings, p16, p24, witness, prover_state):
+        """AirTable::prove(settings, merkle_hash, merkle_compress, prover_state, witness)
+        (crates/air/src/prove.rs:49-66): appends the proof to a caller's ProverState
+        (whirlaway_amd.transcript), continuing its Fiat-Shamir state."""
+        p16 = np.ascontiguousarray(p16, dtype=np.uint32)
+        p24 = np.ascontiguousarray(p24, dtype=np.uint32)
+        st = settings.c_struct()
+        L = lib()
+        if isinstance(witness, Witness):
+            check(L.wl_air_prove_resident_into(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24), witness.ptr,
+                                               prover_state.ptr))
+        else:
+            w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1)
+            if w.size != NCOLS << self.log_length:
+                raise ValueError("witness must hold 165 columns of 2^log_length values")
+            check(L.wl_air_prove_into(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24), u32ptr(w),
+                                      prover_state.ptr))
+
     def prove(self, settings, p16, p24, witness):
         """Returns proof_data (uint32 canonical field elements).  witness: 165 x 2^n
         column-major (uint32) or a resident Witness."""

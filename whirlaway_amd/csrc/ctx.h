@@ -327,6 +327,21 @@ struct wl_table {
     wk::ChainLin *chain_dev = nullptr;
 };
 
+// Fiat-Shamir state of whir-p3's ProverState / VerifierState over
+// DuplexChallenger<F, Poseidon2-16, 16, 8> (src/examples/poseidon2.rs:27,157-163):
+// the sponge, the proof_data written (prover) or being read (verifier).  Not
+// copyable: ts points at p16.
+struct wl_transcript {
+    kb::P2Consts16 p16;           // Montgomery form
+    wh::Transcript ts;            // prover: ts.data is proof_data
+    std::vector<uint32_t> proof;  // verifier: the proof_data read from pos on
+    size_t pos = 0;
+    bool verifier = false;
+    wl_transcript() : ts(&p16) {}
+    wl_transcript(const wl_transcript &) = delete;
+    wl_transcript &operator=(const wl_transcript &) = delete;
+};
+
 // Trace resident in HBM as the commitment's packed polynomial (Montgomery form,
 // 165 columns x 2^n then zeros up to 2^(n+8)).
 struct wl_witness {

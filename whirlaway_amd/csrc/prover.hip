@@ -510,18 +510,24 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
 }
 
 // --------------------------------------------------------------- AIR prove --
-// trace: device, Montgomery, 165 columns x 2^n inside a zero-padded 2^(n+8) buffer
-std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2Dev &ph,
-                                const P2Dev *pdev, const F *packed) {
+// AirTable::build_whir_params (crates/air/src/table.rs:72-98): the committed
+// polynomial has log_length + log2_up(n_witness_columns) variables
+WhirCfg air_whir_config(uint32_t log_length, uint32_t n_witness_columns, const wl_air_settings *st) {
+    const int v = (int)log_length + (int)wh::log2_up(n_witness_columns);
+    return wh::whir_config(v, (int)st->security_bits, 16, (int)st->folding_factor_first, (int)st->folding_factor,
+                           (int)st->log_inv_rate, (int)st->rs_domain_initial_reduction, (int)st->soundness);
+}
+
+// trace: device, Montgomery, 165 columns x 2^n inside a zero-padded 2^(n+8) buffer.
+// Appends to the transcript ts; absorb_first (optional): domain-separator words the
+// transcript observes before the first message, absorbed while the first commit runs.
+void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const F *packed, Transcript &ts,
+               const std::vector<uint32_t> *absorb_first) {
     wl_ctx *c = t->ctx;
     const int n = (int)t->log_length, skips = (int)t->univariate_skips, sec = (int)st->security_bits;
     const size_t N = (size_t)1 << n;
     const int lc = (int)wh::log2_up(AIR_NCOLS);
-    const int v = n + lc;
-    WhirCfg cfg = wh::whir_config(v, sec, 16, (int)st->folding_factor_first, (int)st->folding_factor,
-                                  (int)st->log_inv_rate, (int)st->rs_domain_initial_reduction, (int)st->soundness);
-    Transcript ts(&ph.p16);
-    const std::vector<uint32_t> domsep = wh::whir_domsep(cfg);  // observed during the first commit
+    WhirCfg cfg = air_whir_config(t->log_length, AIR_NCOLS, st);
     c->ring_reset();
 
     auto tp = std::chrono::steady_clock::now();
@@ -534,7 +540,7 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     };
 
     WhirWitness w;
-    whir_commit(c, cfg, pdev, ts, packed, (size_t)AIR_NCOLS * N, w, &domsep);
+    whir_commit(c, cfg, pdev, ts, packed, (size_t)AIR_NCOLS * N, w, absorb_first);
     lap(0);
     grind(c, ts, pdev, pow_bits_air(sec, (int)wh::log2_up(AIR_NCONS)));
     EF alpha = ts.sample_ef();
@@ -800,7 +806,6 @@ std::vector<uint32_t> air_prove(wl_table *t, const wl_air_settings *st, const P2
     fp.insert(fp.end(), ich.begin(), ich.end());
     whir_prove(c, cfg, pdev, ts, w, fp, packed_value);
     lap(5);
-    return ts.data;
 }
 
 }  // namespace
@@ -936,20 +941,37 @@ void wl_air_table_free(wl_table *t) {
     delete t;
 }
 
+// Proves into `ts`: a caller's transcript (wl_transcript), or, when null, a fresh
+// one over p16 that first observes this build's domain separator.
 static int prove_common(wl_table *t, const wl_air_settings *st, const uint32_t *p16c,
                         const uint32_t *p24c, F *packed, uint32_t *proof_out, size_t proof_cap,
-                        size_t *proof_len) {
+                        size_t *proof_len, wl_transcript *caller_ts = nullptr) {
     wl_ctx *c = t->ctx;
     P2Dev ph;
     load_p2(p16c, p24c, ph);
+    if (caller_ts && memcmp(&caller_ts->p16, &ph.p16, sizeof ph.p16) != 0) {
+        g_err = "the transcript's Poseidon2-16 differs from the Merkle compression's";
+        return WL_ERR_ARG;
+    }
     const P2Dev *pd = context_p2(c, ph);
     std::vector<uint32_t> proof;
     try {
-        proof = air_prove(t, st, ph, pd, packed);
+        if (caller_ts) {
+            air_prove(t, st, pd, packed, caller_ts->ts, nullptr);
+        } else {
+            Transcript ts(&ph.p16);
+            const std::vector<uint32_t> domsep = wh::whir_domsep(air_whir_config(t->log_length, AIR_NCOLS, st));
+            air_prove(t, st, pd, packed, ts, &domsep);
+            proof = std::move(ts.data);
+        }
         HIP_CHECK(hipStreamSynchronize(c->stream));  // nothing of this proof left in flight
     } catch (...) {
         c->release_all();  // device waits for host values must not outlive the failure
         throw;
+    }
+    if (caller_ts) {
+        if (proof_len) *proof_len = caller_ts->ts.data.size();
+        return WL_OK;
     }
     if (proof_len) *proof_len = proof.size();
     if (proof_out) {
@@ -968,24 +990,70 @@ static int check_settings(const wl_table *t, const wl_air_settings *st) {
     return wl_air_settings_validate(t->log_length, AIR_NCOLS, st);
 }
 
-int wl_air_prove(wl_table *t, const wl_air_settings *st, const uint32_t *p16c, const uint32_t *p24c,
-                 const uint32_t *witness, uint32_t *proof_out, size_t proof_cap, size_t *proof_len) {
+// packed_multilinear (crates/utils/src/multilinear.rs:169-181) of a host witness of
+// 165 canonical columns, in Montgomery form, zero-padded to 2^(n+8)
+static void upload_packed(wl_ctx *c, const uint32_t *witness, uint32_t log_length, F *packed) {
+    const size_t N = (size_t)1 << log_length, V = N << 8;
+    wk::fill_zero(packed + AIR_NCOLS * N, (V - AIR_NCOLS * N) * 4, c->stream);
+    HIP_CHECK(hipMemcpyAsync(packed, witness, AIR_NCOLS * N * 4, hipMemcpyHostToDevice, c->stream));
+    wk::convert_to_mont((uint32_t *)packed, AIR_NCOLS * N, c->stream);
+}
+
+static int prove_host_witness(wl_table *t, const wl_air_settings *st, const uint32_t *p16c, const uint32_t *p24c,
+                              const uint32_t *witness, uint32_t *proof_out, size_t proof_cap, size_t *proof_len,
+                              wl_transcript *ts) {
     int rc = check_settings(t, st);
     if (rc) return rc;
+    if (!witness) { g_err = "null witness"; return WL_ERR_ARG; }
     try {
         wl_ctx *c = t->ctx;
         HIP_CHECK(hipSetDevice(c->device));
-        size_t N = (size_t)1 << t->log_length;
-        size_t v = t->log_length + 8;
-        DBuf<F> packed(c->pool, (size_t)1 << v);
-        wk::fill_zero(packed.p + AIR_NCOLS * N, (((size_t)1 << v) - AIR_NCOLS * N) * 4, c->stream);
-        HIP_CHECK(hipMemcpyAsync(packed.p, witness, AIR_NCOLS * N * 4, hipMemcpyHostToDevice, c->stream));
-        wk::convert_to_mont((uint32_t *)packed.p, AIR_NCOLS * N, c->stream);
-        return prove_common(t, st, p16c, p24c, packed, proof_out, proof_cap, proof_len);
+        DBuf<F> packed(c->pool, (size_t)1 << (t->log_length + 8));
+        upload_packed(c, witness, t->log_length, packed);
+        return prove_common(t, st, p16c, p24c, packed, proof_out, proof_cap, proof_len, ts);
     } catch (const std::exception &e) {
         g_err = e.what();
         return WL_ERR_DEVICE;
     }
+}
+
+int wl_air_prove(wl_table *t, const wl_air_settings *st, const uint32_t *p16c, const uint32_t *p24c,
+                 const uint32_t *witness, uint32_t *proof_out, size_t proof_cap, size_t *proof_len) {
+    return prove_host_witness(t, st, p16c, p24c, witness, proof_out, proof_cap, proof_len, nullptr);
+}
+
+int wl_air_prove_into(wl_table *t, const wl_air_settings *st, const uint32_t *p16c, const uint32_t *p24c,
+                      const uint32_t *witness, wl_transcript *ts) {
+    if (!ts || ts->verifier) { g_err = "a prover transcript is required"; return WL_ERR_ARG; }
+    return prove_host_witness(t, st, p16c, p24c, witness, nullptr, 0, nullptr, ts);
+}
+
+int wl_air_prove_resident_into(wl_table *t, const wl_air_settings *st, const uint32_t *p16c,
+                               const uint32_t *p24c, const wl_witness *w, wl_transcript *ts) {
+    if (!ts || ts->verifier) { g_err = "a prover transcript is required"; return WL_ERR_ARG; }
+    int rc = check_settings(t, st);
+    if (rc) return rc;
+    if (!w || w->log_length != t->log_length) { g_err = "witness size mismatch"; return WL_ERR_ARG; }
+    try {
+        HIP_CHECK(hipSetDevice(t->ctx->device));
+        return prove_common(t, st, p16c, p24c, w->packed, nullptr, 0, nullptr, ts);
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return WL_ERR_DEVICE;
+    }
+}
+
+int wl_air_domain_separator(uint32_t log_length, uint32_t n_witness_columns, const wl_air_settings *st,
+                            uint32_t *out, size_t cap, size_t *len) {
+    int rc = wl_air_settings_validate(log_length, n_witness_columns, st);
+    if (rc) return rc;
+    const std::vector<uint32_t> d = wh::whir_domsep(air_whir_config(log_length, n_witness_columns, st));
+    if (len) *len = d.size();
+    if (out) {
+        if (cap < d.size()) { g_err = "buffer too small"; return WL_ERR_BUFFER; }
+        memcpy(out, d.data(), d.size() * 4);
+    }
+    return WL_OK;
 }
 
 
@@ -995,11 +1063,8 @@ wl_witness *wl_witness_upload(wl_ctx *c, const uint32_t *witness, uint32_t log_l
         wl_witness *w = new wl_witness();
         w->ctx = c;
         w->log_length = log_length;
-        size_t N = (size_t)1 << log_length, V = N << 8;
-        w->packed = (F *)c->pool.get(V * 4);
-        wk::fill_zero(w->packed + AIR_NCOLS * N, (V - AIR_NCOLS * N) * 4, c->stream);
-        HIP_CHECK(hipMemcpyAsync(w->packed, witness, AIR_NCOLS * N * 4, hipMemcpyHostToDevice, c->stream));
-        wk::convert_to_mont((uint32_t *)w->packed, AIR_NCOLS * N, c->stream);
+        w->packed = (F *)c->pool.get(((size_t)4 << 8) << log_length);
+        upload_packed(c, witness, log_length, w->packed);
         HIP_CHECK(hipStreamSynchronize(c->stream));
         return w;
     } catch (const std::exception &e) {

@@ -1,7 +1,10 @@
-"""crates/utils counterpart: multilinear folds, Poseidon2 and the RS/Merkle core.
+"""crates/utils counterpart: multilinear folds, linear combination, evaluation,
+Poseidon2 and the RS/Merkle core.
 
-fold_multilinear_in_large_field mirrors crates/utils/src/multilinear.rs:90-123;
-fold_chain is the kernel micro-benchmark of benches/folding_benchmarks.rs.
+fold_multilinear_in_small_field / _in_large_field mirror
+crates/utils/src/multilinear.rs:10-52 / 90-123, multilinears_linear_combination
+:125-147; mle_evaluate is EvaluationsList::evaluate; fold_chain is the kernel
+micro-benchmark of benches/folding_benchmarks.rs.
 """
 import ctypes
 
@@ -18,6 +21,46 @@ def fold_multilinear_in_large_field(ctx, m, scalars, is_ef=False):
     log_n = n_elems.bit_length() - 1
     out = np.zeros((n_elems >> k) * 8, dtype=np.uint32)
     check(lib().wl_fold_large(ctx.ptr, u32ptr(m), int(is_ef), log_n, u32ptr(scalars), k, u32ptr(out)))
+    return out
+
+
+def fold_multilinear_in_small_field(ctx, m, scalars, is_ef=False):
+    """out[i] = sum_j scalars[j] m[i + j N/K]: K base-field scalars, F or EF table."""
+    m = np.ascontiguousarray(m, dtype=np.uint32)
+    scalars = np.ascontiguousarray(scalars, dtype=np.uint32).reshape(-1)
+    k = scalars.size.bit_length() - 1
+    if scalars.size != 1 << k:
+        raise ValueError("the number of scalars must be a power of two")
+    w = 8 if is_ef else 1
+    n_elems = m.size // w
+    out = np.zeros((n_elems >> k) * w, dtype=np.uint32)
+    check(lib().wl_fold_small(ctx.ptr, u32ptr(m), int(is_ef), n_elems.bit_length() - 1, u32ptr(scalars), k,
+                              u32ptr(out)))
+    return out
+
+
+def multilinears_linear_combination(ctx, cols, scalars):
+    """sum_c scalars[c] cols[c]: cols (n_cols, 2^n) base field, scalars (n_cols, 8) EF."""
+    cols = np.ascontiguousarray(cols, dtype=np.uint32)
+    n_cols, N = cols.shape
+    sc = np.ascontiguousarray(scalars, dtype=np.uint32).reshape(-1)
+    if sc.size != 8 * n_cols:
+        raise ValueError("one EF scalar per column")
+    out = np.zeros(N * 8, dtype=np.uint32)
+    check(lib().wl_multilinears_linear_combination(ctx.ptr, u32ptr(cols), n_cols, N.bit_length() - 1, u32ptr(sc),
+                                                   u32ptr(out)))
+    return out
+
+
+def mle_evaluate(ctx, evals, point, is_ef=False):
+    """EvaluationsList::evaluate: the multilinear extension at an EF point (n x 8)."""
+    evals = np.ascontiguousarray(evals, dtype=np.uint32)
+    pt = np.ascontiguousarray(point, dtype=np.uint32).reshape(-1)
+    n = pt.size // 8
+    if evals.size != (1 << n) * (8 if is_ef else 1):
+        raise ValueError("2^n evaluations for an n-coordinate point")
+    out = np.zeros(8, dtype=np.uint32)
+    check(lib().wl_mle_evaluate(ctx.ptr, u32ptr(evals), int(is_ef), n, u32ptr(pt), u32ptr(out)))
     return out
 
 
