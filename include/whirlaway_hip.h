@@ -268,6 +268,19 @@ int wl_multilinears_linear_combination(wl_ctx *ctx, const uint32_t *cols, uint32
 int wl_mle_evaluate(wl_ctx *ctx, const uint32_t *evals, int is_ef, uint32_t n_vars, const uint32_t *point,
                     uint32_t *out);
 
+/* No counterpart in the reference: wl_mle_evaluate for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc).  evals_dev (2^n_vars canonical F, or EF when
+ * is_ef) and out_dev (8 canonical words) are device memory the caller owns; the
+ * point (n_vars EF) is host memory, passed in the kernel arguments.  The call
+ * allocates nothing: its scratch (eq tables and block partials) is a fixed
+ * 1,081,344-byte device array per GPU, held until the process ends, so n_vars is
+ * at most 28 (WL_ERR_ARG beyond).  Runs on ctx's stream and returns when the result
+ * is in out_dev.  Calls that overlap on one GPU (other threads or contexts) share
+ * that array and race. */
+int wl_mle_evaluate_noalloc(wl_ctx *ctx, const uint32_t *evals_dev, int is_ef, uint32_t n_vars,
+                            const uint32_t *point, uint32_t *out_dev);
+
 /* Kernel micro (a), benches/folding_benchmarks.rs: fold 2^log_n base-field
  * evaluations by n_folds successive EF challenges (1 - c, c), most-significant
  * variable first; after each fold the intermediate table is checked by the

@@ -53,6 +53,7 @@ def lib():
         L.wl_fold_small.argtypes = [vp, u32p, ctypes.c_int, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p]
         L.wl_multilinears_linear_combination.argtypes = [vp, u32p, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p]
         L.wl_mle_evaluate.argtypes = [vp, u32p, ctypes.c_int, ctypes.c_uint32, u32p, u32p]
+        L.wl_mle_evaluate_noalloc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_uint32, u32p, vp]
         L.wl_fold_chain.argtypes = [vp, u32p, ctypes.c_uint32, u32p, ctypes.c_uint32, u32p, u32p, dp]
         L.wl_sumcheck_a2b.argtypes = [vp, u32p, u32p, u32p, ctypes.c_uint32, u32p, u32p, u32p, u32p,
                                       u32p, ctypes.c_size_t, szp, dp]

@@ -10,6 +10,7 @@
 // A block walks whole rows hi, so each lane multiplies its row sum by eq_hi once;
 // the input is read once, in canonical form, and converted on load.
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include "../../include/whirlaway_hip.h"
 #include "ctx.h"
 
@@ -144,6 +145,30 @@ __global__ void k_fold_small(const F *in, size_t M, const F *sc, int K, F *out) 
     }
 }
 
+// Scratch of wl_mle_evaluate_noalloc: eq_hi, eq_lo and the block partials of the
+// largest evaluation (28 variables), one copy per device for the whole process.
+// Host code uses the device address that hipGetSymbolAddress returns for the
+// current device, never the name.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (((size_t)1 << MAX_HALF) * 2 + DOT_BLOCKS) * sizeof(EF);
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES];  // per device, set on the first call there
+
+// device address of this device's scratch (the device must be current)
+void *device_scratch(int device) {
+    if (device < 0 || device >= MAX_DEVICES) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("no device address for the evaluation scratch");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
 }  // namespace wu
 
 namespace {
@@ -226,6 +251,26 @@ int wl_mle_evaluate(wl_ctx *c, const uint32_t *evals, int is_ef, uint32_t n_vars
         wu::mle_eval_enqueue(din.p, is_ef != 0, (int)n_vars, pt.data(), scratch.p, dout.p, c->stream);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(out, dout.p, 32, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        return WL_OK;
+    });
+}
+
+int wl_mle_evaluate_noalloc(wl_ctx *c, const uint32_t *evals_dev, int is_ef, uint32_t n_vars, const uint32_t *point,
+                            uint32_t *out_dev) {
+    if (!evals_dev || !out_dev || (n_vars && !point)) { wl_set_error("bad arguments"); return WL_ERR_ARG; }
+    if (n_vars > (uint32_t)wu::MAX_VARS || wu::scratch_bytes((int)n_vars) > wu::NOALLOC_SCRATCH_BYTES) {
+        wl_set_error("wl_mle_evaluate_noalloc: at most 28 variables (its scratch holds 2 x 2^14 + 1024 EF)");
+        return WL_ERR_ARG;
+    }
+    return guard(c, [&]() {
+        EF pt[wu::MAX_VARS];  // the point travels in the kernel arguments
+        for (uint32_t i = 0; i < n_vars; i++) pt[i] = wh::to_mont_ef(point + 8 * (size_t)i);
+        void *scr = wu::device_scratch(c->device);
+        // every scratch entry that is read is written first in the same call, so
+        // the previous call's contents need no clearing
+        wu::mle_eval_enqueue(evals_dev, is_ef != 0, (int)n_vars, pt, scr, out_dev, c->stream);
+        HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(c->stream));
         return WL_OK;
     });
