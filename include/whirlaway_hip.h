@@ -290,6 +290,39 @@ int wl_mle_evaluate_noalloc(wl_ctx *ctx, const uint32_t *evals_dev, int is_ef, u
 int wl_fold_chain(wl_ctx *ctx, const uint32_t *in, uint32_t log_n, const uint32_t *challenges,
                   uint32_t n_folds, uint32_t *out_all, uint32_t *final_out, double *ms);
 
+/* ------------------------------------------------------ generic sumcheck --
+ * A SumcheckComputation (crates/sumcheck/src/sc_computation.rs:8-36) given as a
+ * constraint program -- the symbolic constraints of an AIR
+ * (get_symbolic_constraints, crates/air/src/table.rs:42) or any polynomial of the
+ * point -- in straight-line SSA form: 4 words per instruction {op, a, b, 0}.
+ * Value i < n_inputs is point[i] (the i-th multilinear; for an AIR, local column
+ * i for i < width, next-row column i - width above); instruction j defines value
+ * n_inputs + j, except WL_OP_ASSERT, which defines none and adds
+ * batching[k] * value(a) for its k-th occurrence (ConstraintFolder::assert_zero,
+ * crates/utils/src/constraints_folder.rs:55-60).  WL_OP_CONST: a is a canonical
+ * constant.  At most 256 values may be live at once. */
+enum { WL_OP_CONST = 0, WL_OP_ADD = 1, WL_OP_SUB = 2, WL_OP_MUL = 3, WL_OP_NEG = 4, WL_OP_ASSERT = 5 };
+/* SumcheckGrinding (crates/sumcheck/src/lib.rs:14-31): NONE, AUTO (value = security
+ * bits), CUSTOM (value = pow bits) */
+enum { WL_GRIND_NONE = 0, WL_GRIND_AUTO = 1, WL_GRIND_CUSTOM = 2 };
+
+/* sumcheck::prove(skips, multilinears, computation, degree, batching_scalars,
+ * eq_factor, is_zerofier, prover_state, sum, n_rounds, grinding,
+ * missing_mul_factor) (crates/sumcheck/src/prove.rs:22-93, rounds :97-254) on the
+ * device.  mles: n_mles tables of 2^n_vars values, contiguous, F (or EF when
+ * mles_are_ef); the program reads n_mles inputs.  batching: n_batching EF (at
+ * least one per assert).  eq_factor: n_vars - skips + 1 EF, or NULL.  sum: EF.
+ * n_rounds: 0 for all (n_vars - skips + 1).  missing_mul_factor: EF or NULL.
+ * Round polynomials (coefficients) and grinding witnesses are appended to ts.
+ * Outputs (any may be NULL): challenges (n_rounds EF), folded (the n_mles tables
+ * after the last round, 2^(n_vars - skips - n_rounds + 1) EF each), the final sum. */
+int wl_sumcheck_prove(wl_ctx *ctx, wl_transcript *ts, uint32_t skips, const uint32_t *mles, uint32_t n_mles,
+                      int mles_are_ef, uint32_t n_vars, const uint32_t *program, uint32_t n_instr, uint32_t degree,
+                      const uint32_t *batching, uint32_t n_batching, const uint32_t *eq_factor, int is_zerofier,
+                      const uint32_t *sum, uint32_t n_rounds, int grinding, uint32_t grinding_value,
+                      const uint32_t *missing_mul_factor, uint32_t *challenges_out, uint32_t *folded_out,
+                      uint32_t *sum_out);
+
 /* Kernel micro (b): sumcheck::prove(1, [a, b], a^2 * b, degree 3, [1],
  * Some(z), false, ...) (crates/sumcheck/src/prove.rs:22-93) over a transcript
  * whose challenger is Poseidon2KoalaBear<16> with constants p16 and an empty

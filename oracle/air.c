@@ -170,7 +170,7 @@ void or_air_prove_ts(const or_air_settings_t *s, const or_p2_bench_t *b, or_ts_t
         mles[c] = up;
         mles[NC + c] = dn2;
     }
-    or_sc_comp_t comp = {OR_SC_AIR, &b->air, ap, NC};
+    or_sc_comp_t comp = {OR_SC_AIR, &b->air, ap, NC, NULL};
     ef_t *ch = (ef_t *)malloc((size_t)nz * sizeof(ef_t));
     ef_t fe[2 * NC], fsum;
     or_sc_prove(skips, 2 * NC, 1, mles, n, &comp, 3, zc, 1, ts, ef_zero(), sec, ch, fe, &fsum);
@@ -224,7 +224,7 @@ void or_air_prove_ts(const or_air_settings_t *s, const or_p2_bench_t *b, or_ts_t
     ef_t inner_sum = or_mle_eval_ef(mixed, n, pt);
     free(mixed);
     const void *imles[2] = {mu, batched};
-    or_sc_comp_t icomp = {OR_SC_PRODUCT, NULL, NULL, 0};
+    or_sc_comp_t icomp = {OR_SC_PRODUCT, NULL, NULL, 0, NULL};
     ef_t *ich = (ef_t *)malloc((size_t)n * sizeof(ef_t));
     ef_t ie[2], isum;
     or_sc_prove(1, 2, 0, imles, n, &icomp, 2, NULL, 0, ts, inner_sum, sec, ich, ie, &isum);
@@ -321,5 +321,244 @@ out:
     free(lde);
     free(fp);
     or_ts_free(&ts);
+    return rc;
+}
+
+/* ======================================================== generic AIR ======
+ * AirTable::prove / verify for any AIR whose constraints are a straight-line
+ * program over the point [up(0..C), down(0..C)] (SumcheckComputation for
+ * A: Air, crates/sumcheck/src/sc_computation.rs:17-36), with the first
+ * n_preprocessed columns preprocessed (crates/air/src/table.rs:18,60-66):
+ * they join the zerocheck (prove.rs:109-113) but are neither committed nor sent;
+ * the verifier evaluates them itself (verify.rs:114-136). */
+
+int or_gair_whir_config(or_whir_cfg_t *c, const or_air_settings_t *s, const or_gair_t *a) {
+    int v = a->log_n + (int)or_log2_up((size_t)(a->n_columns - a->n_preprocessed));
+    return or_whir_config(c, v, s->security_bits, WHIR_POW_BITS, s->folding_f0, s->folding_f,
+                          s->log_inv_rate, s->rs_reduction0, s->soundness);
+}
+
+void or_gair_prove(const or_air_settings_t *s, const or_gair_t *a, const f_t *witness, const or_p2_t *p16,
+                   const or_p2_t *p24, or_ts_t *ts) {
+    int n = a->log_n, skips = s->univariate_skips, sec = s->security_bits;
+    int C = a->n_columns, np = a->n_preprocessed, nw = C - np;
+    int ncons = or_prog_n_asserts(&a->prog);
+    size_t N = (size_t)1 << n;
+    int lc = (int)or_log2_up((size_t)nw);
+    or_whir_cfg_t cfg;
+    or_gair_whir_config(&cfg, s, a);
+    int v = n + lc;
+    f_t *packed = (f_t *)calloc((size_t)1 << v, sizeof(f_t));
+    memcpy(packed, witness, N * (size_t)nw * sizeof(f_t));
+    or_whir_witness_t w;
+    or_whir_commit(&w, &cfg, p16, p24, ts, packed);
+    free(packed);
+
+    or_ts_pow(ts, pow_bits_for(sec, (int)or_log2_up((size_t)ncons)));
+    ef_t alpha = or_ts_sample_ef(ts);
+    ef_t *ap = (ef_t *)malloc((size_t)(ncons ? ncons : 1) * sizeof(ef_t));
+    if (ncons) ap[0] = ef_one();
+    for (int i = 1; i < ncons; i++) ap[i] = ef_mul(ap[i - 1], alpha);
+    or_ts_pow(ts, pow_bits_for(sec, n));
+    int nz = n + 1 - skips;
+    ef_t *zc = (ef_t *)malloc((size_t)nz * sizeof(ef_t));
+    for (int i = 0; i < nz; i++) zc[i] = or_ts_sample_ef(ts);
+
+    /* columns_up_and_down of preprocessed ++ witness (crates/air/src/utils.rs:140-160) */
+    f_t *ud = (f_t *)malloc(2 * (size_t)C * N * sizeof(f_t));
+    const void **mles = (const void **)malloc(2 * (size_t)C * sizeof(void *));
+    for (int c = 0; c < C; c++) {
+        const f_t *col = c < np ? a->preprocessed + (size_t)c * N : witness + (size_t)(c - np) * N;
+        f_t *up = ud + (size_t)c * N, *dn = ud + (size_t)(C + c) * N;
+        memcpy(up, col, N * sizeof(f_t));
+        up[N - 1] = col[N - 2];
+        memcpy(dn, col + 1, (N - 1) * sizeof(f_t));
+        dn[N - 1] = col[N - 1];
+        mles[c] = up;
+        mles[C + c] = dn;
+    }
+    or_sc_comp_t comp = {OR_SC_PROGRAM, NULL, ap, C, &a->prog};
+    ef_t *ch = (ef_t *)malloc((size_t)nz * sizeof(ef_t));
+    ef_t *fe = (ef_t *)malloc(2 * (size_t)C * sizeof(ef_t));
+    ef_t fsum;
+    or_sc_prove(skips, 2 * C, 1, mles, n, &comp, a->degree, zc, 1, ts, ef_zero(), sec, ch, fe, &fsum);
+    free(ud);
+    free(mles);
+
+    or_ts_add_ef(ts, fe + np, (size_t)nw);
+    or_ts_add_ef(ts, fe + C + np, (size_t)nw);
+    or_ts_pow(ts, pow_bits_for(sec, lc));
+    ef_t cbs[32];
+    for (int i = 0; i < lc; i++) cbs[i] = or_ts_sample_ef(ts);
+    ef_t *eqc = (ef_t *)malloc(((size_t)1 << lc) * sizeof(ef_t));
+    or_eval_eq(cbs, lc, eqc);
+    ef_t *batched = (ef_t *)malloc(N * sizeof(ef_t));
+#pragma omp parallel for schedule(static)
+    for (size_t x = 0; x < N; x++) {
+        ef_t acc = ef_zero();
+        for (int c = 0; c < nw; c++) acc = ef_add(acc, ef_mul_f(eqc[c], witness[(size_t)c * N + x]));
+        batched[x] = acc;
+    }
+    ef_t alpha2 = or_ts_sample_ef(ts);
+    ef_t *mixed = (ef_t *)malloc(N * sizeof(ef_t));
+    for (size_t x = 0; x < N; x++) {
+        ef_t up = x == N - 1 ? batched[N - 2] : batched[x];
+        ef_t dn = x == N - 1 ? batched[N - 1] : batched[x + 1];
+        mixed[x] = ef_add(up, ef_mul(dn, alpha2));
+    }
+    int K = 1 << skips;
+    size_t M = N / (size_t)K;
+    ef_t *eqr = (ef_t *)malloc(M * sizeof(ef_t));
+    or_eval_eq(ch + 1, n - skips, eqr);
+    ef_t sub[64];
+    for (int j = 0; j < K; j++) {
+        ef_t acc = ef_zero();
+        for (size_t i = 0; i < M; i++) acc = ef_add(acc, ef_mul(mixed[(size_t)j * M + i], eqr[i]));
+        sub[j] = acc;
+    }
+    free(eqr);
+    or_ts_add_ef(ts, sub, (size_t)K);
+    ef_t *pt = (ef_t *)malloc((size_t)n * sizeof(ef_t));
+    for (int i = 0; i < skips; i++) pt[i] = or_ts_sample_ef(ts);
+    for (int i = 0; i < n - skips; i++) pt[skips + i] = ch[1 + i];
+    ef_t *mu = (ef_t *)malloc(N * sizeof(ef_t));
+    ef_t *md = (ef_t *)malloc(N * sizeof(ef_t));
+    matrix_up_folded(pt, n, mu);
+    matrix_down_folded(pt, n, md);
+    for (size_t x = 0; x < N; x++) mu[x] = ef_add(mu[x], ef_mul(md[x], alpha2));
+    free(md);
+    ef_t inner_sum = or_mle_eval_ef(mixed, n, pt);
+    free(mixed);
+    const void *imles[2] = {mu, batched};
+    or_sc_comp_t icomp = {OR_SC_PRODUCT, NULL, NULL, 0, NULL};
+    ef_t *ich = (ef_t *)malloc((size_t)n * sizeof(ef_t));
+    ef_t ie[2], isum;
+    or_sc_prove(1, 2, 0, imles, n, &icomp, 2, NULL, 0, ts, inner_sum, sec, ich, ie, &isum);
+    free(mu);
+    free(batched);
+    ef_t *fp = (ef_t *)malloc((size_t)v * sizeof(ef_t));
+    for (int i = 0; i < lc; i++) fp[i] = cbs[i];
+    for (int i = 0; i < n; i++) fp[lc + i] = ich[i];
+    or_whir_prove(&w, ts, fp, ie[1]);
+    or_whir_witness_free(&w);
+    free(fp);
+    free(ich);
+    free(pt);
+    free(ch);
+    free(fe);
+    free(zc);
+    free(eqc);
+    free(ap);
+}
+
+/* column_up / column_down of a preprocessed column folded by the outer selectors
+ * and evaluated at the rest of the outer point (crates/air/src/verify.rs:114-136) */
+static ef_t pre_eval(const f_t *col, int n, int skips, int down, const ef_t *osel, const ef_t *rest) {
+    size_t N = (size_t)1 << n, M = N >> skips;
+    int K = 1 << skips;
+    ef_t *folded = (ef_t *)malloc(M * sizeof(ef_t));
+    for (size_t i = 0; i < M; i++) {
+        ef_t acc = ef_zero();
+        for (int j = 0; j < K; j++) {
+            size_t x = (size_t)j * M + i;
+            f_t val = down ? (x == N - 1 ? col[N - 1] : col[x + 1]) : (x == N - 1 ? col[N - 2] : col[x]);
+            acc = ef_add(acc, ef_mul_f(osel[j], val));
+        }
+        folded[i] = acc;
+    }
+    ef_t r = or_mle_eval_ef(folded, n - skips, rest);
+    free(folded);
+    return r;
+}
+
+int or_gair_verify(const or_air_settings_t *s, const or_gair_t *a, const or_p2_t *p16, const or_p2_t *p24,
+                   or_ts_t *ts) {
+    int n = a->log_n, skips = s->univariate_skips, sec = s->security_bits;
+    int C = a->n_columns, np = a->n_preprocessed, nw = C - np;
+    int ncons = or_prog_n_asserts(&a->prog);
+    int lc = (int)or_log2_up((size_t)nw);
+    int K = 1 << skips;
+    or_whir_cfg_t cfg;
+    if (or_gair_whir_config(&cfg, s, a)) return 1;
+    int rc = 0;
+    ef_t *zc = NULL, *opt = NULL, *ipt = NULL, *lde = NULL, *fp = NULL, *wud = NULL, *gp = NULL, *ap = NULL,
+         *eqc = NULL;
+    or_whir_commitment_t cm;
+    if (or_whir_parse_commitment(&cfg, ts, &cm)) { rc = 1; goto out; }
+    if (or_ts_check_pow(ts, pow_bits_for(sec, (int)or_log2_up((size_t)ncons)))) { rc = 3; goto out; }
+    ef_t alpha = or_ts_sample_ef(ts);
+    if (or_ts_check_pow(ts, pow_bits_for(sec, n))) { rc = 3; goto out; }
+    int nz = n - skips + 1;
+    zc = (ef_t *)malloc((size_t)nz * sizeof(ef_t));
+    for (int i = 0; i < nz; i++) zc[i] = or_ts_sample_ef(ts);
+    ef_t scsum, oval;
+    opt = (ef_t *)malloc((size_t)nz * sizeof(ef_t));
+    if (or_sc_verify_uskip(ts, a->degree + 1, n, skips, sec, &scsum, opt, &oval)) { rc = 4; goto out; }
+    if (!ef_is_zero(scsum)) { rc = 6; goto out; }
+    wud = (ef_t *)malloc(2 * (size_t)nw * sizeof(ef_t));
+    if (or_ts_next_ef(ts, wud, 2 * (size_t)nw)) { rc = 3; goto out; }
+    ef_t osel[64], zsel[64];
+    or_selectors_at_ef(skips, opt[0], osel);
+    /* global point [pre_up, wit_up, pre_down, wit_down] */
+    gp = (ef_t *)malloc(2 * (size_t)C * sizeof(ef_t));
+    for (int c = 0; c < np; c++) {
+        const f_t *col = a->preprocessed + (size_t)c * ((size_t)1 << n);
+        gp[c] = pre_eval(col, n, skips, 0, osel, opt + 1);
+        gp[C + c] = pre_eval(col, n, skips, 1, osel, opt + 1);
+    }
+    for (int c = 0; c < nw; c++) {
+        gp[np + c] = wud[c];
+        gp[C + np + c] = wud[nw + c];
+    }
+    ap = (ef_t *)malloc((size_t)(ncons ? ncons : 1) * sizeof(ef_t));
+    if (ncons) ap[0] = ef_one();
+    for (int i = 1; i < ncons; i++) ap[i] = ef_mul(ap[i - 1], alpha);
+    ef_t gce = or_prog_eval_ef(&a->prog, gp, ap);
+    or_selectors_at_ef(skips, zc[0], zsel);
+    ef_t dot = ef_zero();
+    for (int i = 0; i < K; i++) dot = ef_add(dot, ef_mul(zsel[i], osel[i]));
+    ef_t lhs = ef_mul(ef_mul(dot, or_eq_poly_outside(zc + 1, opt + 1, n - skips)), gce);
+    if (!ef_eq(lhs, oval)) { rc = 6; goto out; }
+    if (or_ts_check_pow(ts, pow_bits_for(sec, lc))) { rc = 3; goto out; }
+    ef_t cbs[32];
+    for (int i = 0; i < lc; i++) cbs[i] = or_ts_sample_ef(ts);
+    ef_t alpha2 = or_ts_sample_ef(ts);
+    ef_t sub[64];
+    if (or_ts_next_ef(ts, sub, (size_t)K)) { rc = 3; goto out; }
+    eqc = (ef_t *)malloc(((size_t)1 << lc) * sizeof(ef_t));
+    or_eval_eq(cbs, lc, eqc);
+    ef_t l = ef_zero(), ru = ef_zero(), rd = ef_zero();
+    for (int i = 0; i < K; i++) l = ef_add(l, ef_mul(sub[i], osel[i]));
+    for (int c = 0; c < nw; c++) {
+        ru = ef_add(ru, ef_mul(wud[c], eqc[c]));
+        rd = ef_add(rd, ef_mul(wud[nw + c], eqc[c]));
+    }
+    if (!ef_eq(l, ef_add(ru, ef_mul(rd, alpha2)))) { rc = 6; goto out; }
+    ef_t eps[16];
+    for (int i = 0; i < skips; i++) eps[i] = or_ts_sample_ef(ts);
+    ef_t isum, ival;
+    ipt = (ef_t *)malloc((size_t)n * sizeof(ef_t));
+    if (or_sc_verify(ts, n, 2, sec, &isum, ipt, &ival)) { rc = 4; goto out; }
+    if (!ef_eq(isum, or_mle_eval_ef(sub, skips, eps))) { rc = 6; goto out; }
+    lde = (ef_t *)malloc((size_t)(2 * n) * sizeof(ef_t));
+    for (int i = 0; i < skips; i++) lde[i] = eps[i];
+    for (int i = 0; i < n - skips; i++) lde[skips + i] = opt[1 + i];
+    for (int i = 0; i < n; i++) lde[n + i] = ipt[i];
+    ef_t up = matrix_up_lde(lde, 2 * n), dn2 = matrix_down_lde(lde, 2 * n);
+    ef_t expected = ef_mul(ival, ef_inv(ef_add(up, ef_mul(alpha2, dn2))));
+    fp = (ef_t *)malloc((size_t)(lc + n) * sizeof(ef_t));
+    for (int i = 0; i < lc; i++) fp[i] = cbs[i];
+    for (int i = 0; i < n; i++) fp[lc + i] = ipt[i];
+    if (or_whir_verify(&cfg, p16, p24, ts, &cm, fp, expected)) { rc = 2; goto out; }
+out:
+    free(zc);
+    free(opt);
+    free(ipt);
+    free(lde);
+    free(fp);
+    free(wud);
+    free(gp);
+    free(ap);
+    free(eqc);
     return rc;
 }

@@ -204,7 +204,7 @@ size_t or_api_sc_a2b(void *h, const f_t *a, const f_t *b, int n, const f_t *z, f
     for (size_t x = 0; x < N; x++) sum = ef_add(sum, ef_mul_f(eq[x], f_mul(f_mul(a[x], a[x]), b[x])));
     free(eq);
     const void *mles[2] = {a, b};
-    or_sc_comp_t comp = {OR_SC_A2B, NULL, NULL, 0};
+    or_sc_comp_t comp = {OR_SC_A2B, NULL, NULL, 0, NULL};
     ef_t fs;
     or_sc_prove(1, 2, 1, mles, n, &comp, 3, (const ef_t *)z, 0, &ts, sum, 128,
                 (ef_t *)challenges, (ef_t *)final_evals, &fs);
@@ -268,4 +268,37 @@ size_t or_api_domsep(int log_n, int sec, int f0, int f, int rate, int skips, int
     or_whir_cfg_t cfg;
     if (or_air_whir_config(&cfg, &s, log_n)) return 0;
     return or_whir_domsep(&cfg, out, cap);
+}
+
+/* ---- generic sumcheck (crates/sumcheck/src/prove.rs:22-93) with a constraint
+ * program; mles contiguous (n_mles x 2^n_vars, F when base else EF) ---- */
+void or_api_sc_prove(void *h, int skips, const f_t *mles, int n_mles, int base, int n_vars, const uint32_t *code,
+                     int n_instr, int degree, const f_t *batching, const f_t *eq_factor, int is_zerofier,
+                     const f_t *sum, int n_rounds, int grind_kind, int grind_value, const f_t *missing,
+                     f_t *challenges, f_t *folded, f_t *final_sum) {
+    size_t N = (size_t)1 << n_vars, w = base ? 1 : 8;
+    const void **ptrs = (const void **)malloc((size_t)n_mles * sizeof(void *));
+    for (int m = 0; m < n_mles; m++) ptrs[m] = mles + (size_t)m * N * w;
+    or_prog_t prog = {n_mles, n_instr, code};
+    or_sc_comp_t comp = {OR_SC_PROGRAM, NULL, (const ef_t *)batching, 0, &prog};
+    ef_t s, fs;
+    memcpy(s.c, sum, 32);
+    ef_t mm;
+    if (missing) memcpy(mm.c, missing, 32);
+    or_sc_prove_ex(skips, n_mles, base, ptrs, n_vars, &comp, degree, (const ef_t *)eq_factor, is_zerofier,
+                   &((or_api_ts_t *)h)->ts, s, n_rounds, grind_kind, grind_value, missing ? &mm : NULL,
+                   (ef_t *)challenges, (ef_t *)folded, &fs);
+    memcpy(final_sum, fs.c, 32);
+    free(ptrs);
+}
+int or_api_prog_check(const uint32_t *code, int n_instr, int n_inputs) {
+    or_prog_t p = {n_inputs, n_instr, code};
+    return or_prog_check(&p);
+}
+void or_api_prog_eval(const uint32_t *code, int n_instr, int n_inputs, int base, const f_t *pt,
+                      const f_t *batching, f_t *out) {
+    or_prog_t p = {n_inputs, n_instr, code};
+    ef_t r = base ? or_prog_eval_f(&p, pt, (const ef_t *)batching)
+                  : or_prog_eval_ef(&p, (const ef_t *)pt, (const ef_t *)batching);
+    memcpy(out, r.c, 32);
 }

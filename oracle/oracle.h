@@ -170,14 +170,34 @@ void or_air_generate_trace(const or_air_consts_t *c, const f_t *inputs, int log_
 ef_t or_air_eval_f(const or_air_consts_t *c, const f_t *local, const ef_t *alpha_pows);
 ef_t or_air_eval_ef(const or_air_consts_t *c, const ef_t *local, const ef_t *alpha_pows);
 
+/* ------------------------------------------------------ constraint programs -- */
+/* A SumcheckComputation given as a straight-line program (the symbolic
+ * constraints of get_symbolic_constraints, crates/air/src/table.rs:42, in SSA
+ * form): value i < n_inputs is point[i]; instruction j (4 words: op, a, b, 0)
+ * defines value n_inputs + j.  ASSERT folds value a into the result with the next
+ * batching scalar (ConstraintFolder::assert_zero, crates/utils/src/constraints_folder.rs:55-60).
+ * CONST: a is the canonical constant. */
+enum { OR_OP_CONST = 0, OR_OP_ADD = 1, OR_OP_SUB = 2, OR_OP_MUL = 3, OR_OP_NEG = 4, OR_OP_ASSERT = 5 };
+typedef struct {
+    int n_inputs, n_instr;
+    const uint32_t *code;
+} or_prog_t;
+int or_prog_check(const or_prog_t *p); /* 0 when every operand refers to an earlier value */
+int or_prog_n_asserts(const or_prog_t *p);
+ef_t or_prog_eval_f(const or_prog_t *p, const f_t *pt, const ef_t *batching);
+ef_t or_prog_eval_ef(const or_prog_t *p, const ef_t *pt, const ef_t *batching);
+
 /* -------------------------------------------------------------- sumcheck -- */
-enum { OR_SC_AIR = 0, OR_SC_PRODUCT = 1, OR_SC_A2B = 2 };
+enum { OR_SC_AIR = 0, OR_SC_PRODUCT = 1, OR_SC_A2B = 2, OR_SC_PROGRAM = 3 };
 typedef struct {
     int kind;
     const or_air_consts_t *air;
-    const ef_t *alpha_pows;
+    const ef_t *alpha_pows;  /* batching scalars (AIR and PROGRAM) */
     int n_cols;  /* for AIR: columns per row (up) ; the point has 2*n_cols entries */
+    const or_prog_t *prog;   /* PROGRAM */
 } or_sc_comp_t;
+/* SumcheckGrinding (crates/sumcheck/src/lib.rs:14-31) */
+enum { OR_GRIND_NONE = 0, OR_GRIND_AUTO = 1, OR_GRIND_CUSTOM = 2 };
 
 /* crates/sumcheck/src/prove.rs:22-93.  mles: n_mles arrays (F if base_field else EF)
  * of 2^n_vars.  Outputs: challenges (n_rounds), final evals (n_mles EF), final sum. */
@@ -185,6 +205,14 @@ void or_sc_prove(int skips, int n_mles, int base_field, const void *const *mles,
                  const or_sc_comp_t *comp, int degree, const ef_t *eq_factor, int is_zerofier,
                  or_ts_t *ts, ef_t sum, int sec_bits, ef_t *challenges, ef_t *final_evals,
                  ef_t *final_sum);
+/* The full interface of crates/sumcheck/src/prove.rs:22-35: n_rounds (0 = all,
+ * n_vars - skips + 1), grinding (kind OR_GRIND_*, value = security bits for AUTO,
+ * bits for CUSTOM), missing_mul_factor (NULL = None).  folded: the n_mles tables
+ * after the last round, 2^(n_vars - skips - n_rounds + 1) EF each (may be NULL). */
+void or_sc_prove_ex(int skips, int n_mles, int base_field, const void *const *mles, int n_vars,
+                    const or_sc_comp_t *comp, int degree, const ef_t *eq_factor, int is_zerofier,
+                    or_ts_t *ts, ef_t sum, int n_rounds, int grind_kind, int grind_value,
+                    const ef_t *missing_mul_factor, ef_t *challenges, ef_t *folded, ef_t *final_sum);
 /* crates/sumcheck/src/verify.rs:23-113 ; returns 0 on success */
 int or_sc_verify(or_ts_t *ts, int n_vars, int degree, int sec_bits, ef_t *sum, ef_t *point,
                  ef_t *value);
@@ -277,6 +305,22 @@ void or_air_prove_ts(const or_air_settings_t *s, const or_p2_bench_t *b, or_ts_t
 /* crates/air/src/verify.rs:55-249 ; returns 0 on accept, error code otherwise */
 int or_air_verify(const or_air_settings_t *s, const or_p2_bench_t *b, const f_t *proof,
                   size_t len);
+
+/* Generic AIR (any A: Air given as a constraint program; crates/air/src/table.rs:32-56) */
+typedef struct {
+    int n_columns;            /* air.width(): preprocessed + witness columns */
+    int n_preprocessed;       /* the first columns of the point */
+    int log_n;
+    const f_t *preprocessed;  /* n_preprocessed columns x 2^log_n */
+    or_prog_t prog;           /* constraints over [up(0..C), down(0..C)] */
+    int degree;               /* constraint_degree */
+} or_gair_t;
+int or_gair_whir_config(or_whir_cfg_t *c, const or_air_settings_t *s, const or_gair_t *a);
+/* witness: (n_columns - n_preprocessed) columns x 2^log_n; appends to ts */
+void or_gair_prove(const or_air_settings_t *s, const or_gair_t *a, const f_t *witness, const or_p2_t *p16,
+                   const or_p2_t *p24, or_ts_t *ts);
+int or_gair_verify(const or_air_settings_t *s, const or_gair_t *a, const or_p2_t *p16, const or_p2_t *p24,
+                   or_ts_t *ts);
 
 /* whir config used by AirTable::build_whir_params (crates/air/src/table.rs:72-98) */
 int or_air_whir_config(or_whir_cfg_t *c, const or_air_settings_t *s, int log_n);

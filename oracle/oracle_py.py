@@ -87,6 +87,12 @@ def lib():
         L.or_api_prove_ts.argtypes = [vp] + [ctypes.c_int] * 7 + [vp]
         L.or_api_domsep.restype = szt
         L.or_api_domsep.argtypes = [ctypes.c_int] * 8 + [u32p, szt]
+        ci = ctypes.c_int
+        L.or_api_sc_prove.argtypes = [vp, ci, u32p, ci, ci, ci, u32p, ci, ci, u32p, u32p, ci, u32p, ci, ci, ci,
+                                      u32p, u32p, u32p, u32p]
+        L.or_api_prog_check.restype = ci
+        L.or_api_prog_check.argtypes = [u32p, ci, ci]
+        L.or_api_prog_eval.argtypes = [u32p, ci, ci, ci, u32p, u32p, u32p]
         L.or_ntt.argtypes = [u32p, ctypes.c_int]
         L.or_mobius.argtypes = [u32p, ctypes.c_int]
         L.or_chacha_block.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, u32p]
@@ -307,3 +313,34 @@ class Transcript:
         if n == 0:
             return np.zeros(0, dtype=np.uint32)
         return np.ctypeslib.as_array(lib().or_api_ts_data(self.h), shape=(n,)).copy()
+
+
+def sc_prove(ts, skips, mles, code, n_instr, degree, batching, eq_factor, is_zerofier, sum, n_rounds=0,
+             grinding=(1, 128), missing=None, is_ef=False):
+    """sumcheck::prove with a constraint program on the oracle; mles (n_mles, 2^n[, 8])."""
+    m = np.ascontiguousarray(mles, dtype=np.uint32)
+    n_mles, N = m.shape[0], m.shape[1]
+    n = N.bit_length() - 1
+    nr = n_rounds or (n - skips + 1)
+    ch = np.zeros(8 * nr, dtype=np.uint32)
+    fl = 1 << (n - skips - nr + 1)
+    folded = np.zeros(n_mles * fl * 8, dtype=np.uint32)
+    fs = np.zeros(8, dtype=np.uint32)
+    conv = lambda v: None if v is None else np.ascontiguousarray(v, dtype=np.uint32).reshape(-1)
+    lib().or_api_sc_prove(ts.h, skips, ptr(m.reshape(-1)), n_mles, 0 if is_ef else 1, n,
+                          ptr(np.ascontiguousarray(code, dtype=np.uint32)), n_instr, degree, ptr(conv(batching)),
+                          ptr(conv(eq_factor)), int(is_zerofier), ptr(conv(sum)), n_rounds, grinding[0], grinding[1],
+                          ptr(conv(missing)), ptr(ch), ptr(folded), ptr(fs))
+    return ch.reshape(nr, 8), folded.reshape(n_mles, fl, 8), fs
+
+
+def prog_check(code, n_instr, n_inputs):
+    return lib().or_api_prog_check(ptr(np.ascontiguousarray(code, dtype=np.uint32)), n_instr, n_inputs)
+
+
+def prog_eval(code, n_instr, n_inputs, point, batching, is_ef=False):
+    out = np.zeros(8, dtype=np.uint32)
+    lib().or_api_prog_eval(ptr(np.ascontiguousarray(code, dtype=np.uint32)), n_instr, n_inputs, 0 if is_ef else 1,
+                           ptr(np.ascontiguousarray(point, dtype=np.uint32).reshape(-1)),
+                           ptr(np.ascontiguousarray(batching, dtype=np.uint32).reshape(-1)), ptr(out))
+    return out

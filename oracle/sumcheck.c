@@ -24,11 +24,85 @@ static int sc_pow_bits(int sec_bits, int degree) {
     int r = sec_bits - t;
     return r < 0 ? 0 : r;
 }
+/* SumcheckGrinding::pow_bits for each mode */
+static int g_grind_kind = OR_GRIND_AUTO, g_grind_value = 128;
+static int grind_bits(int degree) {
+    if (g_grind_kind == OR_GRIND_NONE) return 0;
+    if (g_grind_kind == OR_GRIND_CUSTOM) return g_grind_value;
+    return sc_pow_bits(g_grind_value, degree);
+}
+
+/* ---------- constraint programs (straight-line SSA) ---------- */
+int or_prog_check(const or_prog_t *p) {
+    if (p->n_inputs < 0 || p->n_instr < 0) return -1;
+    for (int j = 0; j < p->n_instr; j++) {
+        const uint32_t *w = p->code + 4 * (size_t)j;
+        uint32_t lim = (uint32_t)(p->n_inputs + j);
+        int nops = (w[0] == OR_OP_ADD || w[0] == OR_OP_SUB || w[0] == OR_OP_MUL) ? 2 : w[0] == OR_OP_CONST ? 0 : 1;
+        if (w[0] > OR_OP_ASSERT) return -1;
+        if (w[0] == OR_OP_CONST && w[1] >= OR_P) return -1;
+        for (int o = 0; o < nops; o++) {
+            uint32_t v = w[1 + o];
+            if (v >= lim) return -1;
+            /* an ASSERT defines no value */
+            if (v >= (uint32_t)p->n_inputs && p->code[4 * (size_t)(v - p->n_inputs)] == OR_OP_ASSERT) return -1;
+        }
+    }
+    return 0;
+}
+int or_prog_n_asserts(const or_prog_t *p) {
+    int k = 0;
+    for (int j = 0; j < p->n_instr; j++) k += p->code[4 * (size_t)j] == OR_OP_ASSERT;
+    return k;
+}
+ef_t or_prog_eval_f(const or_prog_t *p, const f_t *pt, const ef_t *batching) {
+    f_t *v = (f_t *)malloc((size_t)(p->n_inputs + p->n_instr) * sizeof(f_t));
+    memcpy(v, pt, (size_t)p->n_inputs * sizeof(f_t));
+    ef_t acc = ef_zero();
+    int k = 0;
+    for (int j = 0; j < p->n_instr; j++) {
+        const uint32_t *w = p->code + 4 * (size_t)j;
+        f_t r = 0;
+        switch (w[0]) {
+        case OR_OP_CONST: r = w[1]; break;
+        case OR_OP_ADD: r = f_add(v[w[1]], v[w[2]]); break;
+        case OR_OP_SUB: r = f_sub(v[w[1]], v[w[2]]); break;
+        case OR_OP_MUL: r = f_mul(v[w[1]], v[w[2]]); break;
+        case OR_OP_NEG: r = f_neg(v[w[1]]); break;
+        default: r = v[w[1]]; acc = ef_add(acc, ef_mul_f(batching[k++], r)); break;
+        }
+        v[p->n_inputs + j] = r;
+    }
+    free(v);
+    return acc;
+}
+ef_t or_prog_eval_ef(const or_prog_t *p, const ef_t *pt, const ef_t *batching) {
+    ef_t *v = (ef_t *)malloc((size_t)(p->n_inputs + p->n_instr) * sizeof(ef_t));
+    memcpy(v, pt, (size_t)p->n_inputs * sizeof(ef_t));
+    ef_t acc = ef_zero();
+    int k = 0;
+    for (int j = 0; j < p->n_instr; j++) {
+        const uint32_t *w = p->code + 4 * (size_t)j;
+        ef_t r;
+        switch (w[0]) {
+        case OR_OP_CONST: r = ef_from_f(w[1]); break;
+        case OR_OP_ADD: r = ef_add(v[w[1]], v[w[2]]); break;
+        case OR_OP_SUB: r = ef_sub(v[w[1]], v[w[2]]); break;
+        case OR_OP_MUL: r = ef_mul(v[w[1]], v[w[2]]); break;
+        case OR_OP_NEG: r = ef_neg(v[w[1]]); break;
+        default: r = v[w[1]]; acc = ef_add(acc, ef_mul(batching[k++], r)); break;
+        }
+        v[p->n_inputs + j] = r;
+    }
+    free(v);
+    return acc;
+}
 
 /* ---------- point evaluation (SumcheckComputation::eval) ---------- */
 static ef_t comp_eval_f(const or_sc_comp_t *c, const f_t *pt) {
     switch (c->kind) {
     case OR_SC_AIR: return or_air_eval_f(c->air, pt, c->alpha_pows);
+    case OR_SC_PROGRAM: return or_prog_eval_f(c->prog, pt, c->alpha_pows);
     case OR_SC_PRODUCT: return ef_from_f(f_mul(pt[0], pt[1]));
     default: return ef_from_f(f_mul(f_mul(pt[0], pt[0]), pt[1]));
     }
@@ -36,6 +110,7 @@ static ef_t comp_eval_f(const or_sc_comp_t *c, const f_t *pt) {
 static ef_t comp_eval_ef(const or_sc_comp_t *c, const ef_t *pt) {
     switch (c->kind) {
     case OR_SC_AIR: return or_air_eval_ef(c->air, pt, c->alpha_pows);
+    case OR_SC_PROGRAM: return or_prog_eval_ef(c->prog, pt, c->alpha_pows);
     case OR_SC_PRODUCT: return ef_mul(pt[0], pt[1]);
     default: return ef_mul(ef_square(pt[0]), pt[1]);
     }
@@ -212,7 +287,8 @@ static ef_t **sc_round(int skips, int n_mles, int base, void **mles, int *n_vars
     challenges[round] = c;
     *sum = or_poly_eval(p, plen, c);
     *n_vars -= skips;
-    or_ts_pow(ts, sc_pow_bits(sec_bits, (degree + (eq_factor ? 1 : 0)) * (K - 1)));
+    (void)sec_bits;
+    or_ts_pow(ts, grind_bits((degree + (eq_factor ? 1 : 0)) * (K - 1)));
 
     ef_t *fs = (ef_t *)malloc((size_t)K * sizeof(ef_t));
     if (skips == 1) { fs[0] = ef_sub(ef_one(), c); fs[1] = c; }
@@ -242,30 +318,42 @@ static ef_t **sc_round(int skips, int n_mles, int base, void **mles, int *n_vars
     return out;
 }
 
-void or_sc_prove(int skips, int n_mles, int base_field, const void *const *mles, int n_vars,
-                 const or_sc_comp_t *comp, int degree, const ef_t *eq_factor, int is_zerofier,
-                 or_ts_t *ts, ef_t sum, int sec_bits, ef_t *challenges, ef_t *final_evals,
-                 ef_t *final_sum) {
-    int n_rounds = n_vars - skips + 1;
-    ef_t missing = ef_zero();
-    int has_missing = 0;
+void or_sc_prove_ex(int skips, int n_mles, int base_field, const void *const *mles, int n_vars,
+                    const or_sc_comp_t *comp, int degree, const ef_t *eq_factor, int is_zerofier,
+                    or_ts_t *ts, ef_t sum, int n_rounds, int grind_kind, int grind_value,
+                    const ef_t *missing_mul_factor, ef_t *challenges, ef_t *folded, ef_t *final_sum) {
+    if (n_rounds <= 0) n_rounds = n_vars - skips + 1;
+    g_grind_kind = grind_kind;
+    g_grind_value = grind_value;
+    ef_t missing = missing_mul_factor ? *missing_mul_factor : ef_zero();
+    int has_missing = missing_mul_factor != NULL;
     int nv = n_vars;
     ef_t **cur = sc_round(skips, n_mles, base_field, (void **)mles, &nv, comp, eq_factor,
-                          is_zerofier, ts, degree, &sum, sec_bits, challenges, 0, &missing,
+                          is_zerofier, ts, degree, &sum, 0, challenges, 0, &missing,
                           &has_missing);
     for (int i = 1; i < n_rounds; i++) {
         ef_t **nxt = sc_round(1, n_mles, 0, (void **)cur, &nv, comp, eq_factor, 0, ts, degree,
-                              &sum, sec_bits, challenges, i, &missing, &has_missing);
+                              &sum, 0, challenges, i, &missing, &has_missing);
         for (int m = 0; m < n_mles; m++) free(cur[m]);
         free(cur);
         cur = nxt;
     }
+    size_t len = (size_t)1 << nv;
     for (int m = 0; m < n_mles; m++) {
-        if (final_evals) final_evals[m] = cur[m][0];
+        if (folded) memcpy(folded + (size_t)m * len, cur[m], len * sizeof(ef_t));
         free(cur[m]);
     }
     free(cur);
     *final_sum = sum;
+    g_grind_kind = OR_GRIND_AUTO;
+}
+
+void or_sc_prove(int skips, int n_mles, int base_field, const void *const *mles, int n_vars,
+                 const or_sc_comp_t *comp, int degree, const ef_t *eq_factor, int is_zerofier,
+                 or_ts_t *ts, ef_t sum, int sec_bits, ef_t *challenges, ef_t *final_evals,
+                 ef_t *final_sum) {
+    or_sc_prove_ex(skips, n_mles, base_field, mles, n_vars, comp, degree, eq_factor, is_zerofier, ts,
+                   sum, 0, OR_GRIND_AUTO, sec_bits, NULL, challenges, final_evals, final_sum);
 }
 
 /* ---------------- verifier (crates/sumcheck/src/verify.rs) ---------------- */

@@ -98,6 +98,9 @@ def lib():
         L.wl_transcript_check_pow.argtypes = [vp, ctypes.c_uint32]
         L.wl_transcript_remaining.restype = ctypes.c_size_t
         L.wl_transcript_remaining.argtypes = [vp]
+        ui = ctypes.c_uint32
+        L.wl_sumcheck_prove.argtypes = [vp, vp, ui, u32p, ui, ctypes.c_int, ui, u32p, ui, ui, u32p, ui, u32p,
+                                        ctypes.c_int, u32p, ui, ctypes.c_int, ui, u32p, u32p, u32p, u32p]
         L.wl_kernel_timing.argtypes = [vp, ctypes.c_uint32]
         L.wl_kernel_timing_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), dp, dp]
         _lib = L
@@ -118,4 +121,4 @@ def check(rc):
 
 
 from .air import AirSettings, AirTable, Context  # noqa: E402
-from . import engine, sumcheck, transcript, utils  # noqa: E402,F401
+from . import engine, program, sumcheck, transcript, utils  # noqa: E402,F401

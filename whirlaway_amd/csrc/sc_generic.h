@@ -1,0 +1,58 @@
+// sc_generic.h -- the sumcheck of crates/sumcheck/src/prove.rs:22-254 for any
+// computation given as a constraint program (host interface of sc_generic.hip).
+//
+// A program is straight-line SSA over the point (the values of the n_inputs
+// multilinears at one hypercube point): 4 words per instruction {op, a, b, 0};
+// value i < n_inputs is input i, instruction j defines value n_inputs + j (except
+// ASSERT, which defines none); CONST takes the canonical constant in a; ASSERT
+// adds batching[k] * value(a) to the result for its k-th occurrence
+// (ConstraintFolder::assert_zero, crates/utils/src/constraints_folder.rs:55-60).
+// The host compiles it to slot code for the device: operands read from a small
+// register file, inputs loaded where first used, slots reused once their value
+// is dead.
+#pragma once
+#include <vector>
+#include "ctx.h"
+
+namespace wg {
+using namespace kb;
+
+enum Op : uint32_t { OP_CONST = 0, OP_ADD = 1, OP_SUB = 2, OP_MUL = 3, OP_NEG = 4, OP_ASSERT = 5, OP_LOAD = 6 };
+
+struct Program {
+    int n_inputs = 0;
+    int n_asserts = 0;
+    int n_slots = 0;                 // register-file size the slot code needs
+    std::vector<uint32_t> dev;       // slot code, 2 words per instruction
+    wl::DBuf<uint32_t> dev_buf;      // dev on the device (after upload)
+};
+
+// Validates the SSA program and compiles it; throws std::runtime_error on a
+// malformed program (operand out of range or referring to an ASSERT, unknown op).
+void compile(const uint32_t *code, size_t n_instr, int n_inputs, Program &out);
+void upload(wl_ctx *c, Program &p);
+
+// SumcheckGrinding (crates/sumcheck/src/lib.rs:14-31)
+enum Grinding { GRIND_NONE = 0, GRIND_AUTO = 1, GRIND_CUSTOM = 2 };
+
+struct Args {
+    int skips = 1;
+    int degree = 1;                          // constraints_degree
+    const std::vector<EF> *batching = nullptr;
+    const std::vector<EF> *eq_factor = nullptr;  // n_vars - skips + 1 entries, or none
+    bool is_zerofier = false;
+    EF sum{};
+    int n_rounds = 0;                        // 0: n_vars - skips + 1
+    int grind_kind = GRIND_AUTO, grind_value = 128;
+    const EF *missing_mul_factor = nullptr;
+};
+
+// sumcheck::prove on n_mles device tables of 2^n_vars entries, contiguous
+// ([m][x], F when base, else EF, Montgomery form).  Appends to ts (grinding on
+// the device through pdev's Poseidon2-16).  Returns the challenges; `folded`
+// receives the n_mles EF tables after the last round (2^(n_vars - skips -
+// n_rounds + 1) entries each, contiguous) and `sum` the final claim.
+std::vector<EF> prove(wl_ctx *c, wh::Transcript &ts, const wl::P2Dev *pdev, const void *tables, bool base,
+                      int n_mles, int n_vars, const Program &prog, const Args &a, wl::DBuf<EF> &folded, EF &sum);
+
+}  // namespace wg
