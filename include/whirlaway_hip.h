@@ -74,6 +74,18 @@ wl_table *wl_air_table_new(wl_ctx *ctx, const uint32_t *round_constants, uint32_
                            uint32_t univariate_skips);
 void wl_air_table_free(wl_table *table);
 
+/* AirTable::new(air, log_length, univariate_skips, preprocessed_columns,
+ * constraint_degree) (crates/air/src/table.rs:32-56) for any AIR: width
+ * n_columns, the first n_preprocessed of them preprocessed (n_preprocessed x
+ * 2^log_length canonical, column-major: prove.rs:109-113 chains them before the
+ * witness), constraints as a program over [local row, next row] (2 n_columns
+ * inputs; see wl_sumcheck_prove), the i-th assert receiving alpha^i.  Such a table
+ * proves with wl_air_prove / wl_air_prove_into on a witness of n_columns -
+ * n_preprocessed columns and verifies with wl_air_verify. */
+wl_table *wl_air_table_new_program(wl_ctx *ctx, uint32_t n_columns, uint32_t n_preprocessed,
+                                   const uint32_t *preprocessed, uint32_t log_length, uint32_t univariate_skips,
+                                   const uint32_t *program, uint32_t n_instr, uint32_t constraint_degree);
+
 /* AirTable::prove (crates/air/src/prove.rs:49-226) including the transcript set
  * up by DomainSeparator/to_prover_state (src/examples/poseidon2.rs:155-163).
  * p16/p24: Poseidon2KoalaBear<16>/<24> constants (canonical): ext_init[4][W],

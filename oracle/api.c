@@ -302,3 +302,66 @@ void or_api_prog_eval(const uint32_t *code, int n_instr, int n_inputs, int base,
                   : or_prog_eval_ef(&p, (const ef_t *)pt, (const ef_t *)batching);
     memcpy(out, r.c, 32);
 }
+
+/* ---- generic AIR (crates/air/src/table.rs:32-56, prove.rs, verify.rs) ---- */
+static void p24_from_packed(or_p2_t *p, const f_t *c) {
+    memset(p, 0, sizeof *p);
+    p->width = 24;
+    p->rounds_f = 8;
+    p->rounds_p = 23;
+    int k = 0;
+    for (int r = 0; r < 4; r++) for (int i = 0; i < 24; i++) p->ext_init[r][i] = c[k++];
+    for (int r = 0; r < 4; r++) for (int i = 0; i < 24; i++) p->ext_term[r][i] = c[k++];
+    for (int r = 0; r < 23; r++) p->internal[r] = c[k++];
+}
+static void gair(or_gair_t *a, int C, int np, int log_n, const f_t *pre, const uint32_t *code, int n_instr,
+                 int degree) {
+    a->n_columns = C;
+    a->n_preprocessed = np;
+    a->log_n = log_n;
+    a->preprocessed = pre;
+    a->prog.n_inputs = 2 * C;
+    a->prog.n_instr = n_instr;
+    a->prog.code = code;
+    a->degree = degree;
+}
+/* appends the proof to transcript handle h (whose Poseidon2-16 is p16c) */
+void or_api_gair_prove(int C, int np, int log_n, const f_t *pre, const uint32_t *code, int n_instr, int degree,
+                       int sec, int f0, int f, int rate, int skips, int rs0, int sa, const f_t *witness,
+                       const f_t *p16c, const f_t *p24c, void *h) {
+    or_gair_t a;
+    gair(&a, C, np, log_n, pre, code, n_instr, degree);
+    or_air_settings_t s;
+    settings(&s, sec, f0, f, rate, skips, rs0, sa);
+    or_p2_t p16, p24;
+    p16_from_packed(&p16, p16c);
+    p24_from_packed(&p24, p24c);
+    or_gair_prove(&s, &a, witness, &p16, &p24, &((or_api_ts_t *)h)->ts);
+}
+/* verifies proof against the transcript that first observes domsep; 0 on accept */
+int or_api_gair_verify(int C, int np, int log_n, const f_t *pre, const uint32_t *code, int n_instr, int degree,
+                       int sec, int f0, int f, int rate, int skips, int rs0, int sa, const f_t *p16c,
+                       const f_t *p24c, const f_t *domsep, size_t dn, const f_t *proof, size_t len) {
+    or_gair_t a;
+    gair(&a, C, np, log_n, pre, code, n_instr, degree);
+    or_air_settings_t s;
+    settings(&s, sec, f0, f, rate, skips, rs0, sa);
+    or_p2_t p16, p24;
+    p16_from_packed(&p16, p16c);
+    p24_from_packed(&p24, p24c);
+    or_ts_t ts;
+    or_ts_init_verifier(&ts, &p16, domsep, dn, proof, len);
+    int rc = or_gair_verify(&s, &a, &p16, &p24, &ts);
+    if (!rc && ts.pos != ts.len) rc = 7; /* trailing words */
+    or_ts_free(&ts);
+    return rc;
+}
+size_t or_api_gair_domsep(int nw, int log_n, int sec, int f0, int f, int rate, int skips, int rs0, int sa,
+                          f_t *out, size_t cap) {
+    or_air_settings_t s;
+    settings(&s, sec, f0, f, rate, skips, rs0, sa);
+    or_whir_cfg_t cfg;
+    int v = log_n + (int)or_log2_up((size_t)nw);
+    if (or_whir_config(&cfg, v, sec, 16, f0, f, rate, rs0, sa)) return 0;
+    return or_whir_domsep(&cfg, out, cap);
+}

@@ -93,6 +93,12 @@ def lib():
         L.or_api_prog_check.restype = ci
         L.or_api_prog_check.argtypes = [u32p, ci, ci]
         L.or_api_prog_eval.argtypes = [u32p, ci, ci, ci, u32p, u32p, u32p]
+        L.or_api_gair_prove.argtypes = [ci, ci, ci, u32p, u32p, ci, ci] + [ci] * 7 + [u32p, u32p, u32p, vp]
+        L.or_api_gair_verify.restype = ci
+        L.or_api_gair_verify.argtypes = [ci, ci, ci, u32p, u32p, ci, ci] + [ci] * 7 + [u32p, u32p, u32p, szt, u32p,
+                                                                                        szt]
+        L.or_api_gair_domsep.restype = szt
+        L.or_api_gair_domsep.argtypes = [ci] * 9 + [u32p, szt]
         L.or_ntt.argtypes = [u32p, ctypes.c_int]
         L.or_mobius.argtypes = [u32p, ctypes.c_int]
         L.or_chacha_block.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, u32p]
@@ -344,3 +350,42 @@ def prog_eval(code, n_instr, n_inputs, point, batching, is_ef=False):
                            ptr(np.ascontiguousarray(point, dtype=np.uint32).reshape(-1)),
                            ptr(np.ascontiguousarray(batching, dtype=np.uint32).reshape(-1)), ptr(out))
     return out
+
+
+class GenericAir:
+    """AirTable for an AIR given as a constraint program (oracle): width C, the first
+    n_pre columns preprocessed (pre: n_pre x 2^log_n), code over [local, next]."""
+
+    def __init__(self, C, n_pre, log_n, pre, code, n_instr, degree):
+        self.C, self.n_pre, self.log_n, self.n_instr, self.degree = C, n_pre, log_n, n_instr, degree
+        self.pre = np.ascontiguousarray(pre if pre is not None else np.zeros(1, np.uint32), dtype=np.uint32)
+        self.code = np.ascontiguousarray(code, dtype=np.uint32)
+
+    def _args(self, kw):
+        s = dict(DEFAULT, **kw)
+        return (self.C, self.n_pre, self.log_n, ptr(self.pre.reshape(-1)), ptr(self.code), self.n_instr, self.degree,
+                s["sec"], s["f0"], s["f"], s["rate"], s["skips"], s["rs0"], s["sa"])
+
+    def domain_separator(self, **kw):
+        s = dict(DEFAULT, **kw)
+        out = np.zeros(4096, dtype=np.uint32)
+        k = lib().or_api_gair_domsep(self.C - self.n_pre, self.log_n, s["sec"], s["f0"], s["f"], s["rate"],
+                                     s["skips"], s["rs0"], s["sa"], ptr(out), out.size)
+        return out[:k].copy()
+
+    def prove_into(self, ts, witness, p16, p24, **kw):
+        w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1)
+        lib().or_api_gair_prove(*self._args(kw), ptr(w), ptr(np.ascontiguousarray(p16, dtype=np.uint32)),
+                                ptr(np.ascontiguousarray(p24, dtype=np.uint32)), ts.h)
+
+    def prove(self, witness, p16, p24, **kw):
+        ts = Transcript(p16, self.domain_separator(**kw))
+        self.prove_into(ts, witness, p16, p24, **kw)
+        return ts.proof_data()
+
+    def verify(self, proof, p16, p24, domsep=None, **kw):
+        d = self.domain_separator(**kw) if domsep is None else np.ascontiguousarray(domsep, dtype=np.uint32)
+        pr = np.ascontiguousarray(proof, dtype=np.uint32)
+        return lib().or_api_gair_verify(*self._args(kw), ptr(np.ascontiguousarray(p16, dtype=np.uint32)),
+                                        ptr(np.ascontiguousarray(p24, dtype=np.uint32)), ptr(d), d.size, ptr(pr),
+                                        pr.size)

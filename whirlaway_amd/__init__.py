@@ -43,6 +43,9 @@ def lib():
         L.wl_air_table_new.restype = vp
         L.wl_air_table_new.argtypes = [vp, u32p, ctypes.c_uint32, ctypes.c_uint32]
         L.wl_air_table_free.argtypes = [vp]
+        L.wl_air_table_new_program.restype = vp
+        L.wl_air_table_new_program.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_uint32,
+                                               ctypes.c_uint32, u32p, ctypes.c_uint32, ctypes.c_uint32]
         L.wl_air_prove.argtypes = [vp, vp, u32p, u32p, u32p, u32p, ctypes.c_size_t, szp]
         L.wl_witness_upload.restype = vp
         L.wl_witness_upload.argtypes = [vp, u32p, ctypes.c_uint32]

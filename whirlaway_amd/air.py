@@ -168,6 +168,61 @@ class AirTable:
                 self._out_lock.release()
 
 
+class ProgramAirTable:
+    """AirTable::<F, EF, A>::new(air, log_length, univariate_skips, preprocessed_columns,
+    constraint_degree) (crates/air/src/table.rs:32-56) for an AIR given as a
+    ConstraintProgram over [local row, next row] (whirlaway_amd.program)."""
+
+    def __init__(self, ctx, program, log_length, constraint_degree, preprocessed=None, univariate_skips=4):
+        pre = None if preprocessed is None else np.ascontiguousarray(preprocessed, dtype=np.uint32)
+        n_pre = 0 if pre is None else pre.reshape(-1, 1 << log_length).shape[0]
+        if pre is not None and pre.size != n_pre << log_length:
+            raise ValueError("preprocessed columns hold 2^log_length values each")
+        self.ctx = ctx
+        self.log_length = log_length
+        self.n_columns = program.width
+        self.n_preprocessed = n_pre
+        self.n_witness = self.n_columns - n_pre
+        self._code = np.ascontiguousarray(program.code(), dtype=np.uint32)
+        self.ptr = lib().wl_air_table_new_program(ctx.ptr, self.n_columns, n_pre, u32ptr(pre), log_length,
+                                                  univariate_skips, u32ptr(self._code), program.n_instr,
+                                                  constraint_degree)
+        if not self.ptr:
+            raise WhirlawayError(lib().wl_last_error().decode())
+
+    def close(self):
+        if self.ptr:
+            lib().wl_air_table_free(self.ptr)
+            self.ptr = None
+
+    def _witness(self, witness):
+        w = np.ascontiguousarray(witness, dtype=np.uint32).reshape(-1)
+        if w.size != self.n_witness << self.log_length:
+            raise ValueError(f"witness must hold {self.n_witness} columns of 2^log_length values")
+        return w
+
+    def prove(self, settings, p16, p24, witness):
+        """proof_data over a fresh transcript (this build's domain separator)."""
+        p16 = np.ascontiguousarray(p16, dtype=np.uint32)
+        p24 = np.ascontiguousarray(p24, dtype=np.uint32)
+        w = self._witness(witness)
+        st = settings.c_struct()
+        n = ctypes.c_size_t(0)
+        check(lib().wl_air_prove(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24), u32ptr(w), None, 0,
+                                 ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=np.uint32)
+        check(lib().wl_air_prove(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24), u32ptr(w), u32ptr(out),
+                                 out.size, ctypes.byref(n)))
+        return out
+
+    def prove_into(self, settings, p16, p24, witness, prover_state):
+        p16 = np.ascontiguousarray(p16, dtype=np.uint32)
+        p24 = np.ascontiguousarray(p24, dtype=np.uint32)
+        st = settings.c_struct()
+        check(lib().wl_air_prove_into(self.ptr, ctypes.byref(st), u32ptr(p16), u32ptr(p24),
+                                      u32ptr(self._witness(witness)), prover_state.ptr))
+
+
 class Witness:
     """Trace resident in HBM (wl_witness_upload)."""
 

@@ -316,11 +316,16 @@ struct wl_ctx {
     }
 };
 
-// AirTable for the Poseidon2 AIR: trace length, skips and round constants.
+namespace wg { struct AirSpec; }
+
+// AirTable: trace length, skips and either the Poseidon2 AIR's round constants
+// (the specialised kernels) or, when gen is set, an AIR given as a constraint
+// program with its preprocessed columns.
 struct wl_table {
     wl_ctx *ctx;
     uint32_t log_length;
     uint32_t univariate_skips;
+    wg::AirSpec *gen = nullptr;
     wk::AirConsts air_host;
     wk::ChainLin chain_host;
     wk::AirConsts *air_dev = nullptr;   // from the context's pool

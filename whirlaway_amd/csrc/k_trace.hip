@@ -125,6 +125,7 @@ int wl_p2_bench_inputs(uint64_t seed, uint32_t log_n, uint32_t *air_rc, uint32_t
 
 wl_witness *wl_air_generate_witness(wl_table *t, const uint32_t *inputs) {
     if (!t || !inputs) { wl_set_error("null argument"); return nullptr; }
+    if (t->gen) { wl_set_error("trace generation is for the Poseidon2 AIR table"); return nullptr; }
     wl_ctx *c = t->ctx;
     try {
         HIP_CHECK(hipSetDevice(c->device));

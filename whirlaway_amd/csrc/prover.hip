@@ -17,6 +17,7 @@
 #include <vector>
 #include "../../include/whirlaway_hip.h"
 #include "ctx.h"
+#include "sc_generic.h"
 
 using namespace kb;
 using wh::Transcript;
@@ -808,6 +809,119 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
     lap(5);
 }
 
+// AirTable::prove (crates/air/src/prove.rs:49-226) for an AIR given as a
+// constraint program: the same protocol with the zerocheck run by the generic
+// sumcheck over the 2C up/down columns of preprocessed ++ witness
+// (prove.rs:109-131), inner sums of the witness columns only (:135-145), column
+// batching over the witness (:147-160).  packed: device, Montgomery, the nw
+// witness columns zero-padded to 2^(n + log2_up(nw)).
+void air_prove_generic(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const F *packed, Transcript &ts,
+                       const std::vector<uint32_t> *absorb_first) {
+    wl_ctx *c = t->ctx;
+    const wg::AirSpec &sp = *t->gen;
+    const int n = (int)t->log_length, skips = (int)t->univariate_skips, sec = (int)st->security_bits;
+    const size_t N = (size_t)1 << n;
+    const int C = sp.n_columns, np = sp.n_preprocessed, nw = C - np;
+    const int lc = (int)wh::log2_up((size_t)nw), ncons = sp.prog.n_asserts;
+    WhirCfg cfg = air_whir_config(t->log_length, (uint32_t)nw, st);
+    c->ring_reset();
+    WhirWitness w;
+    whir_commit(c, cfg, pdev, ts, packed, (size_t)nw * N, w, absorb_first);
+    grind(c, ts, pdev, pow_bits_air(sec, (int)wh::log2_up((size_t)std::max(1, ncons))));
+    const EF alpha = ts.sample_ef();
+    std::vector<EF> ap(std::max(1, ncons));
+    ap[0] = eone();
+    for (int i = 1; i < ncons; i++) ap[i] = emul(ap[i - 1], alpha);
+    grind(c, ts, pdev, pow_bits_air(sec, n));
+    std::vector<EF> zc(n + 1 - skips);
+    for (auto &z : zc) z = ts.sample_ef();
+    // ---- zerocheck ----
+    std::vector<EF> ch, fe(2 * (size_t)C);
+    {
+        DBuf<F> ud(c->pool, 2 * (size_t)C * N);
+        wg::columns_up_down(sp.pre.p, np, packed, C, n, ud, c->stream);
+        wg::Args a;
+        a.skips = skips;
+        a.degree = sp.degree;
+        a.batching = &ap;
+        a.eq_factor = &zc;
+        a.is_zerofier = true;
+        a.sum = ezero();
+        a.grind_kind = wg::GRIND_AUTO;
+        a.grind_value = sec;
+        DBuf<EF> folded;
+        EF fsum;
+        ch = wg::prove(c, ts, pdev, ud.p, true, 2 * C, n, sp.prog, a, folded, fsum);
+        c->fetch(fe.data(), folded.p, fe.size() * sizeof(EF));
+    }
+    // ---- inner sums of the witness columns, up then down ----
+    ts.add_ef(fe.data() + np, (size_t)nw);
+    ts.add_ef(fe.data() + C + np, (size_t)nw);
+    grind(c, ts, pdev, pow_bits_air(sec, lc));
+    std::vector<EF> cbs(lc);
+    for (auto &x : cbs) x = ts.sample_ef();
+    // inner-sumcheck tables [matrix | batched column], contiguous
+    DBuf<EF> tabs(c->pool, 2 * N);
+    EF *Mt = tabs.p, *B = tabs.p + N;
+    {
+        DBuf<EF> eqc = lc ? eq_dev(c, cbs, eone()) : upload_ef(c, std::vector<EF>(1, eone()));
+        wk::lincomb_cols(packed, n, nw, eqc, B, c->stream);
+    }
+    const EF alpha2 = ts.sample_ef();
+    DBuf<EF> da2 = upload_ef(c, std::vector<EF>(1, alpha2));
+    DBuf<EF> mixed(c->pool, N);
+    wk::mix_updown(B, n, da2, mixed, c->stream);
+    const int K = 1 << skips;
+    const size_t M = N >> skips;
+    std::vector<EF> sub;
+    {
+        std::vector<EF> chr(ch.begin() + 1, ch.end());
+        DBuf<EF> eqr = chr.empty() ? upload_ef(c, std::vector<EF>(1, eone())) : eq_dev(c, chr, eone());
+        const int nb = wk::dot_blocks(M);
+        DBuf<EF> parts(c->pool, (size_t)K * nb);
+        wk::seg_dot(mixed, eqr, M, K, parts, c->stream);
+        sub = reduce_many(c, parts, K, nb);
+    }
+    ts.add_ef(sub.data(), sub.size());
+    std::vector<EF> pt;
+    for (int i = 0; i < skips; i++) pt.push_back(ts.sample_ef());
+    for (size_t i = 1; i < ch.size(); i++) pt.push_back(ch[i]);
+    EF inner_sum;
+    {
+        DBuf<EF> E = eq_dev(c, pt, eone());
+        wk::build_matrix(E, n, da2, Mt, c->stream);
+        const int nb = wk::dot_blocks(N);
+        DBuf<EF> parts(c->pool, nb);
+        wk::seg_dot(mixed, E, N, 1, parts, c->stream);
+        inner_sum = reduce_one(c, parts, nb);
+    }
+    mixed.reset();
+    // ---- inner sumcheck: InnerSumcheckCircuit = point[0] * point[1] (prove.rs:193-208) ----
+    wg::Program prod;
+    {
+        const uint32_t code[8] = {wg::OP_MUL, 0, 1, 0, wg::OP_ASSERT, 2, 0, 0};
+        wg::compile(code, 2, 2, prod);
+        wg::upload(c, prod);
+    }
+    const std::vector<EF> one(1, eone());
+    wg::Args ia;
+    ia.skips = 1;
+    ia.degree = 2;
+    ia.batching = &one;
+    ia.sum = inner_sum;
+    ia.grind_kind = wg::GRIND_AUTO;
+    ia.grind_value = sec;
+    DBuf<EF> ifolded;
+    EF isum;
+    std::vector<EF> ich = wg::prove(c, ts, pdev, tabs.p, false, 2, n, prod, ia, ifolded, isum);
+    EF fin[2];
+    c->fetch(fin, ifolded.p, sizeof fin);
+    tabs.reset();
+    std::vector<EF> fp(cbs);
+    fp.insert(fp.end(), ich.begin(), ich.end());
+    whir_prove(c, cfg, pdev, ts, w, fp, fin[1]);
+}
+
 }  // namespace
 
 // Coefficients of the partial-round chain as linear maps of (init, post_sbox):
@@ -938,7 +1052,52 @@ void wl_air_table_free(wl_table *t) {
     (void)hipStreamSynchronize(t->ctx->stream);
     t->ctx->pool.put(t->air_dev);
     t->ctx->pool.put(t->chain_dev);
+    delete t->gen;
     delete t;
+}
+
+wl_table *wl_air_table_new_program(wl_ctx *c, uint32_t n_columns, uint32_t n_preprocessed,
+                                   const uint32_t *preprocessed, uint32_t log_length, uint32_t univariate_skips,
+                                   const uint32_t *program, uint32_t n_instr, uint32_t constraint_degree) {
+    wl_table *t = nullptr;
+    try {
+        if (!c || !program || (n_preprocessed && !preprocessed)) throw std::runtime_error("null argument");
+        if (n_columns < 1 || n_preprocessed >= n_columns || n_columns > 32767)
+            throw std::runtime_error("an AIR needs at least one witness column and at most 32767 columns");
+        if (log_length < 2 || log_length > 28 || univariate_skips < 1 || univariate_skips >= log_length)
+            throw std::runtime_error("univariate_skips must be in [1, log_length)");
+        if (constraint_degree < 1) throw std::runtime_error("constraint_degree must be at least 1");
+        HIP_CHECK(hipSetDevice(c->device));
+        t = new wl_table();
+        t->ctx = c;
+        t->log_length = log_length;
+        t->univariate_skips = univariate_skips;
+        t->gen = new wg::AirSpec();
+        wg::AirSpec &sp = *t->gen;
+        sp.n_columns = (int)n_columns;
+        sp.n_preprocessed = (int)n_preprocessed;
+        sp.degree = (int)constraint_degree;
+        wg::compile(program, n_instr, 2 * (int)n_columns, sp.prog);
+        wg::upload(c, sp.prog);
+        const size_t N = (size_t)1 << log_length;
+        sp.pre = DBuf<F>(c->pool, std::max<size_t>(1, (size_t)n_preprocessed * N));
+        if (n_preprocessed) {
+            HIP_CHECK(hipMemcpyAsync(sp.pre.p, preprocessed, (size_t)n_preprocessed * N * 4, hipMemcpyHostToDevice,
+                                     c->stream));
+            wk::convert_to_mont((uint32_t *)sp.pre.p, (size_t)n_preprocessed * N, c->stream);
+        }
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        return t;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        if (t) { delete t->gen; delete t; }
+        return nullptr;
+    }
+}
+
+// committed (witness) columns of a table
+static uint32_t witness_cols(const wl_table *t) {
+    return t->gen ? (uint32_t)(t->gen->n_columns - t->gen->n_preprocessed) : (uint32_t)AIR_NCOLS;
 }
 
 // Proves into `ts`: a caller's transcript (wl_transcript), or, when null, a fresh
@@ -955,13 +1114,17 @@ static int prove_common(wl_table *t, const wl_air_settings *st, const uint32_t *
     }
     const P2Dev *pd = context_p2(c, ph);
     std::vector<uint32_t> proof;
+    auto run = [&](Transcript &ts, const std::vector<uint32_t> *domsep) {
+        if (t->gen) air_prove_generic(t, st, pd, packed, ts, domsep);
+        else air_prove(t, st, pd, packed, ts, domsep);
+    };
     try {
         if (caller_ts) {
-            air_prove(t, st, pd, packed, caller_ts->ts, nullptr);
+            run(caller_ts->ts, nullptr);
         } else {
             Transcript ts(&ph.p16);
-            const std::vector<uint32_t> domsep = wh::whir_domsep(air_whir_config(t->log_length, AIR_NCOLS, st));
-            air_prove(t, st, pd, packed, ts, &domsep);
+            const std::vector<uint32_t> domsep = wh::whir_domsep(air_whir_config(t->log_length, witness_cols(t), st));
+            run(ts, &domsep);
             proof = std::move(ts.data);
         }
         HIP_CHECK(hipStreamSynchronize(c->stream));  // nothing of this proof left in flight
@@ -987,16 +1150,19 @@ static int check_settings(const wl_table *t, const wl_air_settings *st) {
         g_err = "settings.univariate_skips differs from the table's";
         return WL_ERR_ARG;
     }
-    return wl_air_settings_validate(t->log_length, AIR_NCOLS, st);
+    return wl_air_settings_validate(t->log_length, witness_cols(t), st);
 }
 
 // packed_multilinear (crates/utils/src/multilinear.rs:169-181) of a host witness of
-// 165 canonical columns, in Montgomery form, zero-padded to 2^(n+8)
-static void upload_packed(wl_ctx *c, const uint32_t *witness, uint32_t log_length, F *packed) {
-    const size_t N = (size_t)1 << log_length, V = N << 8;
-    wk::fill_zero(packed + AIR_NCOLS * N, (V - AIR_NCOLS * N) * 4, c->stream);
-    HIP_CHECK(hipMemcpyAsync(packed, witness, AIR_NCOLS * N * 4, hipMemcpyHostToDevice, c->stream));
-    wk::convert_to_mont((uint32_t *)packed, AIR_NCOLS * N, c->stream);
+// ncols canonical columns, in Montgomery form, zero-padded to 2^(n + log2_up(ncols))
+static size_t packed_len(uint32_t log_length, uint32_t ncols) {
+    return (size_t)1 << (log_length + wh::log2_up(ncols));
+}
+static void upload_packed(wl_ctx *c, const uint32_t *witness, uint32_t log_length, uint32_t ncols, F *packed) {
+    const size_t N = (size_t)1 << log_length, V = packed_len(log_length, ncols);
+    wk::fill_zero(packed + ncols * N, (V - ncols * N) * 4, c->stream);
+    HIP_CHECK(hipMemcpyAsync(packed, witness, ncols * N * 4, hipMemcpyHostToDevice, c->stream));
+    wk::convert_to_mont((uint32_t *)packed, ncols * N, c->stream);
 }
 
 static int prove_host_witness(wl_table *t, const wl_air_settings *st, const uint32_t *p16c, const uint32_t *p24c,
@@ -1008,8 +1174,9 @@ static int prove_host_witness(wl_table *t, const wl_air_settings *st, const uint
     try {
         wl_ctx *c = t->ctx;
         HIP_CHECK(hipSetDevice(c->device));
-        DBuf<F> packed(c->pool, (size_t)1 << (t->log_length + 8));
-        upload_packed(c, witness, t->log_length, packed);
+        const uint32_t nw = witness_cols(t);
+        DBuf<F> packed(c->pool, packed_len(t->log_length, nw));
+        upload_packed(c, witness, t->log_length, nw, packed);
         return prove_common(t, st, p16c, p24c, packed, proof_out, proof_cap, proof_len, ts);
     } catch (const std::exception &e) {
         g_err = e.what();
@@ -1033,7 +1200,7 @@ int wl_air_prove_resident_into(wl_table *t, const wl_air_settings *st, const uin
     if (!ts || ts->verifier) { g_err = "a prover transcript is required"; return WL_ERR_ARG; }
     int rc = check_settings(t, st);
     if (rc) return rc;
-    if (!w || w->log_length != t->log_length) { g_err = "witness size mismatch"; return WL_ERR_ARG; }
+    if (!w || w->log_length != t->log_length || t->gen) { g_err = "witness size mismatch"; return WL_ERR_ARG; }
     try {
         HIP_CHECK(hipSetDevice(t->ctx->device));
         return prove_common(t, st, p16c, p24c, w->packed, nullptr, 0, nullptr, ts);
@@ -1063,8 +1230,8 @@ wl_witness *wl_witness_upload(wl_ctx *c, const uint32_t *witness, uint32_t log_l
         wl_witness *w = new wl_witness();
         w->ctx = c;
         w->log_length = log_length;
-        w->packed = (F *)c->pool.get(((size_t)4 << 8) << log_length);
-        upload_packed(c, witness, log_length, w->packed);
+        w->packed = (F *)c->pool.get(packed_len(log_length, AIR_NCOLS) * 4);
+        upload_packed(c, witness, log_length, AIR_NCOLS, w->packed);
         HIP_CHECK(hipStreamSynchronize(c->stream));
         return w;
     } catch (const std::exception &e) {
@@ -1092,7 +1259,7 @@ int wl_air_prove_resident(wl_table *t, const wl_air_settings *st, const uint32_t
                           size_t proof_cap, size_t *proof_len) {
     int rc = check_settings(t, st);
     if (rc) return rc;
-    if (!w || w->log_length != t->log_length) { g_err = "witness size mismatch"; return WL_ERR_ARG; }
+    if (!w || w->log_length != t->log_length || t->gen) { g_err = "witness size mismatch"; return WL_ERR_ARG; }
     try {
         HIP_CHECK(hipSetDevice(t->ctx->device));
         return prove_common(t, st, p16c, p24c, w->packed, proof_out, proof_cap, proof_len);

@@ -32,6 +32,20 @@ struct Program {
 void compile(const uint32_t *code, size_t n_instr, int n_inputs, Program &out);
 void upload(wl_ctx *c, Program &p);
 
+// An AIR given as a constraint program over [local(width), next(width)]
+// (AirTable::new, crates/air/src/table.rs:32-56): the first n_preprocessed columns
+// are preprocessed (held here on the device), the others are the witness.
+struct AirSpec {
+    int n_columns = 0, n_preprocessed = 0, degree = 1;
+    Program prog;
+    wl::DBuf<F> pre;  // n_preprocessed x 2^log_length, Montgomery
+};
+
+// columns_up_and_down (crates/air/src/utils.rs:140-160) of preprocessed ++
+// witness: out = [up(col_0) .. up(col_{C-1}), down(col_0) .. down(col_{C-1})],
+// 2C tables of 2^log_n F; up(c)[N-1] = c[N-2], down(c) = c[1..] ++ c[N-1].
+void columns_up_down(const F *pre, int n_pre, const F *wit, int n_cols, int log_n, F *out, hipStream_t s);
+
 // SumcheckGrinding (crates/sumcheck/src/lib.rs:14-31)
 enum Grinding { GRIND_NONE = 0, GRIND_AUTO = 1, GRIND_CUSTOM = 2 };
 

@@ -108,6 +108,19 @@ __global__ void k_fold_tables(const T *in, size_t len, int K, const EF *sc, EF *
     }
 }
 
+__global__ void k_up_down(const F *pre, int n_pre, const F *wit, int n_cols, size_t N, F *out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n_cols * N) return;
+    const size_t c = t / N, x = t - c * N;
+    const F *col = (int)c < n_pre ? pre + c * N : wit + (c - n_pre) * N;
+    out[t] = col[x == N - 1 ? N - 2 : x];                    // up
+    out[(size_t)n_cols * N + t] = col[x == N - 1 ? N - 1 : x + 1];  // down
+}
+void columns_up_down(const F *pre, int n_pre, const F *wit, int n_cols, int log_n, F *out, hipStream_t s) {
+    const size_t N = (size_t)1 << log_n, total = (size_t)n_cols * N;
+    k_up_down<<<nblk(total, 256), 256, 0, s>>>(pre, n_pre, wit, n_cols, N, out);
+}
+
 // --------------------------------------------------------------- compile --
 void compile(const uint32_t *code, size_t n_instr, int n_inputs, Program &out) {
     if (n_inputs < 1 || n_inputs > 65535) throw std::runtime_error("a program reads 1 to 65535 inputs");
