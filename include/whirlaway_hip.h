@@ -69,7 +69,8 @@ void wl_destroy(wl_ctx *ctx);
 /* AirTable::new for the Poseidon2-16 KoalaBear AIR (crates/air/src/table.rs:32-56,
  * called at src/examples/poseidon2.rs:140-146).  round_constants: the AIR's
  * RoundConstants (148 canonical u32: beginning[4][16], partial[20],
- * ending[4][16]; p3-poseidon2-air RoundConstants::from_rng order). */
+ * ending[4][16]; p3-poseidon2-air RoundConstants::from_rng order).  ctx may be
+ * NULL for a table that only verifies (wl_air_verify, no device needed). */
 wl_table *wl_air_table_new(wl_ctx *ctx, const uint32_t *round_constants, uint32_t log_length,
                            uint32_t univariate_skips);
 void wl_air_table_free(wl_table *table);
@@ -81,7 +82,8 @@ void wl_air_table_free(wl_table *table);
  * witness), constraints as a program over [local row, next row] (2 n_columns
  * inputs; see wl_sumcheck_prove), the i-th assert receiving alpha^i.  Such a table
  * proves with wl_air_prove / wl_air_prove_into on a witness of n_columns -
- * n_preprocessed columns and verifies with wl_air_verify. */
+ * n_preprocessed columns and verifies with wl_air_verify; with ctx NULL it only
+ * verifies. */
 wl_table *wl_air_table_new_program(wl_ctx *ctx, uint32_t n_columns, uint32_t n_preprocessed,
                                    const uint32_t *preprocessed, uint32_t log_length, uint32_t univariate_skips,
                                    const uint32_t *program, uint32_t n_instr, uint32_t constraint_degree);
@@ -116,6 +118,36 @@ int wl_air_prove_into(wl_table *table, const wl_air_settings *settings, const ui
  * *len receives the length; out may be NULL. */
 int wl_air_domain_separator(uint32_t log_length, uint32_t n_witness_columns, const wl_air_settings *settings,
                             uint32_t *out, size_t cap, size_t *len);
+
+/* ----------------------------------------------------------------- verify --
+ * AirTable::verify(settings, merkle_hash, merkle_compress, verifier_state,
+ * log_length) (crates/air/src/verify.rs:55-249) on the host, for either kind of
+ * table: verifier_state is a verifier transcript (wl_transcript_new_verifier over
+ * the proof_data and the domain separator the prover's transcript started from),
+ * read in the prover's order.  WL_OK, or WL_ERR_VERIFY with *error set to the
+ * AirVerifError kind (verify.rs:26-34): 1 InvalidPcsCommitment, 2
+ * InvalidPcsOpening, 3 Fs, 4 Sumcheck, 5 InvalidBoundaryCondition, 6 SumMismatch. */
+int wl_air_verify(const wl_table *table, const wl_air_settings *settings, const uint32_t *p16, const uint32_t *p24,
+                  wl_transcript *verifier_state, int *error);
+/* Batch verification on n_threads host threads of n proofs of one table made by
+ * wl_air_prove (this build's domain separator); results[i]: 0 accepted, else the
+ * AirVerifError kind (3 also for words left over after the proof). */
+int wl_air_verify_batch(const wl_table *table, const wl_air_settings *settings, const uint32_t *p16,
+                        const uint32_t *p24, const uint32_t *const *proofs, const size_t *proof_lens,
+                        size_t n_proofs, int n_threads, int *results);
+
+/* Wire format of a proof (proof_data, src/examples/poseidon2.rs:178, whose size
+ * the reference counts at log2(p) bits per word, :193): a 48-byte header of
+ * little-endian u32 {"WLP1", version 1, log_length, n_witness_columns, the 7
+ * settings words, number of words}, the words packed at 31 bits each
+ * (little-endian bit order), and a 32-bit FNV-1a checksum of everything before
+ * it.  Deserialization refuses a wrong magic, version, length or checksum and
+ * non-canonical words; *proof_len receives the word count (proof may be NULL). */
+size_t wl_proof_serialized_size(size_t proof_len);
+int wl_proof_serialize(const wl_air_settings *settings, uint32_t log_length, uint32_t n_witness_columns,
+                       const uint32_t *proof, size_t proof_len, uint8_t *out, size_t cap, size_t *out_len);
+int wl_proof_deserialize(const uint8_t *in, size_t in_len, wl_air_settings *settings, uint32_t *log_length,
+                         uint32_t *n_witness_columns, uint32_t *proof, size_t cap, size_t *proof_len);
 
 /* Witness resident in HBM (the commitment's packed polynomial,
  * crates/utils/src/multilinear.rs:169-181, built on the device). */

@@ -331,3 +331,22 @@ def test_prove_into_caller_transcript(gpu_ctx, oracle, n):
     with pytest.raises(wa.WhirlawayError):
         table.prove_into(wa.AirSettings(), p16, p24, b.trace(), other)
     table.close()
+
+
+def test_product_verifier_accepts_gpu_proofs(gpu_ctx, oracle):
+    """GPU proofs (2^17 Poseidon2 and a program AIR) pass the product verifier,
+    alone and in a batch, and a changed word is rejected."""
+    import whirlaway_amd as wa
+    from whirlaway_amd.proof import verify_batch
+    _say("product verifier on GPU proofs")
+    n = 17
+    rc, inp, p16, p24 = wa.air.bench_inputs(0, n)
+    table = wa.AirTable(gpu_ctx, rc, n)
+    w = wa.air.generate_witness(table, inp)
+    proof = table.prove(wa.AirSettings(), p16, p24, w)
+    w.close()
+    bad = proof.copy()
+    bad[1000] = (int(bad[1000]) + 1) % P
+    assert verify_batch(table, wa.AirSettings(), p16, p24, [proof, bad, proof], n_threads=3)[::2] == [0, 0]
+    assert verify_batch(table, wa.AirSettings(), p16, p24, [bad])[0] != 0
+    table.close()

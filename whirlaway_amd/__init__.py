@@ -104,6 +104,15 @@ def lib():
         ui = ctypes.c_uint32
         L.wl_sumcheck_prove.argtypes = [vp, vp, ui, u32p, ui, ctypes.c_int, ui, u32p, ui, ui, u32p, ui, u32p,
                                         ctypes.c_int, u32p, ui, ctypes.c_int, ui, u32p, u32p, u32p, u32p]
+        L.wl_air_verify.argtypes = [vp, vp, u32p, u32p, vp, ctypes.POINTER(ctypes.c_int)]
+        L.wl_air_verify_batch.argtypes = [vp, vp, u32p, u32p, ctypes.POINTER(u32p), ctypes.POINTER(ctypes.c_size_t),
+                                          ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        L.wl_proof_serialized_size.restype = ctypes.c_size_t
+        L.wl_proof_serialized_size.argtypes = [ctypes.c_size_t]
+        L.wl_proof_serialize.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_size_t, u8p,
+                                         ctypes.c_size_t, szp]
+        L.wl_proof_deserialize.argtypes = [u8p, ctypes.c_size_t, vp, u32p, u32p, u32p, ctypes.c_size_t, szp]
         L.wl_kernel_timing.argtypes = [vp, ctypes.c_uint32]
         L.wl_kernel_timing_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), dp, dp]
         _lib = L
@@ -124,4 +133,4 @@ def check(rc):
 
 
 from .air import AirSettings, AirTable, Context  # noqa: E402
-from . import engine, program, sumcheck, transcript, utils  # noqa: E402,F401
+from . import engine, program, proof, sumcheck, transcript, utils  # noqa: E402,F401

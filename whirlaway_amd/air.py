@@ -114,7 +114,9 @@ class AirTable:
         # costs its page faults on every call (the returned proof is a copy)
         self._out = np.empty(1 << 20, dtype=np.uint32)
         self._out_lock = threading.Lock()
-        self.ptr = lib().wl_air_table_new(ctx.ptr, u32ptr(rc), log_length, univariate_skips)
+        # ctx None: a table that only verifies (no device)
+        self.ptr = lib().wl_air_table_new(ctx.ptr if ctx is not None else None, u32ptr(rc), log_length,
+                                          univariate_skips)
         if not self.ptr:
             raise WhirlawayError(lib().wl_last_error().decode())
 
@@ -184,7 +186,8 @@ class ProgramAirTable:
         self.n_preprocessed = n_pre
         self.n_witness = self.n_columns - n_pre
         self._code = np.ascontiguousarray(program.code(), dtype=np.uint32)
-        self.ptr = lib().wl_air_table_new_program(ctx.ptr, self.n_columns, n_pre, u32ptr(pre), log_length,
+        self.ptr = lib().wl_air_table_new_program(ctx.ptr if ctx is not None else None, self.n_columns, n_pre,
+                                                  u32ptr(pre), log_length,
                                                   univariate_skips, u32ptr(self._code), program.n_instr,
                                                   constraint_degree)
         if not self.ptr:

@@ -1023,14 +1023,15 @@ int wl_device_count(void) {
 
 wl_table *wl_air_table_new(wl_ctx *c, const uint32_t *rc, uint32_t log_length, uint32_t univariate_skips) {
     try {
-        if (!c || !rc) throw std::runtime_error("null argument");
+        if (!rc) throw std::runtime_error("null argument");
         if (univariate_skips >= log_length) throw std::runtime_error("univariate_skips must be < log_length");
         wl_table *t = new wl_table();
         t->ctx = c;
         t->log_length = log_length;
         t->univariate_skips = univariate_skips;
-        HIP_CHECK(hipSetDevice(c->device));
         for (int i = 0; i < wk::AIR_NRC; i++) t->air_host.rc[i] = to_mont(rc[i] % P);
+        if (!c) return t;  // verification only
+        HIP_CHECK(hipSetDevice(c->device));
         chain_linear_maps(t->chain_host);
         // pool memory and stream-ordered copies: no device-wide synchronization, so
         // contexts on other host threads keep running
@@ -1049,9 +1050,11 @@ wl_table *wl_air_table_new(wl_ctx *c, const uint32_t *rc, uint32_t log_length, u
 
 void wl_air_table_free(wl_table *t) {
     if (!t) return;
-    (void)hipStreamSynchronize(t->ctx->stream);
-    t->ctx->pool.put(t->air_dev);
-    t->ctx->pool.put(t->chain_dev);
+    if (t->ctx) {
+        (void)hipStreamSynchronize(t->ctx->stream);
+        t->ctx->pool.put(t->air_dev);
+        t->ctx->pool.put(t->chain_dev);
+    }
     delete t->gen;
     delete t;
 }
@@ -1061,13 +1064,12 @@ wl_table *wl_air_table_new_program(wl_ctx *c, uint32_t n_columns, uint32_t n_pre
                                    const uint32_t *program, uint32_t n_instr, uint32_t constraint_degree) {
     wl_table *t = nullptr;
     try {
-        if (!c || !program || (n_preprocessed && !preprocessed)) throw std::runtime_error("null argument");
+        if (!program || (n_preprocessed && !preprocessed)) throw std::runtime_error("null argument");
         if (n_columns < 1 || n_preprocessed >= n_columns || n_columns > 32767)
             throw std::runtime_error("an AIR needs at least one witness column and at most 32767 columns");
         if (log_length < 2 || log_length > 28 || univariate_skips < 1 || univariate_skips >= log_length)
             throw std::runtime_error("univariate_skips must be in [1, log_length)");
         if (constraint_degree < 1) throw std::runtime_error("constraint_degree must be at least 1");
-        HIP_CHECK(hipSetDevice(c->device));
         t = new wl_table();
         t->ctx = c;
         t->log_length = log_length;
@@ -1078,8 +1080,14 @@ wl_table *wl_air_table_new_program(wl_ctx *c, uint32_t n_columns, uint32_t n_pre
         sp.n_preprocessed = (int)n_preprocessed;
         sp.degree = (int)constraint_degree;
         wg::compile(program, n_instr, 2 * (int)n_columns, sp.prog);
-        wg::upload(c, sp.prog);
+        sp.code.assign(program, program + 4 * (size_t)n_instr);
         const size_t N = (size_t)1 << log_length;
+        if (n_preprocessed) sp.pre_host.assign(preprocessed, preprocessed + (size_t)n_preprocessed * N);
+        for (uint32_t &w : sp.pre_host)
+            if (w >= P) throw std::runtime_error("preprocessed values must be canonical");
+        if (!c) return t;  // verification only
+        HIP_CHECK(hipSetDevice(c->device));
+        wg::upload(c, sp.prog);
         sp.pre = DBuf<F>(c->pool, std::max<size_t>(1, (size_t)n_preprocessed * N));
         if (n_preprocessed) {
             HIP_CHECK(hipMemcpyAsync(sp.pre.p, preprocessed, (size_t)n_preprocessed * N * 4, hipMemcpyHostToDevice,
@@ -1146,6 +1154,7 @@ static int prove_common(wl_table *t, const wl_air_settings *st, const uint32_t *
 
 static int check_settings(const wl_table *t, const wl_air_settings *st) {
     if (!t || !st) { g_err = "null argument"; return WL_ERR_ARG; }
+    if (!t->ctx) { g_err = "a table created without a context only verifies"; return WL_ERR_ARG; }
     if (st->univariate_skips != t->univariate_skips) {
         g_err = "settings.univariate_skips differs from the table's";
         return WL_ERR_ARG;

@@ -39,6 +39,9 @@ struct AirSpec {
     int n_columns = 0, n_preprocessed = 0, degree = 1;
     Program prog;
     wl::DBuf<F> pre;  // n_preprocessed x 2^log_length, Montgomery
+    // host copies for the verifier: the SSA program and the preprocessed columns (canonical)
+    std::vector<uint32_t> code;
+    std::vector<uint32_t> pre_host;
 };
 
 // columns_up_and_down (crates/air/src/utils.rs:140-160) of preprocessed ++
