@@ -217,7 +217,7 @@ def main():
                     help="extra proofs timed from a host-memory witness (PCIe-inclusive rate)")
     ap.add_argument("--batch-jobs", type=int, default=64, help="batched leg: proofs (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=256, help="serving leg: requests (0 = skip)")
-    ap.add_argument("--serve-rates", default="120,400",
+    ap.add_argument("--serve-rates", default="120,400,1200",
                     help="serving leg: Poisson arrival rates per second, comma-separated; the last one "
                          "saturates the plain path and gives the headline serving figures")
     ap.add_argument("--slots", type=int, default=8, help="engine slots (concurrent proofs)")

@@ -6,6 +6,7 @@ the hypercube instead of folds, direct polynomial evaluation instead of butterfl
 so that a dropped term, a wrong index or a transposed operand fails here.
 """
 import ctypes
+import math
 import itertools
 
 import numpy as np
@@ -392,20 +393,137 @@ def test_product_sumcheck_rounds_brute_force(oracle):
 
 
 # ---------------------------------------------------------------------- WHIR --
-def test_whir_query_counts_capacity_bound(oracle):
-    """CapacityBound: each query is wrong with probability rate = 2^-log_inv_rate, so a
-    round needs ceil((security - max_pow_bits) / log_inv_rate) queries."""
-    for v in (13, 20, 25):
-        cfg = oracle.whir_config(v)
-        psec = 128 - 16
-        for r in cfg["rounds"]:
-            assert r["num_queries"] == -(-psec // r["log_inv_rate"])
-        last = cfg["rounds"][-1] if cfg["rounds"] else None
-        if last is not None:
-            assert cfg["final_queries"] == -(-psec // (last["log_inv_rate"] + last["folding_factor"] - 1))
-        assert all(0 <= r["pow_bits"] <= 16 for r in cfg["rounds"])
-    cfg = oracle.whir_config(25)
-    assert [r["num_queries"] for r in cfg["rounds"]] + [cfg["final_queries"]] == [112, 38, 19, 13]
+# The WHIR schedule (WhirConfig::new) is pinned by the soundness statements it must
+# satisfy, evaluated here from the error terms of the WHIR paper (round-by-round
+# soundness of each step, in bits) rather than from the oracle's formulas: every
+# step reaches the security level with its proof of work, the counts are the
+# smallest that do, and the proof of work is the shortfall.
+FIELD_BITS = 248  # |EF| = p^8
+SEC, MAX_POW = 128, 16
+
+
+def _eta_bits(sa, r):
+    """-log2(eta), the proximity slack: rho/2 at capacity, sqrt(rho)/20 under the
+    Johnson bound (r = log2(1/rho))."""
+    return {2: r + 1, 1: r / 2 + math.log2(20)}[sa]
+
+
+def _list_bits(sa, d, r):
+    """log2 of the list size: 2^d / (rho eta) at capacity, 1 / (2 eta sqrt(rho))
+    under the Johnson bound, 1 in unique decoding."""
+    if sa == 0:
+        return 0.0
+    if sa == 1:
+        return r / 2 - 1 + _eta_bits(1, r)
+    return d + r + _eta_bits(2, r)
+
+
+def _query_bits(sa, r):
+    """-log2 of the chance that a query misses a far word: 1 - delta = rho (capacity),
+    sqrt(rho) (Johnson), (1 + rho) / 2 (unique decoding)."""
+    rho = 2.0 ** -r
+    return {2: r, 1: r / 2, 0: -math.log2((1 + rho) / 2)}[sa]
+
+
+def _ood_bits(sa, d, r, s):
+    """OOD step: two list codewords agree on s random points w.p. (2^d / |EF|)^s,
+    union over the L^2 / 2 pairs."""
+    return s * (FIELD_BITS - d) - (2 * _list_bits(sa, d, r) - 1)
+
+
+def _check_round(sa, d, r, q, ood, pow_bits, next_r):
+    psec = SEC - MAX_POW
+    qb = _query_bits(sa, r)
+    assert q * qb >= psec and (q - 1) * qb < psec, (d, r, q)
+    if sa:
+        assert _ood_bits(sa, d, next_r, ood) >= SEC and _ood_bits(sa, d, next_r, ood - 1) < SEC
+    else:
+        assert ood == 0
+    # combination of the ood + q constraints: a random linear combination of the
+    # list's errors, (ood + q) L / |EF|; the PoW covers the shortfall of the weaker term
+    comb = FIELD_BITS - (math.log2(ood + q) + _list_bits(sa, d, next_r) + 1)
+    assert pow_bits == max(0, math.ceil(SEC - min(q * qb, comb)))
+
+
+@pytest.mark.parametrize("sa", [0, 1, 2])
+@pytest.mark.parametrize("v", [13, 20, 25])
+def test_whir_schedule_meets_the_soundness_bounds(oracle, v, sa):
+    f0, f, rate, rs0 = 7, 4, 1, 5
+    cfg = oracle.whir_config(v, sa=sa)
+    # rounds: fold f0, then f per round, until at most 2^6 coefficients remain
+    assert cfg["final_sumcheck_rounds"] <= 6
+    assert cfg["final_sumcheck_rounds"] == v - f0 - f * cfg["n_rounds"]
+    if cfg["n_rounds"]:
+        assert v - f0 - f * (cfg["n_rounds"] - 1) > 6
+    if sa:
+        assert _ood_bits(sa, v, rate, cfg["commit_ood"]) >= SEC > _ood_bits(sa, v, rate, cfg["commit_ood"] - 1)
+    else:
+        assert cfg["commit_ood"] == 0
+    d, r, logd = v - f0, rate, v + rate
+    for i, rd in enumerate(cfg["rounds"]):
+        fold = f0 if i == 0 else f
+        next_r = r + fold - (rs0 if i == 0 else 1)
+        assert (rd["num_variables"], rd["log_inv_rate"], rd["log_domain_size"]) == (d, r, logd)
+        _check_round(sa, d, r, rd["num_queries"], rd["ood_samples"], rd["pow_bits"], next_r)
+        # folding step: proximity gaps of the 2-function combination, and the sumcheck's
+        # degree-2 polynomial vs the list
+        if sa == 2:
+            prox = FIELD_BITS - (d + 2 * next_r + _eta_bits(2, next_r))
+        elif sa == 1:
+            prox = FIELD_BITS - (2 * d + 7 * (min(1 + math.log2(10) + next_r / 2, _eta_bits(1, next_r)) - 1))
+        else:
+            prox = FIELD_BITS - (d + next_r)
+        fsum = FIELD_BITS - (_list_bits(sa, d, next_r) + 1)
+        assert rd["folding_pow_bits"] == max(0, math.ceil(SEC - min(prox, fsum)))
+        d -= f
+        r = next_r
+        logd -= rs0 if i == 0 else 1
+    fq = cfg["final_queries"]
+    qb = _query_bits(sa, r)
+    assert fq * qb >= SEC - MAX_POW > (fq - 1) * qb
+    assert cfg["final_pow"] == max(0, math.ceil(SEC - fq * qb))
+    assert cfg["final_log_domain"] == logd
+    assert all(0 <= x["pow_bits"] <= MAX_POW for x in cfg["rounds"])
+
+
+def test_poseidon2_round_numbers_from_the_security_rule():
+    """The round counts of Poseidon2KoalaBear<16>/<24> (8 full, 20 / 23 partial) and of
+    the reference's BabyBear alternative (x^7, 13 partial rounds at width 16,
+    src/examples/poseidon2.rs:38-45) follow from the Poseidon / Poseidon2 round-number
+    rule at 128 bits: the smallest R_F * t + R_P satisfying the statistical,
+    interpolation and Groebner-basis bounds (with the 2023/537 correction), plus the
+    margin of 2 full rounds and 7.5 % partial rounds."""
+    def log2binom(n, k):
+        return (math.lgamma(n + 1) - math.lgamma(k + 1) - math.lgamma(n - k + 1)) / math.log(2)
+
+    def secure(p, t, rf, rp, a, m):
+        n = math.ceil(math.log2(p))
+        bounds = [6 if m <= math.floor(math.log2(p) - (a - 1) / 2) * (t + 1) else 10,
+                  1 + math.ceil(math.log(2, a) * min(m, n)) + math.ceil(math.log(t, a)) - rp,
+                  math.log(2, a) * min(m, math.log2(p)) - rp,
+                  t - 1 + math.log(2, a) * min(m / (t + 1), math.log2(p) / 2) - rp,
+                  (t - 2 + m / (2 * math.log2(a)) - rp) / (t - 1)]
+        third = math.floor(t / 3)
+        over = (rf - 1) * t + rp + third + third * rf / 2 + rp + a
+        under = third * rf / 2 + rp + a
+        return rf >= max(math.ceil(x) for x in bounds) and math.ceil(2 * log2binom(over, under)) >= m
+
+    def rounds(p, t, a):
+        best = None
+        for rp in range(1, 200):
+            for rf in range(4, 40, 2):
+                if secure(p, t, rf, rp, a, 128):
+                    cand = (rf + 2, math.ceil(rp * 1.075))
+                    key = (cand[0] * t + cand[1], cand[0])
+                    if best is None or key < best[0]:
+                        best = (key, cand)
+        return best[1]
+
+    assert rounds(P, 16, 3) == (8, 20)
+    assert rounds(P, 24, 3) == (8, 23)
+    assert rounds(2**31 - 2**27 + 1, 16, 7) == (8, 13)
+    assert math.gcd(3, P - 1) == 1  # x^3 permutes KoalaBear (not BabyBear: 3 | p - 1 there)
+    assert math.gcd(3, 2**31 - 2**27) == 3
 
 
 # ---------------------------------------------------------------- whole proof --
