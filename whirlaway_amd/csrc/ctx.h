@@ -10,6 +10,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 #include "common.h"
 #include "host.h"
@@ -123,6 +124,55 @@ struct wl_ctx {
         wk::ntt_tables(twiddle(L), L, t, stream);
         ntt_tab[L] = t;
         return t;
+    }
+    // Host -> device copy of a large pageable buffer (the witness) through pinned
+    // staging: STAGE_THREADS host threads each copy their chunks (CHUNK bytes) into
+    // two pinned slots of their own and enqueue the DMA of each; a slot is refilled
+    // once the DMA that read it has completed (its event).  The copy engine then
+    // runs at PCIe speed while the host copies run in parallel, instead of the
+    // runtime's single staging path for pageable memory.  Returns after every DMA is
+    // enqueued; later work on the stream is ordered behind them.
+    static constexpr int STAGE_THREADS = 8;
+    static constexpr size_t CHUNK = (size_t)4 << 20;
+    uint8_t *stage_slots = nullptr;
+    hipEvent_t stage_ev[2 * STAGE_THREADS] = {};
+    void h2d_staged(void *dst, const void *src, size_t bytes) {
+        if (bytes < 2 * CHUNK) {
+            HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+            return;
+        }
+        const bool fresh = !stage_slots;
+        if (fresh) {
+            HIP_CHECK(hipHostMalloc((void **)&stage_slots, 2 * STAGE_THREADS * CHUNK, hipHostMallocDefault));
+            for (auto &e : stage_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        const size_t nch = (bytes + CHUNK - 1) / CHUNK;
+        const int nt = (int)std::min<size_t>(STAGE_THREADS, nch);
+        // slots of an earlier call may still feed its DMAs: wait on their events first
+        std::vector<char> used(2 * STAGE_THREADS, fresh ? 0 : 1);
+        std::vector<std::string> errs(nt);
+        auto work = [&](int t) {
+            try {
+                for (size_t k = t, j = 0; k < nch; k += nt, j++) {
+                    const int slot = t + (int)(j & 1) * STAGE_THREADS;
+                    if (used[slot]) HIP_CHECK(hipEventSynchronize(stage_ev[slot]));
+                    const size_t off = k * CHUNK, n = std::min(CHUNK, bytes - off);
+                    uint8_t *buf = stage_slots + (size_t)slot * CHUNK;
+                    memcpy(buf, (const uint8_t *)src + off, n);
+                    HIP_CHECK(hipMemcpyAsync((uint8_t *)dst + off, buf, n, hipMemcpyHostToDevice, stream));
+                    HIP_CHECK(hipEventRecord(stage_ev[slot], stream));
+                    used[slot] = 1;
+                }
+            } catch (const std::exception &e) {
+                errs[t] = e.what();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; t++) pool.emplace_back(work, t);
+        work(0);
+        for (auto &th : pool) th.join();
+        for (const auto &e : errs)
+            if (!e.empty()) throw std::runtime_error(e);
     }
     uint32_t *stage(size_t words) {
         if (words > pinned_words) {

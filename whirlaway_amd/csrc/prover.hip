@@ -1000,6 +1000,10 @@ void wl_destroy(wl_ctx *c) {
     for (auto &kv : c->ntt_tab) (void)hipFree(kv.second);
     c->pool.release();
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->stage_slots) {
+        (void)hipHostFree(c->stage_slots);
+        for (auto &e : c->stage_ev) (void)hipEventDestroy(e);
+    }
     if (c->ring) (void)hipHostFree(c->ring);
     if (c->mapped) (void)hipHostFree(c->mapped);
     if (c->mbox) (void)hipHostFree(c->mbox);
@@ -1170,7 +1174,7 @@ static size_t packed_len(uint32_t log_length, uint32_t ncols) {
 static void upload_packed(wl_ctx *c, const uint32_t *witness, uint32_t log_length, uint32_t ncols, F *packed) {
     const size_t N = (size_t)1 << log_length, V = packed_len(log_length, ncols);
     wk::fill_zero(packed + ncols * N, (V - ncols * N) * 4, c->stream);
-    HIP_CHECK(hipMemcpyAsync(packed, witness, ncols * N * 4, hipMemcpyHostToDevice, c->stream));
+    c->h2d_staged(packed, witness, ncols * N * 4);
     wk::convert_to_mont((uint32_t *)packed, ncols * N, c->stream);
 }
 
