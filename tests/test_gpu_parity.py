@@ -185,6 +185,26 @@ def test_proof_settings_variants(gpu_ctx, oracle, n, kw):
     assert b.verify(got, **kw) == 0
 
 
+@pytest.mark.parametrize("n,kw", [(8, {}), (12, {}), SETTINGS_VARIANTS[1]])
+def test_device_transcript_rounds(oracle, monkeypatch, n, kw):
+    """Zerocheck rounds whose transcript runs on the device (one wave's duplex
+    sponge, opt-in per context) give the oracle's proof bytes."""
+    import whirlaway_amd as wa
+    _say(f"full proof n={n} {kw}, device-side zerocheck transcript")
+    monkeypatch.setenv("WHIRLAWAY_DEVICE_ROUNDS", "1")
+    ctx = wa.Context(0)
+    b = oracle.Bench(n, seed=4)
+    want = b.prove(**kw)
+    st = wa.AirSettings(security_bits=kw.get("sec", 128), folding_factor_first=kw.get("f0", 7),
+                        folding_factor=kw.get("f", 4), log_inv_rate=kw.get("rate", 1),
+                        univariate_skips=kw.get("skips", 4), rs_domain_initial_reduction=kw.get("rs0", 5))
+    table = wa.AirTable(ctx, b.air_consts(), n, univariate_skips=kw.get("skips", 4))
+    got = table.prove(st, b.p2_consts(16), b.p2_consts(24), b.trace())
+    table.close()
+    ctx.close()
+    assert np.array_equal(got, want)
+
+
 def test_proof_benchmark_size_identical(gpu_ctx, oracle):
     """n = 17 (the benchmark workload): the GPU proof is byte-identical to the oracle's
     proof of the same inputs, the oracle verifier accepts it and rejects a changed

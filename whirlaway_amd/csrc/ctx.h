@@ -10,6 +10,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <atomic>
 #include <thread>
 #include <vector>
 #include "common.h"
@@ -90,6 +91,10 @@ struct DBuf {
 
 struct wl_ctx {
     int device = 0;
+    // zerocheck rounds without grinding run their transcript on the device (off by
+    // default: one wave's serial Poseidon2-16 duplexes take longer than the host
+    // round trip they replace)
+    bool device_transcript = false;
     hipStream_t stream = nullptr;
     wl::Pool pool;
     uint32_t *pinned = nullptr;   // host staging
@@ -126,53 +131,84 @@ struct wl_ctx {
         return t;
     }
     // Host -> device copy of a large pageable buffer (the witness) through pinned
-    // staging: STAGE_THREADS host threads each copy their chunks (CHUNK bytes) into
-    // two pinned slots of their own and enqueue the DMA of each; a slot is refilled
-    // once the DMA that read it has completed (its event).  The copy engine then
-    // runs at PCIe speed while the host copies run in parallel, instead of the
-    // runtime's single staging path for pageable memory.  Returns after every DMA is
-    // enqueued; later work on the stream is ordered behind them.
-    static constexpr int STAGE_THREADS = 8;
-    static constexpr size_t CHUNK = (size_t)4 << 20;
+    // staging.  The buffer goes in chunks of 2, 4, 8, then 16 MiB (CHUNK) through
+    // STAGE_SLOTS pinned slots of CHUNK bytes; STAGE_THREADS host threads copy a
+    // stripe of every chunk each, and the calling thread enqueues a chunk's DMA as
+    // soon as all stripes are in, in chunk order (small first chunks start the copy
+    // engine early, large later ones keep the per-DMA cost low).  A slot is refilled once the DMA that read it has completed
+    // (its event).  The copy engine then runs at PCIe speed one chunk behind the
+    // host copies, instead of the runtime's single staging path for pageable
+    // memory.  Returns after every DMA is enqueued; later work on the stream is
+    // ordered behind them.
+    static constexpr int STAGE_THREADS = 6, STAGE_SLOTS = 4;
+    static constexpr size_t CHUNK = (size_t)16 << 20, FIRST_CHUNK = (size_t)2 << 20;
     uint8_t *stage_slots = nullptr;
-    hipEvent_t stage_ev[2 * STAGE_THREADS] = {};
+    hipEvent_t stage_ev[STAGE_SLOTS] = {};
+    bool stage_used[STAGE_SLOTS] = {};
     void h2d_staged(void *dst, const void *src, size_t bytes) {
-        if (bytes < 2 * CHUNK) {
+        if (bytes < 4 * FIRST_CHUNK) {
             HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
             return;
         }
-        const bool fresh = !stage_slots;
-        if (fresh) {
-            HIP_CHECK(hipHostMalloc((void **)&stage_slots, 2 * STAGE_THREADS * CHUNK, hipHostMallocDefault));
+        if (!stage_slots) {
+            HIP_CHECK(hipHostMalloc((void **)&stage_slots, STAGE_SLOTS * CHUNK, hipHostMallocDefault));
             for (auto &e : stage_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         }
-        const size_t nch = (bytes + CHUNK - 1) / CHUNK;
-        const int nt = (int)std::min<size_t>(STAGE_THREADS, nch);
-        // slots of an earlier call may still feed its DMAs: wait on their events first
-        std::vector<char> used(2 * STAGE_THREADS, fresh ? 0 : 1);
-        std::vector<std::string> errs(nt);
-        auto work = [&](int t) {
-            try {
-                for (size_t k = t, j = 0; k < nch; k += nt, j++) {
-                    const int slot = t + (int)(j & 1) * STAGE_THREADS;
-                    if (used[slot]) HIP_CHECK(hipEventSynchronize(stage_ev[slot]));
-                    const size_t off = k * CHUNK, n = std::min(CHUNK, bytes - off);
-                    uint8_t *buf = stage_slots + (size_t)slot * CHUNK;
-                    memcpy(buf, (const uint8_t *)src + off, n);
-                    HIP_CHECK(hipMemcpyAsync((uint8_t *)dst + off, buf, n, hipMemcpyHostToDevice, stream));
-                    HIP_CHECK(hipEventRecord(stage_ev[slot], stream));
-                    used[slot] = 1;
+        // slots an earlier call filled may still feed its DMAs
+        for (int k = 0; k < STAGE_SLOTS; k++)
+            if (stage_used[k]) HIP_CHECK(hipEventSynchronize(stage_ev[k]));
+        std::vector<size_t> offs{0};
+        for (size_t len = FIRST_CHUNK; offs.back() < bytes; len = std::min(CHUNK, 2 * len))
+            offs.push_back(std::min(bytes, offs.back() + len));
+        const size_t nch = offs.size() - 1;
+        // filled[k]: stripes of chunk k in its slot; issued: chunks whose DMA is enqueued
+        std::vector<std::atomic<int>> filled(nch);
+        for (auto &f : filled) f.store(0, std::memory_order_relaxed);
+        std::atomic<size_t> issued{0};
+        std::atomic<bool> failed{false};
+        std::string err;
+        auto copier = [&](int t) {
+            for (size_t k = 0; k < nch && !failed.load(std::memory_order_relaxed); k++) {
+                if (k >= STAGE_SLOTS) {
+                    // the slot's previous chunk must have gone out, and its DMA finished
+                    while (issued.load(std::memory_order_acquire) <= k - STAGE_SLOTS) {
+                        if (failed.load(std::memory_order_relaxed)) return;
+                        std::this_thread::yield();
+                    }
+                    if (hipEventSynchronize(stage_ev[k % STAGE_SLOTS]) != hipSuccess) {
+                        failed.store(true);
+                        return;
+                    }
                 }
-            } catch (const std::exception &e) {
-                errs[t] = e.what();
+                const size_t off = offs[k], n = offs[k + 1] - off;
+                const size_t stripe = (n / STAGE_THREADS + 63) & ~(size_t)63;
+                const size_t a = std::min(n, (size_t)t * stripe), b = std::min(n, a + stripe);
+                if (b > a) memcpy(stage_slots + (k % STAGE_SLOTS) * CHUNK + a, (const uint8_t *)src + off + a, b - a);
+                filled[k].fetch_add(1, std::memory_order_release);
             }
         };
         std::vector<std::thread> pool;
-        for (int t = 1; t < nt; t++) pool.emplace_back(work, t);
-        work(0);
+        for (int t = 0; t < STAGE_THREADS; t++) pool.emplace_back(copier, t);
+        try {
+            for (size_t k = 0; k < nch; k++) {
+                while (filled[k].load(std::memory_order_acquire) < STAGE_THREADS) {
+                    if (failed.load(std::memory_order_relaxed)) throw std::runtime_error("staging copy failed");
+                    std::this_thread::yield();
+                }
+                const int slot = (int)(k % STAGE_SLOTS);
+                const size_t off = offs[k], n = offs[k + 1] - off;
+                HIP_CHECK(hipMemcpyAsync((uint8_t *)dst + off, stage_slots + (size_t)slot * CHUNK, n,
+                                         hipMemcpyHostToDevice, stream));
+                HIP_CHECK(hipEventRecord(stage_ev[slot], stream));
+                stage_used[slot] = true;
+                issued.store(k + 1, std::memory_order_release);
+            }
+        } catch (const std::exception &e) {
+            err = e.what();
+            failed.store(true);
+        }
         for (auto &th : pool) th.join();
-        for (const auto &e : errs)
-            if (!e.empty()) throw std::runtime_error(e);
+        if (!err.empty()) throw std::runtime_error(err);
     }
     uint32_t *stage(size_t words) {
         if (words > pinned_words) {
@@ -207,8 +243,9 @@ struct wl_ctx {
     size_t mapped_words = 0;
     uint32_t seq = 0;
     // device words that kernels leave as they found them: {smallest grinding
-    // witness, grinding blocks done} armed as {0xffffffff, 0}, then the block
-    // counter of multi-block publishes (0)
+    // witness, grinding blocks done} armed as {0xffffffff, 0}, the block counter
+    // of multi-block publishes (0), the witness of a grinding group's earlier
+    // passes (0xffffffff)
     uint32_t *sync_dev = nullptr;
     // Poseidon2 constants of the last proof on the device (re-sent only when they change)
     wl::P2Dev *p2_dev = nullptr;
@@ -216,7 +253,7 @@ struct wl_ctx {
     uint32_t *sync_words() {
         if (sync_dev) return sync_dev;
         HIP_CHECK(hipMalloc((void **)&sync_dev, 4 * sizeof(uint32_t)));
-        const uint32_t armed[4] = {0xffffffffu, 0u, 0u, 0u};
+        const uint32_t armed[4] = {0xffffffffu, 0u, 0u, 0xffffffffu};
         HIP_CHECK(hipMemcpyAsync(sync_dev, armed, sizeof(armed), hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         return sync_dev;
@@ -624,14 +661,19 @@ inline void grind(wl_ctx *c, wh::Transcript &ts, const P2Dev *pdev, int bits) {
     // 2^(bits+1) from 16 bits on (e^-2; 2^17 candidates fill 256 CUs with two
     // waves per SIMD), between 2^14 and 2^22
     const int lg = std::min(22, std::max(14, bits < 16 ? bits + 2 : bits + 1));
+    // The first pass goes alone (it usually finds the witness); later passes go
+    // out in groups of four with one publish, where a pass after the one that
+    // finds the witness returns at once.
     const uint64_t chunk = (uint64_t)1 << lg;
-    for (uint64_t base = 0; base < P && w == 0xffffffffu; base += chunk) {
-        uint32_t cnt = (uint32_t)std::min<uint64_t>(chunk, P - base);
-        const uint32_t s = ++c->seq;
-        {
+    for (uint64_t base = 0, group = 1; base < P && w == 0xffffffffu; group = 4) {
+        uint32_t s = 0;
+        for (uint64_t g = 0; g < group && base < P; g++, base += chunk) {
+            const uint32_t cnt = (uint32_t)std::min<uint64_t>(chunk, P - base);
+            const bool publish = g == group - 1 || base + chunk >= P;
+            if (publish) s = ++c->seq;
             KTimer kt(c, KT_GRIND, (double)cnt);
             wk::grind_pass(pdev, st, pos, mask, (uint32_t)base, cnt, gs, c->mapped_dev + wl_ctx::GRIND_OFF,
-                           c->flag_dev(), s, c->stream);
+                           c->flag_dev(), s, publish, c->stream);
         }
         c->wait_flag(s);
         w = c->mapped[wl_ctx::GRIND_OFF];

@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include "common.h"
 #include "kernels.h"
+#include "p16_lanes.h"
 
 namespace wk {
 using namespace kb;
@@ -698,67 +699,6 @@ __global__ void __launch_bounds__(256) k_merkle_level(F *nodes, size_t cnt, cons
 // (quad sums, then half mirror and mirror).  Cuts the dependent chain of one
 // compression about 3x against one lane per permutation; used for the tree
 // levels that have too few nodes to fill the GPU.
-template <int CTRL>
-__device__ __forceinline__ F dpp(F x) {
-    return F{(uint32_t)__builtin_amdgcn_update_dpp(0, (int)x.v, CTRL, 0xF, 0xF, true)};
-}
-constexpr int DPP_QXOR1 = 0xB1;    // quad_perm [1,0,3,2]
-constexpr int DPP_QXOR2 = 0x4E;    // quad_perm [2,3,0,1]
-constexpr int DPP_QNEXT = 0x39;    // quad_perm [1,2,3,0]
-constexpr int DPP_ROR4 = 0x124;    // row_ror:4
-constexpr int DPP_ROR8 = 0x128;    // row_ror:8
-constexpr int DPP_MIRROR = 0x140;  // row_mirror
-constexpr int DPP_HMIRROR = 0x141; // row_half_mirror
-
-__device__ __forceinline__ F ext_row16(F x) {
-    F s = fadd(x, dpp<DPP_QXOR1>(x));
-    s = fadd(s, dpp<DPP_QXOR2>(s));
-    const F nx = dpp<DPP_QNEXT>(x);
-    const F y = fadd(fadd(s, x), fadd(nx, nx));
-    F t = fadd(y, dpp<DPP_ROR4>(y));
-    t = fadd(t, dpp<DPP_ROR8>(t));
-    return fadd(y, t);
-}
-__device__ __forceinline__ F row_sum16(F x) {
-    F s = fadd(x, dpp<DPP_QXOR1>(x));
-    s = fadd(s, dpp<DPP_QXOR2>(s));
-    s = fadd(s, dpp<DPP_HMIRROR>(s));
-    return fadd(s, dpp<DPP_MIRROR>(s));
-}
-// A lane's round constants and diagonal entry, loaded once per kernel: a load
-// inside the rounds would put a cache round trip on the permutation's serial chain
-// (one wave per permutation row, nothing else to hide it).
-struct Row16 {
-    F ei[4], et[4], d;
-    bool first;
-};
-__device__ __forceinline__ Row16 row16_consts(int i, const P2Consts16 &k) {
-    Row16 c;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        c.ei[r] = k.ext_init[r][i];
-        c.et[r] = k.ext_term[r][i];
-    }
-    c.d = F{DDIAG.d16[i]};
-    c.first = i == 0;
-    return c;
-}
-// all 64 lanes must call this (DPP reads neighbours); c from row16_consts(lane & 15)
-__device__ __forceinline__ F p16_row(F x, const Row16 &c, const P2Consts16 &k) {
-    x = ext_row16(x);
-#pragma unroll
-    for (int r = 0; r < 4; r++) x = ext_row16(fcube(fadd(x, c.ei[r])));
-#pragma unroll
-    for (int r = 0; r < 20; r++) {
-        const F y = fcube(fadd(x, k.internal[r]));
-        if (c.first) x = y;
-        x = fadd(row_sum16(x), fmul(x, c.d));
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) x = ext_row16(fcube(fadd(x, c.et[r])));
-    return x;
-}
-
 // Tree levels with parents [cnt, 2cnt), [cnt/2, cnt), ... (`levels` of them): block
 // b owns the 16 parents [cnt + 16b, cnt + 16b + 16) of the first level and the
 // subtree above them; group g (16 lanes) compresses parent g of each level.  The
@@ -887,21 +827,18 @@ void merkle_open(const F *cw, const F *nodes, int width, int nnz, int log_rows, 
 // candidates of a launch finish together, so the launch size (not an early
 // exit) sets the work: the host takes 2 to 4 x 2^bits and launches again on a
 // miss.
-__global__ void __launch_bounds__(256) k_grind_pass(const P2DevT *pc, State16 st, int pos, uint32_t mask,
-                                                   uint32_t base, uint32_t count, uint32_t *state,
-                                                   uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t w = base + i;
-    if (i < count && w < P) {
-        F s[16];
-#pragma unroll
-        for (int k = 0; k < 16; k++) s[k] = st.s[k];
-#pragma unroll
-        for (int k = 0; k < 16; k++)
-            if (k == pos) s[k] = to_mont(w);
-        p16_permute(s, pc->p16);
-        if ((from_mont(s[7]) & mask) == 0) atomicMin(state, w);
-    }
+// Grinding passes over consecutive candidate ranges run back to back, with one
+// publish at the end of a group.  state = {smallest witness of the group, blocks
+// done in this pass, -, smallest witness of the group's earlier passes}, armed as
+// {0xffffffff, 0, -, 0xffffffff}: a pass whose group already holds a witness skips
+// its permutations (every earlier range is below its own, so the minimum stands),
+// and the pass's last block carries the minimum forward or, ending the group,
+// publishes it and re-arms the words.
+__device__ __forceinline__ bool grind_pass_live(const uint32_t *state) {
+    return __hip_atomic_load(state + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0xffffffffu;
+}
+__device__ __forceinline__ void grind_pass_end(uint32_t *state, uint32_t *host_dst, volatile uint32_t *flag,
+                                               uint32_t seq, bool publish) {
     __shared__ uint32_t last;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -911,12 +848,37 @@ __global__ void __launch_bounds__(256) k_grind_pass(const P2DevT *pc, State16 st
     __syncthreads();
     if (last && threadIdx.x == 0) {
         __threadfence();
-        host_dst[0] = __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(state, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t m = __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(state + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence_system();
-        *flag = seq;
+        if (publish) {
+            host_dst[0] = m;
+            __hip_atomic_store(state, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(state + 3, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence_system();
+            *flag = seq;
+        } else {
+            __hip_atomic_store(state + 3, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+        }
     }
+}
+__global__ void __launch_bounds__(256) k_grind_pass(const P2DevT *pc, State16 st, int pos, uint32_t mask,
+                                                   uint32_t base, uint32_t count, uint32_t *state,
+                                                   uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
+                                                   bool publish) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t w = base + i;
+    if (i < count && w < P && grind_pass_live(state)) {
+        F s[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) s[k] = st.s[k];
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (k == pos) s[k] = to_mont(w);
+        p16_permute(s, pc->p16);
+        if ((from_mont(s[7]) & mask) == 0) atomicMin(state, w);
+    }
+    grind_pass_end(state, host_dst, flag, seq, publish);
 }
 // k_grind_pass on lane pairs (for passes of up to 2^16 candidates, which at one
 // candidate per lane leave SIMDs idle): pair i tries w = base + i; the witness
@@ -924,7 +886,12 @@ __global__ void __launch_bounds__(256) k_grind_pass(const P2DevT *pc, State16 st
 // count run the permutation for their partner's sake and record nothing.
 __global__ void __launch_bounds__(256) k_grind_pass_pair(const P2DevT *pc, State16 st, int pos, uint32_t mask,
                                                         uint32_t base, uint32_t count, uint32_t *state,
-                                                        uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq) {
+                                                        uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
+                                                        bool publish) {
+    if (!grind_pass_live(state)) {  // the same for every lane of the grid
+        grind_pass_end(state, host_dst, flag, seq, publish);
+        return;
+    }
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = g >> 1, w = base + i;
     const bool hi = (g & 1) != 0;
@@ -938,32 +905,19 @@ __global__ void __launch_bounds__(256) k_grind_pass_pair(const P2DevT *pc, State
         if (!hi && k == pos) s[k] = to_mont(valid ? w : 0u);
     p16_pair(s, hi, pc->p16, d);
     if (!hi && valid && (from_mont(s[7]) & mask) == 0) atomicMin(state, w);
-    __shared__ uint32_t last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(state + 1, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        host_dst[0] = __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(state, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(state + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence_system();
-        *flag = seq;
-    }
+    grind_pass_end(state, host_dst, flag, seq, publish);
 }
 void grind_pass(const void *pc, State16 st, int pos, uint32_t mask, uint32_t base, uint32_t count,
-                uint32_t *state, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq, hipStream_t s) {
+                uint32_t *state, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq, bool publish,
+                hipStream_t s) {
     if (count <= (1u << 16) && pos < 8) {
         const unsigned blocks = (unsigned)((2ull * count + 255) / 256);
         k_grind_pass_pair<<<blocks, 256, 0, s>>>((const P2DevT *)pc, st, pos, mask, base, count, state, host_dst,
-                                                 flag, seq);
+                                                 flag, seq, publish);
         return;
     }
     k_grind_pass<<<nblk(count), 256, 0, s>>>((const P2DevT *)pc, st, pos, mask, base, count, state, host_dst, flag,
-                                            seq);
+                                            seq, publish);
 }
 __global__ void k_p2_batch(const P2DevT *pc, int width, F *st, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

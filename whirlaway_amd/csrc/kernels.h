@@ -186,10 +186,13 @@ void merkle_open(const F *cw, const F *nodes, int width, int nnz, int log_rows, 
 // folded query values: out[q] = sum_b eqr[b] * row_q[b], rows of 2^fold F (base)
 // or 2^fold EF (8 F each, coefficient-wise) values
 void fold_rows(const F *rows, int nq, int fold, bool base, const EF *eqr, EF *out, hipStream_t s);
-// minimal w in [base, base + count) with P16(state with w at position pos)[7] & mask == 0
-// (0xffffffff: none) to host_dst[0], then seq to *flag; state = {0xffffffff, 0} on entry and is left so
+// minimal w in [base, base + count) with P16(state with w at position pos)[7] & mask == 0,
+// skipped when an earlier pass of the group found one; a publishing pass ends the group:
+// the group's minimum (0xffffffff: none) to host_dst[0], then seq to *flag.  state =
+// {0xffffffff, 0, -, 0xffffffff} on a group's entry and is left so at its end.
 void grind_pass(const void *p16, State16 st, int pos, uint32_t mask, uint32_t base, uint32_t count,
-                uint32_t *state, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq, hipStream_t s);
+                uint32_t *state, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq, bool publish,
+                hipStream_t s);
 // device Poseidon2 (tests): permute n states of width W in place
 void p2_permute_batch(const void *consts, int width, F *states, int n, hipStream_t s);
 

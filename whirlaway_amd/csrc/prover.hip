@@ -17,6 +17,7 @@
 #include <vector>
 #include "../../include/whirlaway_hip.h"
 #include "ctx.h"
+#include "devts.h"
 #include "sc_generic.h"
 
 using namespace kb;
@@ -511,6 +512,27 @@ void whir_prove(wl_ctx *c, const WhirCfg &cfg, const P2Dev *pdev, Transcript &ts
 }
 
 // --------------------------------------------------------------- AIR prove --
+// the host transcript's duplex sponge <-> its device copy
+void sponge_to_device(const Transcript &ts, wk::DevSponge &d) {
+    for (int i = 0; i < 16; i++) d.st[i] = ts.state[i];
+    for (int i = 0; i < 8; i++) { d.inb[i] = ts.in_buf[i]; d.outb[i] = ts.out_buf[i]; }
+    d.n_in = (uint32_t)ts.n_in;
+    d.n_out = (uint32_t)ts.n_out;
+}
+void sponge_from_device(const wk::DevSponge &d, Transcript &ts) {
+    for (int i = 0; i < 16; i++) ts.state[i] = d.st[i];
+    for (int i = 0; i < 8; i++) { ts.in_buf[i] = d.inb[i]; ts.out_buf[i] = d.outb[i]; }
+    ts.n_in = (int)d.n_in;
+    ts.n_out = (int)d.n_out;
+}
+// the Lagrange basis on the nodes 0..3 (coefficient rows) for the device rounds
+wk::Basis4 basis4() {
+    wk::Basis4 b;
+    const std::vector<F> &B = wh::int_basis(4);
+    for (int i = 0; i < 16; i++) b.b[i] = B[i];
+    return b;
+}
+
 // AirTable::build_whir_params (crates/air/src/table.rs:72-98): the committed
 // polynomial has log_length + log2_up(n_witness_columns) variables
 WhirCfg air_whir_config(uint32_t log_length, uint32_t n_witness_columns, const wl_air_settings *st) {
@@ -639,11 +661,47 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
         }
         return reduce_async_await(c, parts, 3, (int)nbx, 8, seq);
     };
+    // Rounds without grinding can run back to back with the transcript on the
+    // device: each round's reduction kernel absorbs the round polynomial into the
+    // device sponge and samples the challenge the next fold reads; the host reads
+    // the sponge, the proof words and the challenges back once.
+    const bool device_rounds = c->device_transcript && nz > 1 && nz < wk::DTS_MAX_ROUNDS &&
+                               pow_bits_sumcheck(sec, 3 + 1) == 0;
+    if (device_rounds) {
+        wk::ZcRounds zr;
+        memset(&zr, 0, sizeof zr);
+        sponge_to_device(ts, zr.sp);
+        zr.sum = sum;
+        zr.missing = missing;
+        for (int j = 1; j < nz; j++) { zr.r[j] = zc[j]; zr.rinv[j] = zc_inv[j]; }
+        DBuf<wk::ZcRounds> dz(c->pool, 1);
+        c->h2d_async(dz.p, &zr, 1);
+        const wk::Basis4 basis = basis4();
+        int ll = l;
+        for (int j = 1; j < nz; j++) {
+            if (j > 1) wk::fold_cols_p(U, AIR_NCOLS, M, ll + 1, &dz.p->chal[j - 1], eq, c->stream);
+            const int nbx = wk::zc_round_wave_blocks(ll);
+            DBuf<EF> parts(c->pool, 3 * (size_t)nbx);
+            {
+                KTimer kt(c, KT_ZC_ROUND, 3.0 * (double)((size_t)1 << (ll - 1)));
+                wk::zc_round_wave(U, M, ll, t->air_dev, t->chain_dev, dap, eq, parts, c->stream);
+            }
+            wk::zc_ts_round(parts, nbx, dz.p, j, basis, pdev, c->stream);
+            ll--;
+        }
+        wk::fold_cols_p(U, AIR_NCOLS, M, ll + 1, &dz.p->chal[nz - 1], eq, c->stream);
+        c->fetch(&zr, dz.p, sizeof zr);
+        sponge_from_device(zr.sp, ts);
+        ts.data.insert(ts.data.end(), zr.words, zr.words + 40 * (size_t)(nz - 1));
+        for (int j = 1; j < nz; j++) ch.push_back(zr.chal[j]);
+        sum = zr.sum;
+        missing = zr.missing;
+    }
     DBuf<EF> parts_cur, parts_next;
     uint32_t seq_cur = 0;
     wl_ctx::HostSlot slot_cur{};
-    if (nz > 1) slot_cur = enqueue_round(l, nullptr, parts_cur, seq_cur);
-    for (int j = 1; j < nz; j++) {
+    if (nz > 1 && !device_rounds) slot_cur = enqueue_round(l, nullptr, parts_cur, seq_cur);
+    for (int j = 1; j < nz && !device_rounds; j++) {
         EF r = zc[j];
         uint32_t seq_next = 0;
         wl_ctx::HostSlot slot_next{};
@@ -980,6 +1038,8 @@ wl_ctx *wl_create(int device) {
         HIP_CHECK(hipSetDevice(device));
         wl_ctx *c = new wl_ctx();
         c->device = device;
+        // WHIRLAWAY_DEVICE_ROUNDS=1 moves the zerocheck rounds' transcript to the device
+        if (const char *e = getenv("WHIRLAWAY_DEVICE_ROUNDS")) c->device_transcript = e[0] == '1';
         HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreate(&c->ev0));
         HIP_CHECK(hipEventCreate(&c->ev1));
