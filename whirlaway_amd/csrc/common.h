@@ -40,11 +40,42 @@ constexpr DiagTables make_diag() {
 constexpr DiagTables DIAG = make_diag();
 __device__ __constant__ static const DiagTables DDIAG = make_diag();
 
+// Tables of the partial rounds by convolution (PartialConv): powers of the
+// internal diagonal's entries 1..W-1 and their column sums, from the canonical
+// diagonal (d = Montgomery image / 2^32, recovered as mont * R^-1).
+constexpr uint64_t cmulmod(uint64_t a, uint64_t b) { return a * b % P; }
+constexpr uint64_t from_cmont(uint32_t m) {  // m R^-1 mod P, R^-1 = (2^32 mod P)^(P-2)
+    uint64_t b = (1ull << 32) % P, e = P - 2, inv = 1;
+    while (e) { if (e & 1) inv = cmulmod(inv, b); b = cmulmod(b, b); e >>= 1; }
+    return cmulmod(m, inv);
+}
+template <int W, int RP>
+constexpr PartialConv<W, RP> make_pconv(const uint32_t *diag_mont) {
+    PartialConv<W, RP> t{};
+    uint64_t pw[W];
+    for (int i = 1; i < W; i++) pw[i] = 1;
+    for (int k = 1; k <= RP; k++) {
+        uint64_t cs = 0;
+        for (int i = 1; i < W; i++) {
+            pw[i] = cmulmod(pw[i], from_cmont(diag_mont[i]));
+            t.dpow[k - 1][i - 1] = cmont(pw[i]);
+            cs = (cs + pw[i]) % P;
+        }
+        t.csum[k - 1] = cmont(cs);
+    }
+    t.wm1 = cmont(W - 1);
+    return t;
+}
+constexpr PartialConv<16, 20> PCONV16 = make_pconv<16, 20>(DIAG.d16);
+constexpr PartialConv<24, 23> PCONV24 = make_pconv<24, 23>(DIAG.d24);
+__device__ __constant__ static const PartialConv<16, 20> DPCONV16 = make_pconv<16, 20>(DIAG.d16);
+__device__ __constant__ static const PartialConv<24, 23> DPCONV24 = make_pconv<24, 23>(DIAG.d24);
+
 // Poseidon2 with the fixed KoalaBear diagonals.
 template <typename C>
-__device__ __forceinline__ void p16_permute(F *s, const C &k) { p2_permute<16, 20>(s, k, DDIAG.d16); }
+__device__ __forceinline__ void p16_permute(F *s, const C &k) { p2_permute_conv<16, 20>(s, k, DPCONV16); }
 template <typename C>
-__device__ __forceinline__ void p24_permute(F *s, const C &k) { p2_permute<24, 23>(s, k, DDIAG.d24); }
+__device__ __forceinline__ void p24_permute(F *s, const C &k) { p2_permute_conv<24, 23>(s, k, DPCONV24); }
 
 // Lazy accumulator of EF x F products: 64-bit per coefficient (4 products of
 // values < P stay below 2^64 after a fold64), reduced once in get().
