@@ -447,12 +447,21 @@ __global__ void __launch_bounds__(256) k_merkle_leaves(const F *cw, int width, i
     F s[24];
 #pragma unroll
     for (int i = 0; i < 24; i++) s[i] = F{0};
+    // the next block's 16 values are loaded before the current permutation, so
+    // their memory latency hides behind it (a commit with few rows has one wave
+    // per SIMD and nothing else to hide it)
+    F nxt[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) nxt[j] = j < width && j < nnz ? cw[(size_t)j * rows + q] : F{0};
     for (int off = 0; off < width; off += 16) {
         int n = width - off < 16 ? width - off : 16;
         // compile-time state indices (a runtime index would place the state in scratch)
 #pragma unroll
         for (int j = 0; j < 16; j++)
-            if (j < n) s[j] = off + j < nnz ? cw[(size_t)(off + j) * rows + q] : F{0};
+            if (j < n) s[j] = nxt[j];
+        const int nof = off + 16;
+#pragma unroll
+        for (int j = 0; j < 16; j++) nxt[j] = nof + j < width && nof + j < nnz ? cw[(size_t)(nof + j) * rows + q] : F{0};
         p24_permute(s, pc->p24);
     }
     F *o = nodes + (rows + r) * 8;
