@@ -96,6 +96,13 @@ struct wl_ctx {
     // round trip they replace)
     bool device_transcript = false;
     hipStream_t stream = nullptr;
+    // a first codeword encoded while the witness uploaded (commit_codeword takes it
+    // instead of encoding cols again): the source it encodes, its shape, the buffer
+    struct PreEncoded {
+        const kb::F *cols = nullptr;
+        int log_rows = 0, n_cols = 0, nnz = 0, la = 0;
+        wl::DBuf<kb::F> cw;
+    } pre0;
     wl::Pool pool;
     uint32_t *pinned = nullptr;   // host staging
     size_t pinned_words = 0;
@@ -140,16 +147,38 @@ struct wl_ctx {
     // host copies, instead of the runtime's single staging path for pageable
     // memory.  Returns after every DMA is enqueued; later work on the stream is
     // ordered behind them.
+    // With `ready`, the DMAs run on a copy stream instead, and as each chunk lands
+    // the work stream waits for it and ready(u0, u1) enqueues work on the units
+    // [u0, u1) of `unit` bytes that are now complete (the last one may be partial),
+    // so that work overlaps the rest of the copy.
     static constexpr int STAGE_THREADS = 6, STAGE_SLOTS = 4;
     static constexpr size_t CHUNK = (size_t)16 << 20, FIRST_CHUNK = (size_t)2 << 20;
     uint8_t *stage_slots = nullptr;
     hipEvent_t stage_ev[STAGE_SLOTS] = {};
     bool stage_used[STAGE_SLOTS] = {};
-    void h2d_staged(void *dst, const void *src, size_t bytes) {
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_order = nullptr;
+    void h2d_staged(void *dst, const void *src, size_t bytes, size_t unit = 0,
+                    const std::function<void(size_t, size_t)> &ready = nullptr) {
+        const size_t n_units = unit ? (bytes + unit - 1) / unit : 0;
         if (bytes < 4 * FIRST_CHUNK) {
             HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+            if (ready) ready(0, n_units);
             return;
         }
+        hipStream_t dma = stream;
+        if (ready) {
+            if (!copy_stream) {
+                HIP_CHECK(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+                HIP_CHECK(hipEventCreateWithFlags(&copy_order, hipEventDisableTiming));
+            }
+            // the copies start after the work already on the stream (the destination
+            // may be pool memory that this work still reads)
+            HIP_CHECK(hipEventRecord(copy_order, stream));
+            HIP_CHECK(hipStreamWaitEvent(copy_stream, copy_order, 0));
+            dma = copy_stream;
+        }
+        size_t units_done = 0;
         if (!stage_slots) {
             HIP_CHECK(hipHostMalloc((void **)&stage_slots, STAGE_SLOTS * CHUNK, hipHostMallocDefault));
             for (auto &e : stage_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -198,17 +227,29 @@ struct wl_ctx {
                 const int slot = (int)(k % STAGE_SLOTS);
                 const size_t off = offs[k], n = offs[k + 1] - off;
                 HIP_CHECK(hipMemcpyAsync((uint8_t *)dst + off, stage_slots + (size_t)slot * CHUNK, n,
-                                         hipMemcpyHostToDevice, stream));
-                HIP_CHECK(hipEventRecord(stage_ev[slot], stream));
+                                         hipMemcpyHostToDevice, dma));
+                HIP_CHECK(hipEventRecord(stage_ev[slot], dma));
                 stage_used[slot] = true;
                 issued.store(k + 1, std::memory_order_release);
+                if (ready) {
+                    const size_t u = k + 1 == nch ? n_units : (off + n) / unit;
+                    if (u > units_done) {
+                        HIP_CHECK(hipStreamWaitEvent(stream, stage_ev[slot], 0));
+                        ready(units_done, u);
+                        units_done = u;
+                    }
+                }
             }
         } catch (const std::exception &e) {
             err = e.what();
             failed.store(true);
         }
         for (auto &th : pool) th.join();
-        if (!err.empty()) throw std::runtime_error(err);
+        if (!err.empty()) {
+            // no copy may still be writing once the caller returns the destination
+            if (ready) (void)hipStreamSynchronize(copy_stream);
+            throw std::runtime_error(err);
+        }
     }
     uint32_t *stage(size_t words) {
         if (words > pinned_words) {

@@ -46,14 +46,20 @@ struct Codeword {
 void commit_codeword(wl_ctx *c, const P2Dev *pdev, Codeword &cwd, const F *cols, int n_cols,
                      int n_nonzero, int log_m, int log_rows) {
     size_t rows = (size_t)1 << log_rows;
-    cwd.cw = DBuf<F>(c->pool, (size_t)std::max(1, n_nonzero) * rows);  // zero columns are not stored
     cwd.nodes = DBuf<F>(c->pool, 2 * rows * 8);
     cwd.width = n_cols;
     cwd.nnz = n_nonzero;
     cwd.log_rows = log_rows;
-    DBuf<F> tmp(c->pool, (size_t)std::max(1, n_nonzero) * rows);
-    F *tw = c->twiddle(log_rows);
-    {
+    wl_ctx::PreEncoded &pre = c->pre0;
+    if (pre.cw.p && pre.cols == cols && pre.log_rows == log_rows && pre.n_cols == n_cols && pre.nnz == n_nonzero) {
+        // encoded while the witness uploaded
+        cwd.cw = std::move(pre.cw);
+        cwd.la = pre.la;
+        pre = wl_ctx::PreEncoded();
+    } else {
+        cwd.cw = DBuf<F>(c->pool, (size_t)std::max(1, n_nonzero) * rows);  // zero columns are not stored
+        DBuf<F> tmp(c->pool, (size_t)std::max(1, n_nonzero) * rows);
+        F *tw = c->twiddle(log_rows);
         // algorithmic HBM bytes: read the evaluations, write the codeword columns
         KTimer kt(c, KT_NTT, 4.0 * (double)n_nonzero * (double)(rows + ((size_t)1 << log_m)));
         cwd.la = wk::rs_encode(cols, n_cols, n_nonzero, log_m, log_rows, cwd.cw, tmp, tw, c->ntt_tabs(log_rows),
@@ -1075,6 +1081,8 @@ void wl_destroy(wl_ctx *c) {
         for (auto &e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
+    if (c->copy_order) (void)hipEventDestroy(c->copy_order);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1248,8 +1256,39 @@ static int prove_host_witness(wl_table *t, const wl_air_settings *st, const uint
         wl_ctx *c = t->ctx;
         HIP_CHECK(hipSetDevice(c->device));
         const uint32_t nw = witness_cols(t);
-        DBuf<F> packed(c->pool, packed_len(t->log_length, nw));
-        upload_packed(c, witness, t->log_length, nw, packed);
+        const size_t N = (size_t)1 << t->log_length, V = packed_len(t->log_length, nw);
+        DBuf<F> packed(c->pool, V);
+        // The first commitment's Reed-Solomon encoding (WhirConfig of the table, as in
+        // whir_commit) runs column group by column group while the rest of the
+        // witness is still on its way; commit_codeword then takes the codeword.
+        const WhirCfg cfg = air_whir_config(t->log_length, nw, st);
+        const int f0 = cfg.f0, log_m = cfg.num_variables - f0;
+        const int log_rows = cfg.num_variables + cfg.starting_log_inv_rate - f0;
+        const size_t colen = (size_t)1 << log_m, rows = (size_t)1 << log_rows;
+        const int n_cols = 1 << f0;
+        const int nnz = (int)std::min<size_t>(n_cols, (nw * N + colen - 1) / colen);
+        DBuf<F> pcw(c->pool, (size_t)nnz * rows), tmp(c->pool, (size_t)nnz * rows);
+        F *tw = c->twiddle(log_rows);
+        const F *tabs = c->ntt_tabs(log_rows);
+        wk::fill_zero(packed + nw * N, (V - nw * N) * 4, c->stream);
+        int la = 0;
+        c->h2d_staged(packed, witness, nw * N * 4, colen * 4, [&](size_t u0, size_t u1) {
+            const size_t e0 = u0 * colen, e1 = std::min(u1 * colen, nw * N);
+            wk::convert_to_mont((uint32_t *)packed.p + e0, e1 - e0, c->stream);
+            la = wk::rs_encode(packed.p + e0, (int)(u1 - u0), (int)(u1 - u0), log_m, log_rows, pcw.p + u0 * rows,
+                               tmp.p + u0 * rows, tw, tabs, c->stream, false);
+        });
+        tmp.reset();
+        c->pre0.cols = packed.p;
+        c->pre0.log_rows = log_rows;
+        c->pre0.n_cols = n_cols;
+        c->pre0.nnz = nnz;
+        c->pre0.la = la;
+        c->pre0.cw = std::move(pcw);
+        struct Unclaimed {  // an encoding the proof did not take goes back to the pool
+            wl_ctx *c;
+            ~Unclaimed() { c->pre0 = wl_ctx::PreEncoded(); }
+        } unclaimed{c};
         return prove_common(t, st, p16c, p24c, packed, proof_out, proof_cap, proof_len, ts);
     } catch (const std::exception &e) {
         g_err = e.what();
