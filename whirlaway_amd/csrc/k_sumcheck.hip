@@ -502,37 +502,51 @@ void fold_cols_p(EF *U, int n_cols, size_t stride, int l, const EF *c, EF *E, hi
     k_fold_cols_p<<<dim3(nblk(h), n_cols + (E && h > 1 ? 1 : 0)), 256, 0, s>>>(U, stride, l, n_cols, c, E);
 }
 
-// down inner sums
-__global__ void k_zc_down_parts(const F *trace, int log_n, int skips, const EF *eqr, EF *parts) {
+// down inner sums: a block takes DOWN_CPB columns of one segment j, so each eq
+// weight it loads serves that many columns, with two elements per thread in flight
+constexpr int DOWN_CPB = 4;
+__global__ void __launch_bounds__(256) k_zc_down_parts(const F *trace, int log_n, int skips, const EF *eqr,
+                                                       EF *parts) {
     const size_t N = (size_t)1 << log_n;
     const size_t M = N >> skips;
-    const int c = blockIdx.x, j = blockIdx.y, K = 1 << skips;
-    const F *col = trace + (size_t)c * N;
-    AccF acc;
-    size_t i = threadIdx.x;
-    // eight loads in flight per thread
-    for (; i + 7 * (size_t)blockDim.x < M; i += 8 * (size_t)blockDim.x) {
-        F v[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            size_t idx = (size_t)j * M + i + (size_t)q * blockDim.x + 1;
-            if (idx > N - 1) idx = N - 1; /* column_down: last row repeats */
-            v[q] = col[idx];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++) acc.add(eqr[i + (size_t)q * blockDim.x], v[q]);
-    }
-    for (; i < M; i += blockDim.x) {
+    const int c0 = blockIdx.x * DOWN_CPB, j = blockIdx.y, K = 1 << skips;
+    AccF acc[DOWN_CPB];
+    const size_t bd = blockDim.x;
+    auto load = [&](size_t i, F *v) {
         size_t idx = (size_t)j * M + i + 1;
-        if (idx > N - 1) idx = N - 1;
-        acc.add(eqr[i], col[idx]);
+        if (idx > N - 1) idx = N - 1; /* column_down: last row repeats */
+#pragma unroll
+        for (int q = 0; q < DOWN_CPB; q++) v[q] = c0 + q < AIR_NCOLS ? trace[(size_t)(c0 + q) * N + idx] : F{0};
+    };
+    size_t i = threadIdx.x;
+    for (; i + bd < M; i += 2 * bd) {
+        F v0[DOWN_CPB], v1[DOWN_CPB];
+        load(i, v0);
+        load(i + bd, v1);
+        const EF e0 = eqr[i], e1 = eqr[i + bd];
+#pragma unroll
+        for (int q = 0; q < DOWN_CPB; q++) {
+            acc[q].add(e0, v0[q]);
+            acc[q].add(e1, v1[q]);
+        }
+    }
+    for (; i < M; i += bd) {
+        F v[DOWN_CPB];
+        load(i, v);
+        const EF e = eqr[i];
+#pragma unroll
+        for (int q = 0; q < DOWN_CPB; q++) acc[q].add(e, v[q]);
     }
     __shared__ EF lds[16];
-    EF t = block_sum(acc.get(), lds);
-    if (threadIdx.x == 0) parts[(size_t)c * K + j] = t;
+#pragma unroll
+    for (int q = 0; q < DOWN_CPB; q++) {
+        EF t = block_sum(acc[q].get(), lds);
+        if (threadIdx.x == 0 && c0 + q < AIR_NCOLS) parts[(size_t)(c0 + q) * K + j] = t;
+    }
 }
 void zc_down_parts(const F *trace, int log_n, int skips, const EF *eqr, EF *parts, hipStream_t s) {
-    k_zc_down_parts<<<dim3(AIR_NCOLS, 1 << skips), 256, 0, s>>>(trace, log_n, skips, eqr, parts);
+    k_zc_down_parts<<<dim3((AIR_NCOLS + DOWN_CPB - 1) / DOWN_CPB, 1 << skips), 256, 0, s>>>(trace, log_n, skips,
+                                                                                           eqr, parts);
 }
 __global__ void k_zc_down_combine(const EF *parts, int K, const EF *S, EF *out) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
