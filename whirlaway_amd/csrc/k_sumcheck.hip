@@ -364,12 +364,13 @@ int zc_round0(const F *trace, int log_n, int skips, const F *sel, int nz, const 
 
 // Round 0 in two passes.  Pass 1: for each column c and x, the 2^skips values
 // up_c[j*M + x] are those of a polynomial of degree < K = 2^skips in the skipped
-// variable at j = 0..K-1; its values at z = K, K+1, ... follow from the
-// backward-difference table at j = K-1 (a_k = nabla^k f(K-1); one step:
-// a_k += a_{k+1} for k = K-2..0, then a_0 = f(next)) -- additions only.
+// variable at j = 0..K-1; its value at z is sum_j L_j(z) up_c[j*M + x] with the
+// Lagrange basis L_j of the nodes 0..K-1 (sel[zi][j] = L_j(K + zi), Montgomery):
+// K products into one 64-bit sum (a fold after every four), one reduction.
 // U0[zi][c][x] = f_c,x(K + zi), zi < nz.
 template <int K>
-__global__ void __launch_bounds__(256) k_zc_lde(const F *trace, int log_n, int nz, F *U0) {
+__global__ void __launch_bounds__(256) k_zc_lde(const F *__restrict__ trace, int log_n, int nz,
+                                                const F *__restrict__ sel, F *__restrict__ U0) {
     const size_t N = (size_t)1 << log_n;
     const size_t M = N / K;
     const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -383,21 +384,18 @@ __global__ void __launch_bounds__(256) k_zc_lde(const F *trace, int log_n, int n
         if (idx == N - 1) idx = N - 2;  // column_up: the last row repeats the one before
         v[j] = col[idx];
     }
-    // forward differences: after pass k, v[i] = Delta^k f(i) for i <= K-1-k, and
-    // v[K-1-k] = Delta^k f(K-1-k) = nabla^k f(K-1) stays untouched afterwards
-#pragma unroll
-    for (int k = 1; k < K; k++)
-#pragma unroll
-        for (int i = 0; i < K - k; i++) v[i] = fsub(v[i + 1], v[i]);
-    F a[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) a[k] = v[K - 1 - k];
     F *out = U0 + (size_t)c * M + x;
     const size_t zstride = (size_t)AIR_NCOLS * M;
+#pragma unroll 2
     for (int zi = 0; zi < nz; zi++) {
+        const F *L = sel + (size_t)zi * K;
+        uint64_t a = 0;
 #pragma unroll
-        for (int k = K - 2; k >= 0; k--) a[k] = fadd(a[k], a[k + 1]);
-        out[(size_t)zi * zstride] = a[0];
+        for (int j = 0; j < K; j++) {
+            if (j && (j & 3) == 0) a = fold64(a);
+            a += (uint64_t)v[j].v * L[j].v;
+        }
+        out[(size_t)zi * zstride] = F{redc64(a)};
     }
 }
 
@@ -417,15 +415,15 @@ __global__ void __launch_bounds__(256) k_zc_air0(const F *U0, size_t M, const Ai
     if (threadIdx.x == 0) parts[(size_t)zi * gridDim.x + blockIdx.x] = t;
 }
 
-int zc_round0_lde(const F *trace, int log_n, int skips, int nz, const AirConsts *air, const EF *ap,
+int zc_round0_lde(const F *trace, int log_n, int skips, int nz, const F *sel, const AirConsts *air, const EF *ap,
                   const EF *eq, F *U0, EF *parts, hipStream_t s) {
     const size_t M = (size_t)1 << (log_n - skips);
     const dim3 g1(nblk(M), AIR_NCOLS);
     switch (skips) {
-        case 1: k_zc_lde<2><<<g1, 256, 0, s>>>(trace, log_n, nz, U0); break;
-        case 2: k_zc_lde<4><<<g1, 256, 0, s>>>(trace, log_n, nz, U0); break;
-        case 3: k_zc_lde<8><<<g1, 256, 0, s>>>(trace, log_n, nz, U0); break;
-        case 4: k_zc_lde<16><<<g1, 256, 0, s>>>(trace, log_n, nz, U0); break;
+        case 1: k_zc_lde<2><<<g1, 256, 0, s>>>(trace, log_n, nz, sel, U0); break;
+        case 2: k_zc_lde<4><<<g1, 256, 0, s>>>(trace, log_n, nz, sel, U0); break;
+        case 3: k_zc_lde<8><<<g1, 256, 0, s>>>(trace, log_n, nz, sel, U0); break;
+        case 4: k_zc_lde<16><<<g1, 256, 0, s>>>(trace, log_n, nz, sel, U0); break;
         default: return -1;
     }
     const unsigned nb = nblk(M);

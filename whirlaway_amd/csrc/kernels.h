@@ -63,10 +63,10 @@ struct State16 { F s[16]; };
 int zc_round0(const F *trace, int log_n, int skips, const F *sel, int nz, const AirConsts *air,
               const EF *alpha_pows, const EF *eq, EF *parts, hipStream_t s);
 // The same sums for z = 2^skips .. 2^skips + nz - 1 in two passes (skips <= 4):
-// the extended rows U0 (nz x 165 x M F scratch) by finite differences, then the
-// AIR per (x, z).  Returns n_blocks, or -1 when skips is not supported.
-int zc_round0_lde(const F *trace, int log_n, int skips, int nz, const AirConsts *air, const EF *alpha_pows,
-                  const EF *eq, F *U0, EF *parts, hipStream_t s);
+// the extended rows U0 (nz x 165 x M F scratch) from the selector rows sel, then
+// the AIR per (x, z).  Returns n_blocks, or -1 when skips is not supported.
+int zc_round0_lde(const F *trace, int log_n, int skips, int nz, const F *sel, const AirConsts *air,
+                  const EF *alpha_pows, const EF *eq, F *U0, EF *parts, hipStream_t s);
 // U[c][x] = sum_j S_j * up_c[j*M + x]  (S: 2^skips EF on device)
 void zc_fold0(const F *trace, int log_n, int skips, const EF *S, EF *U, hipStream_t s);
 // Rounds >= 1: U holds 165 EF columns (column c at U + c * stride) of current

@@ -601,19 +601,21 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
         DBuf<EF> parts(c->pool, (size_t)nzp * nbx);
         {
             KTimer kt(c, KT_ZC_ROUND0, (double)nzp * M);
-            if (skips <= 4) {
-                // consecutive z = K .. zmax: rows extended by finite differences
-                DBuf<F> U0(c->pool, (size_t)nzp * AIR_NCOLS * M);
-                wk::zc_round0_lde(packed, n, skips, nzp, t->air_dev, dap, eq, U0, parts, c->stream);
-            } else {
-                // selector rows of the nzp points (only this generic pass reads them)
-                std::vector<F> sel;
+            // selector rows of the nzp points z = K .. zmax (fixed by skips)
+            static thread_local std::vector<F> sel_cache[32];
+            std::vector<F> &sel = sel_cache[skips];
+            if (sel.empty())
                 for (int z = K; z <= zmax; z++) {
                     std::vector<F> s = wh::selectors_f(skips, fsmall(z));
                     sel.insert(sel.end(), s.begin(), s.end());
                 }
-                DBuf<F> dsel(c->pool, sel.size());
-                c->h2d_async(dsel.p, sel.data(), sel.size());
+            DBuf<F> dsel(c->pool, sel.size());
+            c->h2d_async(dsel.p, sel.data(), sel.size());
+            if (skips <= 4) {
+                // the rows extended to every z first, then the AIR per (x, z)
+                DBuf<F> U0(c->pool, (size_t)nzp * AIR_NCOLS * M);
+                wk::zc_round0_lde(packed, n, skips, nzp, dsel, t->air_dev, dap, eq, U0, parts, c->stream);
+            } else {
                 wk::zc_round0(packed, n, skips, dsel, nzp, t->air_dev, dap, eq, parts, c->stream);
             }
         }
