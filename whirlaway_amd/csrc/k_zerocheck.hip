@@ -13,11 +13,14 @@
 //     computed once per state (32 block tasks, then rows = own block output +
 //     the four blocks' sum of its residue class), not once per row.
 // The folded column values of each point live in LDS, the constants (alpha
-// powers, round constants, chain coefficients) are read through the caches.  Large rounds give each
-// point one wave (4 points per block); small rounds, which are latency-bound,
-// give each point the whole 256-lane block.
+// powers, round constants, chain coefficients) are read through the caches.  The
+// largest rounds (2^11 points and more) give each point 16 lanes, one state
+// element per lane (k_zc_round_r16); the 2^10-point round gives each point one
+// wave (4 points per block); smaller rounds, which are latency-bound, give each
+// point the whole 256-lane block.
 #include "common.h"
 #include "kernels.h"
+#include "p16_lanes.h"
 
 namespace wk {
 using namespace kb;
@@ -187,8 +190,96 @@ __global__ void __launch_bounds__(256) k_zc_round_w(const EF *U, size_t stride, 
     }
 }
 
+// 16 lanes per (point, z), 16 points per block, for the large rounds.  Lane i of a
+// point's 16-lane DPP row holds state element i: the external layers are
+// ext_row16 on each of the 8 coefficients, every S-box and constraint of a full
+// round is one per lane, and the 36 chain dot products (20 partial rounds, the 16
+// inputs of ending round 0) are 2 or 3 per lane, reading the point's 36 chain
+// inputs from LDS.  Same sum as k_zc_round_w (the constraints only add in
+// another order).
+__device__ __forceinline__ EF ext_row16_ef(EF v) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) v.c[c] = ext_row16(v.c[c]);
+    return v;
+}
+__global__ void __launch_bounds__(256) k_zc_round_r16(const EF *__restrict__ U, size_t stride, int l,
+                                                      const AirConsts *__restrict__ air,
+                                                      const ChainLin *__restrict__ cl, const EF *__restrict__ ap,
+                                                      const EF *__restrict__ eq, EF *__restrict__ parts) {
+    __shared__ EF chin[16][36];
+    __shared__ EF vitem[16];
+    const int item = threadIdx.x >> 4, lane = threadIdx.x & 15;
+    const size_t h = (size_t)1 << (l - 1);  // a multiple of 16: every item is a point
+    const int zi = blockIdx.y;
+    const int z = zi ? zi + 1 : 0;  // 0, 2, 3
+    const size_t x = (size_t)blockIdx.x * 16 + item;
+    auto col = [&](int c) -> EF {
+        const EF *p = U + (size_t)c * stride;
+        EF a = p[x];
+        if (z) {
+            const EF d = esub(p[x + h], a);
+            a = eadd(a, edbl(d));
+            if (z == 3) a = eadd(a, d);
+        }
+        return a;
+    };
+    for (int t = lane; t < 36; t += 16) chin[item][t] = col(65 + t);
+    // the state entering beginning round 0: the external layer of the inputs
+    EF s = ext_row16_ef(col(1 + lane));
+    EF acc = ezero();
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+        const EF y = ext_row16_ef(ecube(add_rc(s, air->rc[r * 16 + lane])));
+        const EF post = col(17 + 16 * r + lane);
+        acc = eadd(acc, emul(ap[1 + 16 * r + lane], esub(y, post)));
+        s = post;
+    }
+    __syncthreads();
+    const EF *in = chin[item];
+    auto dot = [&](const F *ca, const F *cb) {
+        AccF d;
+#pragma unroll 4
+        for (int j = 0; j < 16; j++) d.add(in[j], ca[j]);
+#pragma unroll 4
+        for (int q = 0; q < 20; q++) d.add(in[16 + q], cb[q]);
+        return d.get();
+    };
+    auto partial = [&](int hh) {
+        const EF tt = ecube(add_rc(dot(cl->A[hh], cl->B[hh]), air->rc[64 + hh]));
+        acc = eadd(acc, emul(ap[65 + hh], esub(tt, in[16 + hh])));
+    };
+    partial(lane);
+    if (lane < 4) partial(16 + lane);
+    // ending round 0: its S-box on the state the chain leaves
+    s = ecube(add_rc(dot(cl->Ei[lane], cl->Ep[lane]), air->rc[84 + lane]));
+#pragma unroll 1
+    for (int er = 0; er < 4; er++) {
+        if (er) s = ecube(add_rc(s, air->rc[84 + er * 16 + lane]));
+        const EF y = ext_row16_ef(s);
+        const EF post = col(101 + 16 * er + lane);
+        acc = eadd(acc, emul(ap[85 + 16 * er + lane], esub(y, post)));
+        s = post;
+    }
+    if (lane == 0) {
+        const EF e = col(0);
+        acc = eadd(acc, emul(ap[0], emul(e, esub(e, eone()))));
+    }
+#pragma unroll
+    for (int c = 0; c < 8; c++) acc.c[c] = row_sum16(acc.c[c]);
+    if (lane == 0) vitem[item] = emul(acc, eq[x]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        EF t = ezero();
+        for (int i = 0; i < 16; i++) t = eadd(t, vitem[i]);
+        parts[(size_t)zi * gridDim.x + blockIdx.x] = t;
+    }
+}
+
 // points per block for a round with 2^(l-1) points per z
-static int pts_per_block(int l) { return (((size_t)1 << (l - 1)) >= 1024) ? 4 : 1; }
+static int pts_per_block(int l) {
+    const size_t h = (size_t)1 << (l - 1);
+    return h >= 2048 ? 16 : h >= 1024 ? 4 : 1;
+}
 
 int zc_round_wave_blocks(int l) {
     size_t h = (size_t)1 << (l - 1);
@@ -199,7 +290,9 @@ int zc_round_wave_blocks(int l) {
 int zc_round_wave(const EF *U, size_t stride, int l, const AirConsts *air, const ChainLin *cl,
                   const EF *ap, const EF *eq, EF *parts, hipStream_t s) {
     int nb = zc_round_wave_blocks(l);
-    if (pts_per_block(l) == 4)
+    if (pts_per_block(l) == 16)
+        k_zc_round_r16<<<dim3(nb, 3), 256, 0, s>>>(U, stride, l, air, cl, ap, eq, parts);
+    else if (pts_per_block(l) == 4)
         k_zc_round_w<4><<<dim3(nb, 3), 256, 0, s>>>(U, stride, l, air, cl, ap, eq, parts);
     else
         k_zc_round_w<1><<<dim3(nb, 3), 256, 0, s>>>(U, stride, l, air, cl, ap, eq, parts);

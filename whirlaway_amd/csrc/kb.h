@@ -94,6 +94,22 @@ KB_HD F ftimes3(F a) { return fadd(fdbl(a), a); }
 
 // c_k = sum_{i<=k} a_i b_{k-i} + sum_{i>k} a_i (3 b_{k+8-i}); eight products
 // per coefficient in one 64-bit sum (shrunk by fold64 after four), one reduction.
+// b3 = 3 b, coefficient-wise (a caller multiplying by the same b twice passes it).
+KB_HD EF emul_b3(const EF &a, const EF &b, const uint32_t *b3) {
+    EF r;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        uint64_t s = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            uint32_t bb = i <= k ? b.c[k - i].v : b3[k + 8 - i];
+            s += (uint64_t)a.c[i].v * bb;
+            if (i == 3) s = fold64(s);
+        }
+        r.c[k] = F{redc64(s)};
+    }
+    return r;
+}
 KB_HD EF emul(const EF &a, const EF &b) {
     uint32_t b3[8];
 #pragma unroll
@@ -114,7 +130,13 @@ KB_HD EF emul(const EF &a, const EF &b) {
 }
 
 KB_HD EF esqr(const EF &a) { return emul(a, a); }
-KB_HD EF ecube(const EF &a) { return emul(esqr(a), a); }
+// a^3 with 3a computed once for both products
+KB_HD EF ecube(const EF &a) {
+    uint32_t a3[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) a3[j] = ftimes3(a.c[j]).v;
+    return emul_b3(emul_b3(a, a, a3), a, a3);
+}
 
 // ---------------------------------------------------------------- Poseidon2 --
 // Constants block (Montgomery form): ext_init[4][W], ext_term[4][W], internal[RP].
