@@ -35,9 +35,15 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE passes (profiles/prove_n17_hbm_traffic.md), fetch scaled x2.0
 # and write x1.0 by the same counters on a 256 MiB device copy.  (read MiB, write MiB)
 MEASURED_TRAFFIC_N17 = {
-    1: (69.73, 5.75),     # k_merkle_leaves (3 launches) + k_merkle_leaves_pair (1), averaged
-    3: (92.93, 139.00),   # k_ntt2_a + k_ntt2_b (one rs_encode, averaged over the 4 commits)
+    1: (70.03, 5.76),     # k_merkle_leaves (3 launches) + k_merkle_leaves_pair (1), averaged
+    3: (92.86, 139.00),   # k_ntt2_a + k_ntt2_b (one rs_encode, averaged over the 4 commits)
 }
+# VALU instructions per Poseidon2-24 leaf permutation (SQ_INSTS_VALU per wave of the
+# 2^19-row commit / 8 permutations per thread, profiles/prove_n17_valu_issue.md).  The
+# leaf class's live permutation rate times this count, over the VALU issue rate of
+# 1024 SIMDs x 2.4 GHz at 4 cycles per wave64 instruction, is `valu_issue`: the share of
+# the issue rate the leaf hashing uses (the F-mul `frac` counts only its products).
+VALU_INSTR_PER_P24 = 6969
 PHASES = ["commit", "zerocheck_round0", "zerocheck_rounds", "batching", "inner_sumcheck", "whir_open",
           "host_wait_for_device"]
 MODELS = {
@@ -312,6 +318,9 @@ def main():
         # from the untimed all-class pass (one proof)
         "kernel_ms_per_step": {MODELS[k][0].split(" ")[0]: round(v[1], 4) for k, v in per_class.items()},
     }
+    if dom == 1:
+        perms_per_s = work / max(launches, 1) / (avg_ms / 1e3)
+        roofline["valu_issue"] = round(perms_per_s * VALU_INSTR_PER_P24 / 64 * 4 / (1024 * 2.4e9), 4)
 
     cpu = None
     identical = None  # GPU proof == oracle proof of the same workload (None: not compared)
