@@ -490,10 +490,10 @@ __global__ void k_fold_cols_p(EF *U, size_t stride, int l, int n_cols, const EF 
         if (x < q) E[x] = eadd(E[x], E[x + q]);
         return;
     }
-    const EF ch = *chp;
+    const EFm ch = emul_prep(*chp);
     EF *col = U + (size_t)blockIdx.y * stride;
     EF a = col[x];
-    col[x] = eadd(a, emul(ch, esub(col[x + h], a)));
+    col[x] = eadd(a, emulm(esub(col[x + h], a), ch));
 }
 void fold_cols_p(EF *U, int n_cols, size_t stride, int l, const EF *c, EF *E, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
@@ -646,14 +646,14 @@ __global__ void __launch_bounds__(256) k_prod_round_f(EF *A, EF *B, size_t h, in
                                                      EF *parts) {
     EF a0 = ezero(), a2 = ezero();
     const size_t st = (size_t)gridDim.x * blockDim.x;
-    const EF c = fold ? *cp : ezero();
+    const EFm c = emul_prep(fold ? *cp : ezero());
     for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < h; x += st) {
         EF al = A[x], ah = A[x + h], bl = B[x], bh = B[x + h];
         if (fold) {
-            al = eadd(al, emul(c, esub(A[x + 2 * h], al)));
-            ah = eadd(ah, emul(c, esub(A[x + 3 * h], ah)));
-            bl = eadd(bl, emul(c, esub(B[x + 2 * h], bl)));
-            bh = eadd(bh, emul(c, esub(B[x + 3 * h], bh)));
+            al = eadd(al, emulm(esub(A[x + 2 * h], al), c));
+            ah = eadd(ah, emulm(esub(A[x + 3 * h], ah), c));
+            bl = eadd(bl, emulm(esub(B[x + 2 * h], bl), c));
+            bh = eadd(bh, emulm(esub(B[x + 3 * h], bh), c));
             A[x] = al;
             A[x + h] = ah;
             B[x] = bl;
@@ -679,10 +679,10 @@ int prod_round_fused(EF *A, EF *B, int l, bool fold, const EF *c, EF *parts, hip
 __global__ void k_fold_ef2_p(EF *A, EF *B, size_t h, const EF *cp) {
     size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= h) return;
-    const EF c = *cp;
+    const EFm c = emul_prep(*cp);
     EF *T = blockIdx.y ? B : A;
     EF a = T[x];
-    T[x] = eadd(a, emul(c, esub(T[x + h], a)));
+    T[x] = eadd(a, emulm(esub(T[x + h], a), c));
 }
 void fold_ef2_p(EF *A, EF *B, int l, const EF *c, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
@@ -692,7 +692,7 @@ __global__ void k_fold_ef(EF *A, size_t h, const EF *c) {
     size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= h) return;
     EF a = A[x];
-    A[x] = eadd(a, emul(*c, esub(A[x + h], a)));
+    A[x] = eadd(a, emulm(esub(A[x + h], a), emul_prep(*c)));
 }
 void fold_ef(EF *A, int l, const EF *c, hipStream_t s) {
     size_t h = (size_t)1 << (l - 1);
@@ -706,8 +706,9 @@ constexpr int FOLD_TAIL_LOG = 12;
 __global__ void __launch_bounds__(1024) k_fold_ef_tail(EF *A, int l, int nf, const EF *c) {
     __shared__ EF buf[1 << (FOLD_TAIL_LOG - 1)];
     int h = 1 << (l - 1);
+    const EFm c0 = emul_prep(c[0]);
     for (int x = threadIdx.x; x < h; x += blockDim.x) {
-        const EF a = A[x], r = eadd(a, emul(c[0], esub(A[x + h], a)));
+        const EF a = A[x], r = eadd(a, emulm(esub(A[x + h], a), c0));
         buf[x] = r;
         A[x] = r;
     }
@@ -718,7 +719,7 @@ __global__ void __launch_bounds__(1024) k_fold_ef_tail(EF *A, int l, int nf, con
         EF r;
         if (x < h) {
             const EF a = buf[x];
-            r = eadd(a, emul(c[i], esub(buf[x + h], a)));
+            r = eadd(a, emulm(esub(buf[x + h], a), emul_prep(c[i])));
         }
         __syncthreads();
         if (x < h) {
@@ -832,7 +833,8 @@ __global__ void __launch_bounds__(256) k_a2b_round_fused(const F *a0, const F *b
             auto fl = [&](const F *t, size_t i) { return eadd(efrom(t[i]), emulf(c, fsub(t[i + 2 * h], t[i]))); };
             al = fl(a0, x); ah = fl(a0, x + h); bl = fl(b0, x); bh = fl(b0, x + h);
         } else {
-            auto fl = [&](const EF *t, size_t i) { return eadd(t[i], emul(c, esub(t[i + 2 * h], t[i]))); };
+            const EFm cm = emul_prep(c);
+            auto fl = [&](const EF *t, size_t i) { return eadd(t[i], emulm(esub(t[i + 2 * h], t[i]), cm)); };
             al = fl(A, x); ah = fl(A, x + h); bl = fl(B, x); bh = fl(B, x + h);
         }
         A[x] = al; A[x + h] = ah; B[x] = bl; B[x + h] = bh;
@@ -882,7 +884,8 @@ __global__ void __launch_bounds__(256) k_a2b_round_fused_split(const F *a0, cons
             auto fl = [&](const F *t, size_t k) { return eadd(efrom(t[k]), emulf(c, fsub(t[k + 2 * h], t[k]))); };
             al = fl(a0, x); ah = fl(a0, x + h); bl = fl(b0, x); bh = fl(b0, x + h);
         } else {
-            auto fl = [&](const EF *t, size_t k) { return eadd(t[k], emul(c, esub(t[k + 2 * h], t[k]))); };
+            const EFm cm = emul_prep(c);
+            auto fl = [&](const EF *t, size_t k) { return eadd(t[k], emulm(esub(t[k + 2 * h], t[k]), cm)); };
             al = fl(A, x); ah = fl(A, x + h); bl = fl(B, x); bh = fl(B, x + h);
         }
         A[x] = al; A[x + h] = ah; B[x] = bl; B[x + h] = bh;

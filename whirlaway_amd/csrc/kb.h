@@ -110,6 +110,16 @@ KB_HD EF emul_b3(const EF &a, const EF &b, const uint32_t *b3) {
     }
     return r;
 }
+// an extension element that multiplies many others (a fold's challenge): 3c once
+struct EFm { EF c; uint32_t c3[8]; };
+KB_HD EFm emul_prep(const EF &c) {
+    EFm m;
+    m.c = c;
+#pragma unroll
+    for (int j = 0; j < 8; j++) m.c3[j] = ftimes3(c.c[j]).v;
+    return m;
+}
+KB_HD EF emulm(const EF &a, const EFm &m) { return emul_b3(a, m.c, m.c3); }
 KB_HD EF emul(const EF &a, const EF &b) {
     uint32_t b3[8];
 #pragma unroll
