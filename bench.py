@@ -61,6 +61,45 @@ def _progress(msg):
     print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
 
+def run_extra(args, ctx, table, rc, p16, p24, wit, proof):
+    """The same 2^n workload through the generic AIR path (the Poseidon2 AIR as a
+    constraint program, proved by the generic sumcheck) with the proof compared to
+    the specialised one, and the product verifier (AirTable::verify) on that proof:
+    one verification, and a batch on host threads."""
+    import numpy as np
+    import whirlaway_amd as wa
+    from whirlaway_amd.air import download_witness
+    from whirlaway_amd.program import poseidon2_air_program
+    from whirlaway_amd.proof import verify_batch
+    n = args.log_n
+    settings = wa.AirSettings()
+    host_w = download_witness(wit)
+    gtable = wa.air.ProgramAirTable(ctx, poseidon2_air_program(rc), n, 3)
+    gp = gtable.prove(settings, p16, p24, host_w)  # warm-up
+    steps = 5
+    t = time.perf_counter()
+    for _ in range(steps):
+        gp = gtable.prove(settings, p16, p24, host_w)
+    tg = (time.perf_counter() - t) / steps
+    gtable.close()
+    del host_w
+    r = verify_batch(table, settings, p16, p24, [proof])
+    t = time.perf_counter()
+    for _ in range(steps):
+        r = verify_batch(table, settings, p16, p24, [proof], n_threads=1)
+    tv = (time.perf_counter() - t) / steps
+    batch = [proof] * 64
+    t = time.perf_counter()
+    rb = verify_batch(table, settings, p16, p24, batch, n_threads=16)
+    tb = time.perf_counter() - t
+    return {"generic_air": {"ms_per_proof": round(tg * 1e3, 3), "value": round((1 << n) / tg, 1),
+                            "unit": "perms/s", "witness": "host memory",
+                            "identical_to_specialised": bool(gp.size == proof.size and (gp == proof).all())},
+            "verifier": {"ms_per_proof": round(tv * 1e3, 3), "accepted": r[0] == 0,
+                         "batch": {"proofs": len(batch), "threads": 16, "proofs_per_s": round(len(batch) / tb, 1),
+                                   "all_accepted": all(x == 0 for x in rb)}}}
+
+
 def run_micro(args):
     """Kernel micro configs at full size: (a) 2^24 base-field values folded 24 times
     by extension challenges (benches/folding_benchmarks.rs shape), (b) the eq-weighted
@@ -228,6 +267,8 @@ def main():
                          "saturates the plain path and gives the headline serving figures")
     ap.add_argument("--slots", type=int, default=8, help="engine slots (concurrent proofs)")
     ap.add_argument("--no-micro", action="store_true", help="skip the kernel micro leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the generic-AIR and product-verifier leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -343,6 +384,8 @@ def main():
 
     micro = None if args.no_micro else run_micro(args)
     _progress("micro done")
+    extra = None if args.no_extra else run_extra(args, ctx, table, rc, p16, p24, wit, proof)
+    _progress("generic AIR and verifier done")
     batched = run_batched(args) if args.batch_jobs > 0 else None
     _progress("batched done")
     serving = run_serving(args) if args.serve_requests > 0 else None
@@ -371,6 +414,7 @@ def main():
         "phase_ms_per_step": dict(zip(PHASES, [round(x / args.steps, 4) for x in phases])),
         "pcie_inclusive": pcie,
         "micro": micro,
+        "generic_air_and_verifier": extra,
         "batched": batched,
         "serving": serving,
     }
@@ -380,7 +424,10 @@ def main():
     print(json.dumps(line))
     mismatch = [k for k, v in (("oracle", identical), ("pcie", pcie and pcie["same_proof"]),
                                ("batched", batched and batched["identical_to_plain"]),
-                               ("serving", serving and serving["identical_to_plain"])) if v is False]
+                               ("serving", serving and serving["identical_to_plain"]),
+                               ("generic_air", extra and extra["generic_air"]["identical_to_specialised"]),
+                               ("verifier", extra and extra["verifier"]["accepted"]
+                                and extra["verifier"]["batch"]["all_accepted"])) if v is False]
     if mismatch:
         print(f"bench.py: proofs differ ({', '.join(mismatch)})", file=sys.stderr)
         sys.exit(1)
