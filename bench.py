@@ -299,6 +299,12 @@ def main():
     proof = None
     for _ in range(args.warmup):
         proof = table.prove(settings, p16, p24, wit)
+    # the autotuner times the commitment kernels' launch variants on the first calls
+    # of each shape (one candidate per call); untimed proofs until it has decided
+    tune_proofs = 0
+    while wa.autotune.exploring() and tune_proofs < 32:
+        proof = table.prove(settings, p16, p24, wit)
+        tune_proofs += 1
 
     # Untimed pass with HIP events around every kernel class: per-class device time
     # and the dominant class.  Events are stream packets on the proof's critical path,
@@ -321,6 +327,12 @@ def main():
     ctx.kernel_timing([])
     phases = ctx.phase_times(reset=True)
     _progress(f"{args.steps} timed proofs: {ms:.3f} ms each")
+    # launch variants the autotuner chose for this proof's commitment shapes (during warm-up)
+    tuned = {"proofs_after_warmup": tune_proofs,
+             "choices": [{"kernel": e["kind"], "log_rows": e["log_size"], "width": e["width"],
+                          "variant": e["variant"],
+                          "best_us": None if e["best_us"] is None else round(e["best_us"], 2)}
+                         for e in wa.autotune.entries()]}
 
     pcie = None
     if args.pcie_steps > 0:
@@ -412,6 +424,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "phase_ms_per_step": dict(zip(PHASES, [round(x / args.steps, 4) for x in phases])),
+        "autotune": tuned,
         "pcie_inclusive": pcie,
         "micro": micro,
         "generic_air_and_verifier": extra,

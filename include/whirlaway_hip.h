@@ -286,6 +286,29 @@ int wl_engine_wait(wl_engine *e, uint64_t ticket, uint32_t *proof_out, size_t pr
 /* [completed jobs, peak concurrent jobs, peak reserved bytes, budget bytes] */
 void wl_engine_stats(wl_engine *e, uint64_t *out4);
 
+/* --------------------------------------------------------------- autotuner --
+ * No counterpart in the reference (its CPU prover has no launch shapes).  The
+ * commitment kernels have launch variants that compute identical values: kind 0
+ * the split 2^L = 2^LA x 2^LB of the two-pass Reed-Solomon NTT (variants 0..2:
+ * LB = 10, 9, 8), kind 1 Merkle leaf hashing (0: a lane per row, 256-thread
+ * blocks; 1: a lane pair per row; 2, 3: the same in 64-thread blocks), kind 2
+ * the schedule of the tree levels (0..5).  With tuning on (the default;
+ * WHIRLAWAY_AUTOTUNE=0 or mode 0 turns it off) the first calls of a (kind, log
+ * size, width) on a device each run the next variant, timed in place by events
+ * around the call, until every variant has two samples; the fastest then serves
+ * every later call of every context (a process-wide table). */
+/* mode 0: defaults only, 1: tune on first use, other values only query; returns
+ * the previous mode */
+int wl_autotune_mode(int mode);
+/* sets the variant of (device, kind, log size, width); width counts only for
+ * kind 1 (leaf width in field elements) */
+int wl_autotune_force(int device, int kind, int log_size, int width, int variant);
+/* table entries, 6 per entry: device, kind, log size, width, variant (-1 while
+ * its variants are still being timed), best time in ns (-1: forced); returns the
+ * number of entries (written up to cap) */
+size_t wl_autotune_entries(int64_t *out, size_t cap);
+void wl_autotune_clear(void);
+
 /* ---------------------------------------------------------------- kernels --
  * crates/utils/src/multilinear.rs:90-123 fold_multilinear_in_large_field:
  * out[i] = sum_j scalars[j] * in[i + j * 2^(log_n - log_k)], in F (base) or EF,

@@ -115,6 +115,10 @@ def lib():
         L.wl_proof_deserialize.argtypes = [u8p, ctypes.c_size_t, vp, u32p, u32p, u32p, ctypes.c_size_t, szp]
         L.wl_kernel_timing.argtypes = [vp, ctypes.c_uint32]
         L.wl_kernel_timing_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), dp, dp]
+        L.wl_autotune_mode.argtypes = [ctypes.c_int]
+        L.wl_autotune_force.argtypes = [ctypes.c_int] * 5
+        L.wl_autotune_entries.restype = ctypes.c_size_t
+        L.wl_autotune_entries.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -133,4 +137,4 @@ def check(rc):
 
 
 from .air import AirSettings, AirTable, Context  # noqa: E402
-from . import engine, program, proof, sumcheck, transcript, utils  # noqa: E402,F401
+from . import autotune, engine, program, proof, sumcheck, transcript, utils  # noqa: E402,F401

@@ -107,7 +107,7 @@ struct wl_ctx {
     uint32_t *pinned = nullptr;   // host staging
     size_t pinned_words = 0;
     std::map<int, kb::F *> tw;    // twiddle tables per log size
-    std::map<int, kb::F *> ntt_tab;  // compact two-pass NTT tables per log size
+    std::map<int, kb::F *> ntt_tab;  // compact two-pass NTT tables per (log size, split)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double phase_ms[8] = {0};
     // live timing of kernel classes (HIP events on this context's stream)
@@ -125,16 +125,18 @@ struct wl_ctx {
         tw[L] = t;
         return t;
     }
-    // compact twiddle tables of the two-pass RS encoding (nullptr when unused)
-    kb::F *ntt_tabs(int L) {
+    // compact twiddle tables of the two-pass RS encoding with split `variant`
+    // (wk::ntt_split; nullptr when unused)
+    kb::F *ntt_tabs(int L, int variant = 0) {
         int la, lb;
-        if (!wk::ntt_split(L, &la, &lb)) return nullptr;
-        auto it = ntt_tab.find(L);
+        if (!wk::ntt_split(L, &la, &lb, variant)) return nullptr;
+        const int key = L * 32 + lb;
+        auto it = ntt_tab.find(key);
         if (it != ntt_tab.end()) return it->second;
         kb::F *t;
-        HIP_CHECK(hipMalloc(&t, wk::ntt_tables_words(L) * sizeof(kb::F)));
-        wk::ntt_tables(twiddle(L), L, t, stream);
-        ntt_tab[L] = t;
+        HIP_CHECK(hipMalloc(&t, wk::ntt_tables_words(L, variant) * sizeof(kb::F)));
+        wk::ntt_tables(twiddle(L), L, t, stream, variant);
+        ntt_tab[key] = t;
         return t;
     }
     // Host -> device copy of a large pageable buffer (the witness) through pinned

@@ -357,19 +357,25 @@ __global__ void k_ntt_tables(const F *tw_full, int L, int LA, F *out) {
         out[i] = v;
     }
 }
-size_t ntt_tables_words(int L) {
+size_t ntt_tables_words(int L, int variant) {
     int LA, LB;
-    ntt_split(L, &LA, &LB);
+    ntt_split(L, &LA, &LB, variant);
     return ((size_t)1 << (LA - 1)) + ((size_t)1 << (LB - 1)) + 1024 + ((size_t)1 << (L - 10));
 }
-void ntt_tables(const F *tw_full, int L, F *out, hipStream_t s) {
+void ntt_tables(const F *tw_full, int L, F *out, hipStream_t s, int variant) {
     int LA, LB;
-    ntt_split(L, &LA, &LB);
+    ntt_split(L, &LA, &LB, variant);
     k_ntt_tables<<<8, 256, 0, s>>>(tw_full, L, LA, out);
 }
-bool ntt_split(int L, int *LA, int *LB) {
+// LB = min(L - 4, 10) by default (variant 0 or negative); variants 1 and 2 take
+// LB = 9 and 8 where that leaves 4 <= LA <= 10
+bool ntt_split(int L, int *LA, int *LB, int variant) {
     if (L < NTT2_MIN_L || L > NTT2_MAX_L) { *LA = 0; *LB = L; return false; }
     *LB = L - 4 < 10 ? L - 4 : 10;
+    if (variant > 0 && variant < NTT_VARIANTS) {
+        const int lb = 10 - variant, la = L - lb;
+        if (la >= 4 && la <= 10) *LB = lb;
+    }
     *LA = L - *LB;
     return true;
 }
@@ -391,12 +397,12 @@ static void launch_b(const F *Y, int n, int log_m, int LA, const F *tabs, F *cw,
 }
 
 int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int L, F *cw, F *tmp,
-              const F *tw_full, const F *tabs, hipStream_t s, bool zero_fill) {
+              const F *tw_full, const F *tabs, hipStream_t s, bool zero_fill, int variant) {
     size_t rows = (size_t)1 << L;
     if (zero_fill && n_nonzero < n_cols)
         fill_zero(cw + (size_t)n_nonzero * rows, (size_t)(n_cols - n_nonzero) * rows * 4, s);
     int LA, LB;
-    const bool two = ntt_split(L, &LA, &LB);
+    const bool two = ntt_split(L, &LA, &LB, variant);
     if (n_nonzero <= 0) return two ? LA : 0;
     if (L <= 12) {
         // whole column in LDS: 2^L + 2^(L-1) words (dynamic LDS stays below 48 KiB)
@@ -415,7 +421,8 @@ int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int L, F *cw,
             case 9: launch_a<9>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
             default: launch_a<10>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
         }
-        if (LB == 9) launch_b<9>(tmp, n_nonzero, log_m, LA, tabs, cw, s);
+        if (LB == 8) launch_b<8>(tmp, n_nonzero, log_m, LA, tabs, cw, s);
+        else if (LB == 9) launch_b<9>(tmp, n_nonzero, log_m, LA, tabs, cw, s);
         else launch_b<10>(tmp, n_nonzero, log_m, LA, tabs, cw, s);
         return LA;
     }
@@ -742,33 +749,50 @@ __global__ void __launch_bounds__(256) k_merkle_sub(F *nodes, size_t cnt, int le
     }
 }
 
-// commits of up to 2^MERKLE_PAIR_MAX_LOG rows hash with a lane pair per row
-#ifndef MERKLE_PAIR_MAX_LOG
-#define MERKLE_PAIR_MAX_LOG 15
-#endif
-void merkle_leaves(const F *cw, int width, int nnz, int log_rows, int la, const void *pc, F *nodes, hipStream_t s) {
-    size_t rows = (size_t)1 << log_rows;
-    if (log_rows >= 5 && log_rows <= MERKLE_PAIR_MAX_LOG) {
+// Leaf-hashing launch variants (identical digests): 0 a lane per row in blocks of
+// 256, 1 a lane pair per row in blocks of 256, 2 a lane per row in blocks of 64,
+// 3 a lane pair per row in blocks of 64.  The default takes lane pairs for commits
+// of 2^5 .. 2^15 rows, which at a lane per row leave SIMDs idle.
+int merkle_leaves_default(int log_rows) { return log_rows >= 5 && log_rows <= 15 ? 1 : 0; }
+void merkle_leaves(const F *cw, int width, int nnz, int log_rows, int la, const void *pc, F *nodes, hipStream_t s,
+                   int variant) {
+    const size_t rows = (size_t)1 << log_rows;
+    if (variant < 0 || variant >= LEAF_VARIANTS) variant = merkle_leaves_default(log_rows);
+    if ((variant & 1) && log_rows < 5) variant = 0;  // lane pairs need rows a multiple of 32
+    const unsigned cap = variant >= 2 ? 64u : 256u;
+    if (variant & 1) {
         const size_t threads = 2 * rows;  // a multiple of 64
-        const unsigned bs = threads < 256 ? (unsigned)threads : 256u;
+        const unsigned bs = threads < cap ? (unsigned)threads : cap;
         k_merkle_leaves_pair<<<(unsigned)(threads / bs), bs, 0, s>>>(cw, width, nnz, rows, la, log_rows - la,
                                                                      (const P2DevT *)pc, nodes);
         return;
     }
-    k_merkle_leaves<<<nblk(rows), 256, 0, s>>>(cw, width, nnz, rows, la, log_rows - la, (const P2DevT *)pc, nodes);
+    k_merkle_leaves<<<nblk(rows, (int)cap), cap, 0, s>>>(cw, width, nnz, rows, la, log_rows - la,
+                                                         (const P2DevT *)pc, nodes);
 }
 void merkle_build(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s) {
     merkle_leaves(cw, width, width, log_rows, la, pc, nodes, s);
     merkle_nodes(log_rows, pc, nodes, s);
 }
-void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s) {
+// Tree schedules (identical nodes): levels of at least 2^W parents take one thread
+// per node, the lowest P of them (down to 2^W) a lane pair per node instead, and
+// the levels below 2^W go to 16-lane subtrees of up to five levels per block.
+// (W, P) by variant: 0 (15, 1) the default, 1 (15, 0), 2 (14, 1), 3 (14, 2),
+// 4 (16, 1), 5 (13, 2).
+void merkle_tree_schedule(int variant, int *W, int *Pn) {
+    static const int tab[TREE_VARIANTS][2] = {{15, 1}, {15, 0}, {14, 1}, {14, 2}, {16, 1}, {13, 2}};
+    if (variant < 0 || variant >= TREE_VARIANTS) variant = 0;
+    *W = tab[variant][0];
+    *Pn = tab[variant][1];
+}
+void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s, int variant) {
     const P2DevT *p = (const P2DevT *)pc;
+    int W, Pn;
+    merkle_tree_schedule(variant, &W, &Pn);
     size_t rows = (size_t)1 << log_rows;
     size_t cnt = rows >> 1;
-    // wide levels: one thread per node; the 2^15-node level (512 waves at one thread
-    // per node, half the SIMDs) on lane pairs
-    while (cnt >= ((size_t)1 << 15)) {
-        if (cnt == ((size_t)1 << 15))
+    while (cnt >= ((size_t)1 << W)) {
+        if (cnt < ((size_t)1 << (W + Pn)))
             k_merkle_level_pair<<<(unsigned)(2 * cnt / 256), 256, 0, s>>>(nodes, cnt, p);
         else
             k_merkle_level<<<nblk(cnt), 256, 0, s>>>(nodes, cnt, p);

@@ -162,14 +162,18 @@ void ef_to_cols(const EF *in, int m, int k, F *out, hipStream_t s);
 // 2^log_rows words; tabs: ntt_tables(log_rows) (used when ntt_split holds).
 // Columns from n_nonzero on are zero: written as zeros when zero_fill is set, else
 // left unwritten (the Merkle leaf and opening kernels then take them as zero).
+// variant: the split of the two-pass encoding (ntt_split); tabs must be the tables
+// of the same split.
 int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int log_rows, F *cw,
-              F *tmp, const F *tw_full, const F *tabs, hipStream_t s, bool zero_fill = true);
+              F *tmp, const F *tw_full, const F *tabs, hipStream_t s, bool zero_fill = true, int variant = 0);
 // lengths up to 2^12 are encoded in one block per column, 2^14 .. 2^20 by the
-// two-pass register kernels with L = LA + LB, LB = 10, others by LDS four-step kernels
+// two-pass register kernels with L = LA + LB (LB = 10 by default; variants 1, 2:
+// LB = 9, 8), others by LDS four-step kernels
 constexpr int NTT2_MIN_L = 14, NTT2_MAX_L = 20;
-bool ntt_split(int L, int *LA, int *LB);
-size_t ntt_tables_words(int L);
-void ntt_tables(const F *tw_full, int L, F *out, hipStream_t s);
+constexpr int NTT_VARIANTS = 3, LEAF_VARIANTS = 4, TREE_VARIANTS = 6;
+bool ntt_split(int L, int *LA, int *LB, int variant = 0);
+size_t ntt_tables_words(int L, int variant = 0);
+void ntt_tables(const F *tw_full, int L, F *out, hipStream_t s, int variant = 0);
 // twiddle table w_{2^L}^i for i < 2^L
 void twiddles(F *out, int L, hipStream_t s);
 // Merkle tree over rows of a column-major codeword (width F per leaf), heap layout nodes[2*rows][8]
@@ -177,8 +181,14 @@ void twiddles(F *out, int L, hipStream_t s);
 void merkle_build(const F *cw, int width, int log_rows, int la, const void *pc, F *nodes, hipStream_t s);
 // the two halves of merkle_build: leaf hashes (columns from nnz on taken as zero,
 // not read), then the compression levels
-void merkle_leaves(const F *cw, int width, int nnz, int log_rows, int la, const void *pc, F *nodes, hipStream_t s);
-void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s);
+// variant: LEAF_VARIANTS / TREE_VARIANTS launch shapes (negative: the default)
+void merkle_leaves(const F *cw, int width, int nnz, int log_rows, int la, const void *pc, F *nodes, hipStream_t s,
+                   int variant = -1);
+int merkle_leaves_default(int log_rows);
+void merkle_nodes(int log_rows, const void *pc, F *nodes, hipStream_t s, int variant = -1);
+// tree schedule `variant`: levels of at least 2^W parents one thread per node, the
+// lowest Pn of them on lane pairs, the rest in 16-lane subtrees
+void merkle_tree_schedule(int variant, int *W, int *Pn);
 // gather rows (width F each; columns from nnz on are zero) and sibling paths for
 // sorted query indices
 void merkle_open(const F *cw, const F *nodes, int width, int nnz, int log_rows, int la, const uint32_t *idx,

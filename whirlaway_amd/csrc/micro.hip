@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include "../../include/whirlaway_hip.h"
 #include "ctx.h"
+#include "tune.h"
 
 using namespace wl;
 
@@ -287,10 +288,10 @@ int wl_rs_commit(wl_ctx *c, const uint32_t *p16, const uint32_t *p24, const uint
         DBuf<uint32_t> dcols(c->pool, n_cols * m);
         upload_canon(c, dcols, cols, n_cols * m);
         DBuf<F> cw(c->pool, n_cols * rows), tmp(c->pool, n_cols * rows), nodes(c->pool, 2 * rows * 8);
-        int la = wk::rs_encode((const F *)dcols.p, (int)n_cols, (int)n_cols, (int)log_m, (int)log_rows, cw, tmp,
-                               c->twiddle((int)log_rows), c->ntt_tabs((int)log_rows), c->stream);
+        int la = wt::rs_encode(c, (const F *)dcols.p, (int)n_cols, (int)n_cols, (int)log_m, (int)log_rows, cw, tmp);
         HIP_CHECK(hipGetLastError());
-        wk::merkle_build(cw, (int)n_cols, (int)log_rows, la, pd.p, nodes, c->stream);
+        wt::merkle_leaves(c, cw, (int)n_cols, (int)n_cols, (int)log_rows, la, pd.p, nodes);
+        wt::merkle_nodes(c, (int)log_rows, pd.p, nodes);
         download_canon(c, root_out, (uint32_t *)nodes.p + 8, 8);
         if (codeword_out) {
             // storage order -> natural row order

@@ -19,6 +19,7 @@
 #include "ctx.h"
 #include "devts.h"
 #include "sc_generic.h"
+#include "tune.h"
 
 using namespace kb;
 using wh::Transcript;
@@ -59,21 +60,19 @@ void commit_codeword(wl_ctx *c, const P2Dev *pdev, Codeword &cwd, const F *cols,
     } else {
         cwd.cw = DBuf<F>(c->pool, (size_t)std::max(1, n_nonzero) * rows);  // zero columns are not stored
         DBuf<F> tmp(c->pool, (size_t)std::max(1, n_nonzero) * rows);
-        F *tw = c->twiddle(log_rows);
         // algorithmic HBM bytes: read the evaluations, write the codeword columns
         KTimer kt(c, KT_NTT, 4.0 * (double)n_nonzero * (double)(rows + ((size_t)1 << log_m)));
-        cwd.la = wk::rs_encode(cols, n_cols, n_nonzero, log_m, log_rows, cwd.cw, tmp, tw, c->ntt_tabs(log_rows),
-                               c->stream, false);
+        cwd.la = wt::rs_encode(c, cols, n_cols, n_nonzero, log_m, log_rows, cwd.cw, tmp);
         HIP_CHECK(hipGetLastError());
     }
     {
         // one Poseidon2-24 permutation per 16 leaf elements (PaddingFreeSponge rate 16)
         KTimer kt(c, KT_MERKLE_LEAVES, (double)rows * ((n_cols + 15) / 16));
-        wk::merkle_leaves(cwd.cw, n_cols, n_nonzero, log_rows, cwd.la, pdev, cwd.nodes, c->stream);
+        wt::merkle_leaves(c, cwd.cw, n_cols, n_nonzero, log_rows, cwd.la, pdev, cwd.nodes);
     }
     {
         KTimer kt(c, KT_MERKLE_NODES, (double)(rows - 1));
-        wk::merkle_nodes(log_rows, pdev, cwd.nodes, c->stream);
+        wt::merkle_nodes(c, log_rows, pdev, cwd.nodes);
     }
 }
 
@@ -1271,14 +1270,16 @@ static int prove_host_witness(wl_table *t, const wl_air_settings *st, const uint
         const int nnz = (int)std::min<size_t>(n_cols, (nw * N + colen - 1) / colen);
         DBuf<F> pcw(c->pool, (size_t)nnz * rows), tmp(c->pool, (size_t)nnz * rows);
         F *tw = c->twiddle(log_rows);
-        const F *tabs = c->ntt_tabs(log_rows);
+        // the split the tuner chose for this length (the encoding here is not timed)
+        const int nv = wt::ntt_variant(c, log_rows);
+        const F *tabs = c->ntt_tabs(log_rows, nv);
         wk::fill_zero(packed + nw * N, (V - nw * N) * 4, c->stream);
         int la = 0;
         c->h2d_staged(packed, witness, nw * N * 4, colen * 4, [&](size_t u0, size_t u1) {
             const size_t e0 = u0 * colen, e1 = std::min(u1 * colen, nw * N);
             wk::convert_to_mont((uint32_t *)packed.p + e0, e1 - e0, c->stream);
             la = wk::rs_encode(packed.p + e0, (int)(u1 - u0), (int)(u1 - u0), log_m, log_rows, pcw.p + u0 * rows,
-                               tmp.p + u0 * rows, tw, tabs, c->stream, false);
+                               tmp.p + u0 * rows, tw, tabs, c->stream, false, nv);
         });
         tmp.reset();
         c->pre0.cols = packed.p;
