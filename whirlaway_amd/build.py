@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwhirlaway_hip.so")
-SOURCES = ["k_sumcheck.hip", "k_zerocheck.hip", "k_whir.hip", "k_trace.hip", "prover.hip", "micro.hip", "engine.hip", "utils.hip", "transcript.hip", "sc_generic.hip", "verifier.hip", "k_devts.hip", "tune.hip"]
+SOURCES = ["k_sumcheck.hip", "k_zerocheck.hip", "k_whir.hip", "k_trace.hip", "prover.hip", "micro.hip", "engine.hip", "utils.hip", "transcript.hip", "sc_generic.hip", "verifier.hip", "k_devts.hip", "tune.hip", "trace.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

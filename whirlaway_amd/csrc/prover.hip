@@ -19,6 +19,7 @@
 #include "ctx.h"
 #include "devts.h"
 #include "sc_generic.h"
+#include "trace.h"
 #include "tune.h"
 
 using namespace kb;
@@ -567,9 +568,14 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
         tp = now;
     };
 
+    // tracing spans of crates/air/src/prove.rs:48,114,133 (whir-p3's commit and
+    // prove under this build's names)
+    Span span_prove("air: prove");
+    PhaseSpan phase("whir: commit");
     WhirWitness w;
     whir_commit(c, cfg, pdev, ts, packed, (size_t)AIR_NCOLS * N, w, absorb_first);
     lap(0);
+    phase.next("zerocheck");
     grind(c, ts, pdev, pow_bits_air(sec, (int)wh::log2_up(AIR_NCONS)));
     EF alpha = ts.sample_ef();
     std::vector<EF> ap(AIR_NCONS);
@@ -709,6 +715,7 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
     wl_ctx::HostSlot slot_cur{};
     if (nz > 1 && !device_rounds) slot_cur = enqueue_round(l, nullptr, parts_cur, seq_cur);
     for (int j = 1; j < nz && !device_rounds; j++) {
+        Span span_round("sumcheck_round");
         EF r = zc[j];
         uint32_t seq_next = 0;
         wl_ctx::HostSlot slot_next{};
@@ -733,6 +740,7 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
     }
     eq.reset();
     lap(2);
+    phase.next("inner sumchecks");
     // ---- inner sums ----
     // up (column heads of the folded table) is published first and absorbed while
     // the device computes the down sums, published behind it.
@@ -761,12 +769,18 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
     c->wait_flag(s_dn);
     memcpy(dn.data(), c->mapped + DN_OFF, AIR_NCOLS * sizeof(EF));
     ts.add_ef(dn.data(), dn.size());
-    grind(c, ts, pdev, pow_bits_air(sec, lc));
+    {
+        Span span_pow("pow grinding");
+        grind(c, ts, pdev, pow_bits_air(sec, lc));
+    }
     std::vector<EF> cbs(lc);
     for (int i = 0; i < lc; i++) cbs[i] = ts.sample_ef();
     DBuf<EF> eqc = eq_dev(c, cbs, eone());
     DBuf<EF> B(c->pool, N);
-    wk::lincomb_cols(packed, n, AIR_NCOLS, eqc, B, c->stream);
+    {
+        Span span_lc("multilinears_linear_combination");
+        wk::lincomb_cols(packed, n, AIR_NCOLS, eqc, B, c->stream);
+    }
     EF alpha2 = ts.sample_ef();
     DBuf<EF> da2 = upload_ef(c, std::vector<EF>(1, alpha2));
     DBuf<EF> mixed(c->pool, N);
@@ -784,12 +798,16 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
     for (size_t i = 1; i < ch.size(); i++) pt.push_back(ch[i]);
     DBuf<EF> E = eq_dev(c, pt, eone());
     DBuf<EF> Mt(c->pool, N);
-    wk::build_matrix(E, n, da2, Mt, c->stream);
+    {
+        Span span_m("matrix_up_folded + matrix_down_folded");
+        wk::build_matrix(E, n, da2, Mt, c->stream);
+    }
     // the inner sum is first needed with the first inner round's values: publish
     // it aside and read it then
     const size_t ISUM_OFF = wl_ctx::ISUM_OFF;
     uint32_t s_isum;
     {
+        Span span_is("inner sum evaluation");
         int nb = wk::dot_blocks(N);
         DBuf<EF> parts(c->pool, nb);
         wk::seg_dot(mixed, E, N, 1, parts, c->stream);
@@ -834,6 +852,7 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
     wl_ctx::HostSlot islot{};
     if (gpu_rounds > 0) islot = enqueue_inner(n, nullptr, iparts_cur, iseq);
     for (int j = 0; j < gpu_rounds; j++) {
+        Span span_round("sumcheck_round");
         int ll = n - j;
         uint32_t iseq_next = 0;
         wl_ctx::HostSlot islot_next{};
@@ -851,6 +870,7 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
     std::vector<EF> hM(tail), hB(tail);
     c->fetch2(hM.data(), Mt.p, tail * sizeof(EF), hB.data(), B.p, tail * sizeof(EF));
     for (size_t len = tail; len > 1; len >>= 1) {
+        Span span_round("sumcheck_round");
         size_t h = len / 2;
         EF h0 = ezero(), h2 = ezero();
         for (size_t i = 0; i < h; i++) {
@@ -870,6 +890,7 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
     lap(4);
     std::vector<EF> fp(cbs);
     fp.insert(fp.end(), ich.begin(), ich.end());
+    phase.next("whir: prove");
     whir_prove(c, cfg, pdev, ts, w, fp, packed_value);
     lap(5);
 }
@@ -890,8 +911,11 @@ void air_prove_generic(wl_table *t, const wl_air_settings *st, const P2Dev *pdev
     const int lc = (int)wh::log2_up((size_t)nw), ncons = sp.prog.n_asserts;
     WhirCfg cfg = air_whir_config(t->log_length, (uint32_t)nw, st);
     c->ring_reset();
+    Span span_prove("air: prove");
+    PhaseSpan phase("whir: commit");
     WhirWitness w;
     whir_commit(c, cfg, pdev, ts, packed, (size_t)nw * N, w, absorb_first);
+    phase.next("zerocheck");
     grind(c, ts, pdev, pow_bits_air(sec, (int)wh::log2_up((size_t)std::max(1, ncons))));
     const EF alpha = ts.sample_ef();
     std::vector<EF> ap(std::max(1, ncons));
@@ -920,9 +944,13 @@ void air_prove_generic(wl_table *t, const wl_air_settings *st, const P2Dev *pdev
         c->fetch(fe.data(), folded.p, fe.size() * sizeof(EF));
     }
     // ---- inner sums of the witness columns, up then down ----
+    phase.next("inner sumchecks");
     ts.add_ef(fe.data() + np, (size_t)nw);
     ts.add_ef(fe.data() + C + np, (size_t)nw);
-    grind(c, ts, pdev, pow_bits_air(sec, lc));
+    {
+        Span span_pow("pow grinding");
+        grind(c, ts, pdev, pow_bits_air(sec, lc));
+    }
     std::vector<EF> cbs(lc);
     for (auto &x : cbs) x = ts.sample_ef();
     // inner-sumcheck tables [matrix | batched column], contiguous
@@ -930,6 +958,7 @@ void air_prove_generic(wl_table *t, const wl_air_settings *st, const P2Dev *pdev
     EF *Mt = tabs.p, *B = tabs.p + N;
     {
         DBuf<EF> eqc = lc ? eq_dev(c, cbs, eone()) : upload_ef(c, std::vector<EF>(1, eone()));
+        Span span_lc("multilinears_linear_combination");
         wk::lincomb_cols(packed, n, nw, eqc, B, c->stream);
     }
     const EF alpha2 = ts.sample_ef();
@@ -954,7 +983,11 @@ void air_prove_generic(wl_table *t, const wl_air_settings *st, const P2Dev *pdev
     EF inner_sum;
     {
         DBuf<EF> E = eq_dev(c, pt, eone());
-        wk::build_matrix(E, n, da2, Mt, c->stream);
+        {
+            Span span_m("matrix_up_folded + matrix_down_folded");
+            wk::build_matrix(E, n, da2, Mt, c->stream);
+        }
+        Span span_is("inner sum evaluation");
         const int nb = wk::dot_blocks(N);
         DBuf<EF> parts(c->pool, nb);
         wk::seg_dot(mixed, E, N, 1, parts, c->stream);
@@ -984,6 +1017,7 @@ void air_prove_generic(wl_table *t, const wl_air_settings *st, const P2Dev *pdev
     tabs.reset();
     std::vector<EF> fp(cbs);
     fp.insert(fp.end(), ich.begin(), ich.end());
+    phase.next("whir: prove");
     whir_prove(c, cfg, pdev, ts, w, fp, fin[1]);
 }
 

@@ -15,6 +15,7 @@
 #include <type_traits>
 #include "../../include/whirlaway_hip.h"
 #include "sc_generic.h"
+#include "trace.h"
 
 using namespace wl;
 
@@ -240,6 +241,7 @@ std::vector<EF> prove(wl_ctx *c, wh::Transcript &ts, const P2Dev *pdev, const vo
     DBuf<EF> owned;
     int nv = n_vars;
     for (int r = 0; r < nr; r++) {
+        wl::Span span_round("sumcheck_round");  // crates/sumcheck/src/prove.rs:95
         const int k = r == 0 ? skips : 1, K = 1 << k;
         const size_t len = (size_t)1 << nv;
         const bool zerocheck = r == 0 && a.is_zerofier;

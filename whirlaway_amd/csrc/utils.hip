@@ -13,6 +13,7 @@
 #include <mutex>
 #include "../../include/whirlaway_hip.h"
 #include "ctx.h"
+#include "trace.h"
 
 using namespace wl;
 
@@ -226,6 +227,7 @@ int wl_multilinears_linear_combination(wl_ctx *c, const uint32_t *cols, uint32_t
         for (uint32_t i = 0; i < n_cols; i++) sc[i] = wh::to_mont_ef(scalars + 8 * (size_t)i);
         DBuf<EF> dsc = upload_ef(c, sc);
         DBuf<EF> dout(c->pool, N);
+        wl::Span span("multilinears_linear_combination");  // crates/utils/src/multilinear.rs:125
         wk::lincomb_cols((const F *)din.p, (int)log_n, (int)n_cols, dsc, dout, c->stream);
         HIP_CHECK(hipGetLastError());
         wk::convert_from_mont((uint32_t *)dout.p, N * 8, c->stream);

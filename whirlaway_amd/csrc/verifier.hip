@@ -13,6 +13,7 @@
 #include <thread>
 #include "../../include/whirlaway_hip.h"
 #include "sc_generic.h"
+#include "trace.h"
 
 using namespace wl;
 
@@ -395,6 +396,7 @@ EF preprocessed_eval(const uint32_t *col, int n, int skips, bool down, const std
 
 // ------------------------------------------------------------- AIR verify --
 void air_verify(const wl_table *t, const wl_air_settings *st, const P2Consts16 &p16, const P2Consts24 &p24, VS &v) {
+    wl::Span span("air table: verify");  // crates/air/src/verify.rs:54
     const int n = (int)t->log_length, skips = (int)st->univariate_skips, sec = (int)st->security_bits;
     const wg::AirSpec *g = t->gen;
     const int C = g ? g->n_columns : wk::AIR_NCOLS, np = g ? g->n_preprocessed : 0, nw = C - np;
