@@ -58,6 +58,11 @@ class Context:
             lib().wl_destroy(self.ptr)
             self.ptr = None
 
+    def set_device_waits(self, enabled):
+        """wl_set_device_waits: enqueue sumcheck rounds ahead of their challenges behind
+        device-side waits (the default), or only once the challenge is known."""
+        check(lib().wl_set_device_waits(self.ptr, int(bool(enabled))))
+
     def phase_times(self, reset=True):
         out = (ctypes.c_double * 7)()
         lib().wl_phase_times(self.ptr, out, int(reset))

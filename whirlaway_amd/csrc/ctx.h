@@ -11,6 +11,8 @@
 #include <stdexcept>
 #include <string>
 #include <atomic>
+#include <condition_variable>
+#include <mutex>
 #include <thread>
 #include <vector>
 #include "common.h"
@@ -67,6 +69,41 @@ struct Pool {
     }
 };
 
+// Frees of device or pinned memory wait for the whole device, and a proof on a
+// context with device-side waits keeps kernels spinning on the GPU until its host
+// thread releases them.  A free on one thread while another thread's proof has
+// such a wait pending can therefore stall both until the wait times out.  Proofs
+// with device waits enter this guard for their duration; wl_destroy frees only
+// once none is running, and new proofs wait while a free is pending.
+struct FreeGuard {
+    std::mutex mu;
+    std::condition_variable cv;
+    int proofs = 0, freeing = 0;
+    void enter_proof() {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return freeing == 0; });
+        proofs++;
+    }
+    void leave_proof() {
+        std::lock_guard<std::mutex> lk(mu);
+        if (--proofs == 0) cv.notify_all();
+    }
+    void enter_free() {
+        std::unique_lock<std::mutex> lk(mu);
+        freeing++;
+        cv.wait(lk, [&] { return proofs == 0; });
+    }
+    void leave_free() {
+        std::lock_guard<std::mutex> lk(mu);
+        freeing--;
+        cv.notify_all();
+    }
+};
+inline FreeGuard &free_guard() {
+    static FreeGuard g;
+    return g;
+}
+
 template <class T>
 struct DBuf {
     Pool *pool = nullptr;
@@ -95,6 +132,10 @@ struct wl_ctx {
     // default: one wave's serial Poseidon2-16 duplexes take longer than the host
     // round trip they replace)
     bool device_transcript = false;
+    // sumcheck rounds are enqueued once the host has their challenge, instead of
+    // ahead of it behind a device wait (a kernel spinning on mapped memory holds its
+    // hardware queue; streams that share the queue stall behind it)
+    bool host_rounds = false;
     hipStream_t stream = nullptr;
     // a first codeword encoded while the witness uploaded (commit_codeword takes it
     // instead of encoding cols again): the source it encodes, its shape, the buffer
@@ -253,9 +294,11 @@ struct wl_ctx {
             throw std::runtime_error(err);
         }
     }
+    std::vector<void *> retired_host;  // outgrown pinned buffers, freed by wl_destroy
     uint32_t *stage(size_t words) {
         if (words > pinned_words) {
-            if (pinned) (void)hipHostFree(pinned);
+            // no free here: it would wait for the whole device (see wl::FreeGuard)
+            if (pinned) retired_host.push_back(pinned);
             size_t w = std::max(words, (size_t)1 << 20);
             HIP_CHECK(hipHostMalloc((void **)&pinned, w * 4, hipHostMallocDefault));
             pinned_words = w;

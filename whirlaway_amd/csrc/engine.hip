@@ -177,6 +177,10 @@ wl_engine *wl_engine_create(int device, int n_slots, uint64_t mem_budget_bytes) 
     e->budget = mem_budget_bytes;
     for (int i = 0; i < n_slots; i++) {
         wl_ctx *c = wl_create(device);
+        // slots enqueue each sumcheck round once its challenge is known: no kernel
+        // spins on the GPU for a host value, so nothing else in the process can
+        // stall a slot's proof (same throughput as device waits, measured)
+        if (c) wl_set_device_waits(c, 0);
         if (!c) {
             for (wl_ctx *x : e->ctxs) wl_destroy(x);
             delete e;

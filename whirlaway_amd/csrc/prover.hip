@@ -220,7 +220,9 @@ struct SumcheckState {
 std::vector<EF> whir_sc_rounds(wl_ctx *c, Transcript &ts, const P2Dev *pdev, SumcheckState &s, int kn,
                                int pow_bits) {
     std::vector<EF> rs;
-    const bool pipe = pow_bits <= 0;  // grinding launches device work before the challenge exists
+    // grinding launches device work before the challenge exists; a context without
+    // device waits enqueues each round once its challenge is known
+    const bool pipe = pow_bits <= 0 && !c->host_rounds;
     auto enqueue = [&](int m, const EF *cdev, DBuf<EF> &parts, uint32_t &seq) {
         int nb = wk::dot_blocks((size_t)1 << (m - 1));
         parts = DBuf<EF>(c->pool, 2 * (size_t)nb);
@@ -672,6 +674,10 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
             KTimer kt(c, KT_ZC_ROUND, 3.0 * (double)((size_t)1 << (ll - 1)));
             wk::zc_round_wave(U, M, ll, t->air_dev, t->chain_dev, dap, eq, parts, c->stream);
         }
+        if (c->host_rounds) {
+            seq = reduce_async(c, parts, 3, (int)nbx);
+            return wl_ctx::HostSlot{};
+        }
         return reduce_async_await(c, parts, 3, (int)nbx, 8, seq);
     };
     // Rounds without grinding can run back to back with the transcript on the
@@ -719,8 +725,10 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
         EF r = zc[j];
         uint32_t seq_next = 0;
         wl_ctx::HostSlot slot_next{};
-        if (j + 1 < nz) slot_next = enqueue_round(l - 1, (const EF *)slot_cur.dev, parts_next, seq_next);
-        else wk::fold_cols_p(U, AIR_NCOLS, M, l, (const EF *)slot_cur.dev, eq, c->stream);
+        if (!c->host_rounds) {
+            if (j + 1 < nz) slot_next = enqueue_round(l - 1, (const EF *)slot_cur.dev, parts_next, seq_next);
+            else wk::fold_cols_p(U, AIR_NCOLS, M, l, (const EF *)slot_cur.dev, eq, c->stream);
+        }
         std::vector<EF> v3 = collect(c, seq_cur, 3);
         EF e0 = emul(v3[0], missing), e2 = emul(v3[1], missing), e3 = emul(v3[2], missing);
         EF e1 = emul(esub(sum, emul(e0, esub(eone(), r))), zc_inv[j]);
@@ -728,7 +736,13 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
         p = wh::pmul(p, {esub(eone(), r), esub(edbl(r), eone())});
         ts.add_ef(p.data(), p.size());
         EF cj = ts.sample_ef();
-        c->release(slot_cur, &cj, sizeof(EF));
+        if (c->host_rounds) {
+            DBuf<EF> dc = upload_ef(c, std::vector<EF>(1, cj));
+            if (j + 1 < nz) slot_next = enqueue_round(l - 1, dc.p, parts_next, seq_next);
+            else wk::fold_cols_p(U, AIR_NCOLS, M, l, dc.p, eq, c->stream);
+        } else {
+            c->release(slot_cur, &cj, sizeof(EF));
+        }
         ch.push_back(cj);
         sum = wh::peval(p, cj);
         grind(c, ts, pdev, pow_bits_sumcheck(sec, 3 + 1));
@@ -832,6 +846,10 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
         int nb = wk::dot_blocks((size_t)1 << (ll - 1));
         parts = DBuf<EF>(c->pool, 2 * (size_t)nb);
         wk::prod_round_fused(Mt, B, ll, cdev != nullptr, cdev, parts, c->stream);
+        if (c->host_rounds) {
+            seq = reduce_async(c, parts, 2, nb);
+            return wl_ctx::HostSlot{};
+        }
         return reduce_async_await(c, parts, 2, nb, 8, seq);
     };
     // the last rounds run on the host: from tables of 2^HOST_TAIL entries a round
@@ -856,11 +874,19 @@ void air_prove(wl_table *t, const wl_air_settings *st, const P2Dev *pdev, const 
         int ll = n - j;
         uint32_t iseq_next = 0;
         wl_ctx::HostSlot islot_next{};
-        if (j + 1 < gpu_rounds) islot_next = enqueue_inner(ll - 1, (const EF *)islot.dev, iparts_next, iseq_next);
-        else wk::fold_ef2_p(Mt, B, ll, (const EF *)islot.dev, c->stream);
+        if (!c->host_rounds) {
+            if (j + 1 < gpu_rounds) islot_next = enqueue_inner(ll - 1, (const EF *)islot.dev, iparts_next, iseq_next);
+            else wk::fold_ef2_p(Mt, B, ll, (const EF *)islot.dev, c->stream);
+        }
         std::vector<EF> h02 = collect(c, iseq, 2);
         EF cj = inner_round(h02[0], h02[1]).first;
-        c->release(islot, &cj, sizeof(EF));
+        if (c->host_rounds) {
+            DBuf<EF> dc = upload_ef(c, std::vector<EF>(1, cj));
+            if (j + 1 < gpu_rounds) islot_next = enqueue_inner(ll - 1, dc.p, iparts_next, iseq_next);
+            else wk::fold_ef2_p(Mt, B, ll, dc.p, c->stream);
+        } else {
+            c->release(islot, &cj, sizeof(EF));
+        }
         grind(c, ts, pdev, pow_bits_sumcheck(sec, 2));
         iseq = iseq_next;
         islot = islot_next;
@@ -1097,6 +1123,13 @@ void wl_destroy(wl_ctx *c) {
     (void)hipSetDevice(c->device);
     c->release_all();
     (void)hipStreamSynchronize(c->stream);
+    // the frees below wait for the whole device: not while another thread's proof
+    // has device-side waits pending
+    free_guard().enter_free();
+    struct Leave {
+        ~Leave() { free_guard().leave_free(); }
+    } leave;
+    for (void *h : c->retired_host) (void)hipHostFree(h);
     for (auto &kv : c->tw) (void)hipFree(kv.second);
     for (auto &kv : c->ntt_tab) (void)hipFree(kv.second);
     c->pool.release();
@@ -1120,6 +1153,12 @@ void wl_destroy(wl_ctx *c) {
     if (c->copy_order) (void)hipEventDestroy(c->copy_order);
     (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+int wl_set_device_waits(wl_ctx *c, int enabled) {
+    if (!c) { g_err = "null context"; return WL_ERR_ARG; }
+    c->host_rounds = enabled == 0;
+    return WL_OK;
 }
 
 int wl_device_count(void) {
@@ -1229,6 +1268,11 @@ static int prove_common(wl_table *t, const wl_air_settings *st, const uint32_t *
     }
     const P2Dev *pd = context_p2(c, ph);
     std::vector<uint32_t> proof;
+    struct Guard {  // a proof with device-side waits keeps frees elsewhere out (FreeGuard)
+        bool on;
+        explicit Guard(bool waits) : on(waits) { if (on) free_guard().enter_proof(); }
+        ~Guard() { if (on) free_guard().leave_proof(); }
+    } guard(!c->host_rounds);
     auto run = [&](Transcript &ts, const std::vector<uint32_t> *domsep) {
         if (t->gen) air_prove_generic(t, st, pd, packed, ts, domsep);
         else air_prove(t, st, pd, packed, ts, domsep);
