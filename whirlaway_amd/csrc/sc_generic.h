@@ -17,14 +17,25 @@
 namespace wg {
 using namespace kb;
 
-enum Op : uint32_t { OP_CONST = 0, OP_ADD = 1, OP_SUB = 2, OP_MUL = 3, OP_NEG = 4, OP_ASSERT = 5, OP_LOAD = 6 };
+// OP_LOAD and OP_MULC (a product with a constant operand) occur in slot code only
+enum Op : uint32_t { OP_CONST = 0, OP_ADD = 1, OP_SUB = 2, OP_MUL = 3, OP_NEG = 4, OP_ASSERT = 5, OP_LOAD = 6, OP_MULC = 7 };
 
+// The constraints split into `groups` groups, each evaluated by its own threads:
+// slot code of group g at code[off[g] .. off[g + 1]).
+struct Partition {
+    int groups = 0;
+    int n_slots = 0;
+    std::vector<uint32_t> off;
+    std::vector<uint32_t> code;
+};
 struct Program {
     int n_inputs = 0;
     int n_asserts = 0;
-    int n_slots = 0;                 // register-file size the slot code needs
-    std::vector<uint32_t> dev;       // slot code, 2 words per instruction
-    wl::DBuf<uint32_t> dev_buf;      // dev on the device (after upload)
+    int n_slots = 0;                 // register-file size the one-group slot code needs
+    std::vector<uint32_t> dev;       // one-group slot code, 2 words per instruction
+    std::vector<Partition> parts;    // 1, 4 and 16 groups (as many as the constraints allow)
+    wl::DBuf<uint32_t> dev_buf;      // every partition on the device: offsets, then code
+    std::vector<size_t> part_base;   // word offset of each partition in dev_buf
 };
 
 // Validates the SSA program and compiles it; throws std::runtime_error on a

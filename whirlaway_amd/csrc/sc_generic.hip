@@ -12,6 +12,7 @@
 // challenge's scalars in one launch.  All arithmetic is exact in F_p, so the
 // block/tree order of the sums does not change the result.
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <type_traits>
 #include "../../include/whirlaway_hip.h"
 #include "sc_generic.h"
@@ -29,6 +30,9 @@ __device__ __forceinline__ F d_sub(F a, F b) { return fsub(a, b); }
 __device__ __forceinline__ EF d_sub(const EF &a, const EF &b) { return esub(a, b); }
 __device__ __forceinline__ F d_mul(F a, F b) { return fmul(a, b); }
 __device__ __forceinline__ EF d_mul(const EF &a, const EF &b) { return emul(a, b); }
+// x times a constant held in a slot (an F, or an EF with only coefficient 0): EF x F
+__device__ __forceinline__ F d_mulc(F a, F c) { return fmul(a, c); }
+__device__ __forceinline__ EF d_mulc(const EF &a, const EF &c) { return emulf(a, c.c[0]); }
 __device__ __forceinline__ F d_neg(F a) { return fneg(a); }
 __device__ __forceinline__ EF d_neg(const EF &a) { return eneg(a); }
 template <class T> __device__ __forceinline__ T d_const(F c);
@@ -52,21 +56,24 @@ __device__ __forceinline__ EF load_folded(const EF *tab, size_t half, size_t x, 
 }
 
 // Block partials of sum_x eq[x] * sum_k batching[k] c_k(point_z(x)) for the round's
-// evaluation point blockIdx.y (scalars zsc[blockIdx.y][0..K)); tables: n inputs of
-// `len` entries each, contiguous.
+// evaluation point blockIdx.y (scalars zsc[blockIdx.y][0..K)) over the constraints of
+// group blockIdx.z of a partition (part: G + 1 word offsets, then the groups' slot
+// code); tables: n inputs of `len` entries each, contiguous.  An ASSERT names its
+// batching scalar in its destination field.  parts[(z G + g) gridDim.x + block].
 template <class T, int MAXR>
-__global__ void __launch_bounds__(128) k_eval(const T *tables, size_t len, int k, const F *zsc, const uint32_t *prog,
-                                              int n_prog, const EF *batching, const EF *eq, EF *parts) {
+__global__ void __launch_bounds__(128) k_eval(const T *tables, size_t len, int k, const F *zsc, const uint32_t *part,
+                                              int G, const EF *batching, const EF *eq, EF *parts) {
     __shared__ EF lds[16];
     const int K = 1 << k;
     const size_t half = len >> k;
     const F *s = zsc + (size_t)blockIdx.y * K;
+    const uint32_t *prog = part + (G + 1) + part[blockIdx.z];
+    const int n_prog = (int)(part[blockIdx.z + 1] - part[blockIdx.z]) / 2;
     T regs[MAXR];
     EF total = ezero();
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < half; x += stride) {
         EF acc = ezero();
-        int ka = 0;
         for (int pc = 0; pc < n_prog; pc++) {
             const uint32_t w0 = prog[2 * pc], w1 = prog[2 * pc + 1];
             const uint32_t op = w0 & 0xff, dst = w0 >> 8, a = w1 & 0xffff, b = w1 >> 16;
@@ -76,11 +83,11 @@ __global__ void __launch_bounds__(128) k_eval(const T *tables, size_t len, int k
             case OP_ADD: regs[dst] = d_add(regs[a], regs[b]); break;
             case OP_SUB: regs[dst] = d_sub(regs[a], regs[b]); break;
             case OP_MUL: regs[dst] = d_mul(regs[a], regs[b]); break;
+            case OP_MULC: regs[dst] = d_mulc(regs[a], regs[b]); break;
             case OP_NEG: regs[dst] = d_neg(regs[a]); break;
             default: {  // OP_ASSERT
-                if constexpr (std::is_same<T, F>::value) acc = eadd(acc, emulf(batching[ka], regs[a]));
-                else acc = eadd(acc, emul(batching[ka], regs[a]));
-                ka++;
+                if constexpr (std::is_same<T, F>::value) acc = eadd(acc, emulf(batching[dst], regs[a]));
+                else acc = eadd(acc, emul(batching[dst], regs[a]));
             }
             }
         }
@@ -88,7 +95,63 @@ __global__ void __launch_bounds__(128) k_eval(const T *tables, size_t len, int k
         total = eadd(total, acc);
     }
     const EF r = block_sum(total, lds);
-    if (threadIdx.x == 0) parts[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = r;
+    if (threadIdx.x == 0) parts[((size_t)blockIdx.y * G + blockIdx.z) * gridDim.x + blockIdx.x] = r;
+}
+
+// k_eval with the register file in LDS instead of private (scratch) memory, for
+// launches with few blocks: 64 threads per block, slot s of lane l at words
+// [(s W + c) 64 + l] (W words per value), so a wave's accesses hit 64 distinct
+// banks.  An interpreted instruction then waits on LDS, not on scratch.
+template <class T>
+__global__ void __launch_bounds__(64) k_eval_lds(const T *tables, size_t len, int k, const F *zsc, const uint32_t *part,
+                                                 int G, const EF *batching, const EF *eq, EF *parts) {
+    extern __shared__ uint32_t lregs[];
+    __shared__ EF lds[16];
+    constexpr int WD = (int)(sizeof(T) / 4);
+    const int lane = threadIdx.x;
+    auto rd = [&](uint32_t slot) {
+        T v;
+        uint32_t *pv = (uint32_t *)&v;
+#pragma unroll
+        for (int c = 0; c < WD; c++) pv[c] = lregs[(slot * WD + c) * 64 + lane];
+        return v;
+    };
+    auto wr = [&](uint32_t slot, const T &v) {
+        const uint32_t *pv = (const uint32_t *)&v;
+#pragma unroll
+        for (int c = 0; c < WD; c++) lregs[(slot * WD + c) * 64 + lane] = pv[c];
+    };
+    const int K = 1 << k;
+    const size_t half = len >> k;
+    const F *s = zsc + (size_t)blockIdx.y * K;
+    const uint32_t *prog = part + (G + 1) + part[blockIdx.z];
+    const int n_prog = (int)(part[blockIdx.z + 1] - part[blockIdx.z]) / 2;
+    EF total = ezero();
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < half; x += stride) {
+        EF acc = ezero();
+        for (int pc = 0; pc < n_prog; pc++) {
+            const uint32_t w0 = prog[2 * pc], w1 = prog[2 * pc + 1];
+            const uint32_t op = w0 & 0xff, dst = w0 >> 8, a = w1 & 0xffff, b = w1 >> 16;
+            switch (op) {
+            case OP_LOAD: wr(dst, load_folded(tables + (size_t)a * len, half, x, s, K)); break;
+            case OP_CONST: wr(dst, d_const<T>(F{w1})); break;
+            case OP_ADD: wr(dst, d_add(rd(a), rd(b))); break;
+            case OP_SUB: wr(dst, d_sub(rd(a), rd(b))); break;
+            case OP_MUL: wr(dst, d_mul(rd(a), rd(b))); break;
+            case OP_MULC: wr(dst, d_mulc(rd(a), rd(b))); break;
+            case OP_NEG: wr(dst, d_neg(rd(a))); break;
+            default: {  // OP_ASSERT
+                if constexpr (std::is_same<T, F>::value) acc = eadd(acc, emulf(batching[dst], rd(a)));
+                else acc = eadd(acc, emul(batching[dst], rd(a)));
+            }
+            }
+        }
+        if (eq) acc = emul(acc, eq[x]);
+        total = eadd(total, acc);
+    }
+    const EF r = block_sum(total, lds);
+    if (threadIdx.x == 0) parts[((size_t)blockIdx.y * G + blockIdx.z) * gridDim.x + blockIdx.x] = r;
 }
 
 // out[m][i] = sum_j sc[j] in[m][j M + i], M = len / K, for every table m
@@ -123,43 +186,38 @@ void columns_up_down(const F *pre, int n_pre, const F *wit, int n_cols, int log_
 }
 
 // --------------------------------------------------------------- compile --
-void compile(const uint32_t *code, size_t n_instr, int n_inputs, Program &out) {
-    if (n_inputs < 1 || n_inputs > 65535) throw std::runtime_error("a program reads 1 to 65535 inputs");
+namespace {
+
+int n_operands(uint32_t op) {
+    return (op == OP_ADD || op == OP_SUB || op == OP_MUL) ? 2 : (op == OP_NEG || op == OP_ASSERT) ? 1 : 0;
+}
+
+// Slot code of the instructions j with keep[j] set (their operands must be kept or
+// inputs): a linear scan gives each live value a slot, operands are freed before the
+// result is placed, inputs are loaded where first used.  An ASSERT carries its
+// batching index (its rank among all ASSERTs of the program) in the destination
+// field.  Returns the register-file size.
+int emit_slots(const uint32_t *code, size_t n_instr, int n_inputs, const std::vector<char> &keep,
+               const std::vector<int> &assert_rank, std::vector<uint32_t> &dev) {
     const size_t nv = (size_t)n_inputs + n_instr;
-    // last use of every value (inputs and instruction results)
     std::vector<long> last(nv, -1);
-    std::vector<char> is_assert(n_instr, 0);
     for (size_t j = 0; j < n_instr; j++) {
+        if (!keep[j]) continue;
         const uint32_t *w = code + 4 * j;
-        const uint64_t lim = (uint64_t)n_inputs + j;
-        auto use = [&](uint32_t v) {
-            if (v >= lim) throw std::runtime_error("program operand refers to a later value");
-            if (v >= (uint32_t)n_inputs && is_assert[v - n_inputs])
-                throw std::runtime_error("program operand refers to an ASSERT, which defines no value");
-            last[v] = (long)j;
-        };
-        switch (w[0]) {
-        case OP_CONST:
-            if (w[1] >= P) throw std::runtime_error("program constant is not a canonical field element");
-            break;
-        case OP_ADD: case OP_SUB: case OP_MUL: use(w[1]); use(w[2]); break;
-        case OP_NEG: use(w[1]); break;
-        case OP_ASSERT: use(w[1]); is_assert[j] = 1; break;
-        default: throw std::runtime_error("unknown program op");
-        }
+        for (int o = 0; o < n_operands(w[0]); o++) last[w[1 + o]] = (long)j;
     }
-    // linear scan: a slot per live value, operands freed before the result is placed
     std::vector<int> slot(nv, -1), free_slots;
-    int n_slots = 0, n_asserts = 0;
+    int n_slots = 0;
     auto take = [&]() {
         if (!free_slots.empty()) { int s = free_slots.back(); free_slots.pop_back(); return s; }
         return n_slots++;
     };
-    std::vector<uint32_t> dev;
     auto emit = [&](uint32_t op, uint32_t dst, uint32_t w1) { dev.push_back(op | (dst << 8)); dev.push_back(w1); };
+    auto is_const = [&](uint32_t v) { return v >= (uint32_t)n_inputs && code[4 * (v - n_inputs)] == OP_CONST; };
     for (size_t j = 0; j < n_instr; j++) {
+        if (!keep[j]) continue;
         const uint32_t *w = code + 4 * j;
-        const int nops = (w[0] == OP_ADD || w[0] == OP_SUB || w[0] == OP_MUL) ? 2 : (w[0] == OP_CONST ? 0 : 1);
+        const int nops = n_operands(w[0]);
         for (int o = 0; o < nops; o++) {
             const uint32_t v = w[1 + o];
             if (v < (uint32_t)n_inputs && slot[v] < 0) {
@@ -168,10 +226,7 @@ void compile(const uint32_t *code, size_t n_instr, int n_inputs, Program &out) {
             }
         }
         const uint32_t a = nops >= 1 ? (uint32_t)slot[w[1]] : 0, b = nops >= 2 ? (uint32_t)slot[w[2]] : 0;
-        if (w[0] == OP_ASSERT) {
-            emit(OP_ASSERT, 0, a);
-            n_asserts++;
-        }
+        if (w[0] == OP_ASSERT) emit(OP_ASSERT, (uint32_t)assert_rank[j], a);
         // operands whose last use is here (twice-used operands are freed once)
         for (int o = 0; o < nops; o++) {
             const uint32_t v = w[1 + o];
@@ -184,32 +239,179 @@ void compile(const uint32_t *code, size_t n_instr, int n_inputs, Program &out) {
         const size_t vid = (size_t)n_inputs + j;
         if (last[vid] < 0) continue;  // dead value: not computed
         slot[vid] = take();
+        // a product with a constant operand: extension times base field (the constant's
+        // slot holds it in coefficient 0)
         if (w[0] == OP_CONST) emit(OP_CONST, (uint32_t)slot[vid], to_mont(w[1]).v);
+        else if (w[0] == OP_MUL && is_const(w[2])) emit(OP_MULC, (uint32_t)slot[vid], a | (b << 16));
+        else if (w[0] == OP_MUL && is_const(w[1])) emit(OP_MULC, (uint32_t)slot[vid], b | (a << 16));
         else emit(w[0], (uint32_t)slot[vid], a | (b << 16));
     }
-    if (n_slots > 256) throw std::runtime_error("program keeps more than 256 values live");
-    out.n_inputs = n_inputs;
+    return std::max(1, n_slots);
+}
+
+// One partition (see compile): false when the program has too few constraints for G groups.
+bool build_partition(const uint32_t *code, size_t n_instr, int n_inputs, int G, Program &out) {
+    std::vector<char> is_assert(n_instr, 0);
+    std::vector<int> assert_rank(n_instr, -1);
+    int n_asserts = 0;
+    for (size_t j = 0; j < n_instr; j++)
+        if (code[4 * j] == OP_ASSERT) { is_assert[j] = 1; assert_rank[j] = n_asserts++; }
+    if (n_asserts > (1 << 24)) throw std::runtime_error("too many asserted constraints");
     out.n_asserts = n_asserts;
-    out.n_slots = std::max(1, n_slots);
-    out.dev = std::move(dev);
+    if (G > 1 && n_asserts < 2 * G) return false;
+    std::vector<int> weight(n_instr, 1);
+    for (size_t j = 0; j < n_instr; j++)
+        if (code[4 * j] == OP_MUL) weight[j] = 8;
+    long total = 0;
+    {
+        std::vector<char> used(n_instr, 0);
+        for (size_t j = n_instr; j-- > 0;) {
+            if (!is_assert[j] && !used[j]) continue;
+            used[j] = 1;
+            total += weight[j];
+            const uint32_t *w = code + 4 * j;
+            for (int o = 0; o < n_operands(w[0]); o++)
+                if (w[1 + o] >= (uint32_t)n_inputs) used[w[1 + o] - n_inputs] = 1;
+        }
+    }
+    std::vector<uint32_t> stack;
+    Partition part;
+    std::vector<char> keep(n_instr, 0);
+    long group_w = 0;
+    auto close_group = [&]() {
+        part.off.push_back((uint32_t)part.code.size());
+        const int ns = emit_slots(code, n_instr, n_inputs, keep, assert_rank, part.code);
+        part.n_slots = std::max(part.n_slots, ns);
+        part.groups++;
+        std::fill(keep.begin(), keep.end(), 0);
+        group_w = 0;
+    };
+    for (size_t j = 0; j < n_instr; j++) {
+        if (!is_assert[j]) continue;
+        // the cone of this ASSERT
+        stack.assign(1, (uint32_t)j);
+        while (!stack.empty()) {
+            const uint32_t i = stack.back();
+            stack.pop_back();
+            if (keep[i]) continue;
+            keep[i] = 1;
+            group_w += weight[i];
+            const uint32_t *w = code + 4 * i;
+            for (int o = 0; o < n_operands(w[0]); o++)
+                if (w[1 + o] >= (uint32_t)n_inputs) stack.push_back(w[1 + o] - n_inputs);
+        }
+        if (G > 1 && part.groups < G - 1 && group_w * G >= total) close_group();
+    }
+    if (part.groups == 0 || std::count(keep.begin(), keep.end(), 1) > 0) close_group();
+    part.off.push_back((uint32_t)part.code.size());
+    if (part.n_slots > 256) throw std::runtime_error("program keeps more than 256 values live");
+    out.parts.push_back(std::move(part));
+    return true;
+}
+
+}  // namespace
+
+void compile(const uint32_t *code_in, size_t n_instr_in, int n_inputs, Program &out) {
+    if (n_inputs < 1 || n_inputs > 65535) throw std::runtime_error("a program reads 1 to 65535 inputs");
+    {
+        std::vector<char> is_assert(n_instr_in, 0);
+        for (size_t j = 0; j < n_instr_in; j++) {
+            const uint32_t *w = code_in + 4 * j;
+            const uint64_t lim = (uint64_t)n_inputs + j;
+            auto use = [&](uint32_t v) {
+                if (v >= lim) throw std::runtime_error("program operand refers to a later value");
+                if (v >= (uint32_t)n_inputs && is_assert[v - n_inputs])
+                    throw std::runtime_error("program operand refers to an ASSERT, which defines no value");
+            };
+            switch (w[0]) {
+            case OP_CONST:
+                if (w[1] >= P) throw std::runtime_error("program constant is not a canonical field element");
+                break;
+            case OP_ADD: case OP_SUB: case OP_MUL: use(w[1]); use(w[2]); break;
+            case OP_NEG: use(w[1]); break;
+            case OP_ASSERT: use(w[1]); is_assert[j] = 1; break;
+            default: throw std::runtime_error("unknown program op");
+            }
+        }
+    }
+    out.n_inputs = n_inputs;
+    out.parts.clear();
+    // Partitions of the constraints into G groups (G = 1, 4, 16): consecutive
+    // ASSERTs, each group the union of its constraints' dependency cones (shared
+    // subexpressions are computed in every group that needs them), closed once it
+    // holds its share of the program's weighted instructions (a product weighs 8).
+    // Rounds with few points evaluate a point's constraints in G threads at once.
+    for (int G : {1, 4, 16})
+        if (!build_partition(code_in, n_instr_in, n_inputs, G, out)) break;
+    out.n_slots = out.parts[0].n_slots;
+    out.dev = out.parts[0].code;
 }
 
 void upload(wl_ctx *c, Program &p) {
-    p.dev_buf = DBuf<uint32_t>(c->pool, std::max<size_t>(2, p.dev.size()));
-    if (!p.dev.empty()) c->h2d(p.dev_buf.p, p.dev.data(), p.dev.size());
+    std::vector<uint32_t> all;
+    p.part_base.clear();
+    for (const Partition &q : p.parts) {
+        p.part_base.push_back(all.size());
+        all.insert(all.end(), q.off.begin(), q.off.end());
+        all.insert(all.end(), q.code.begin(), q.code.end());
+    }
+    p.dev_buf = DBuf<uint32_t>(c->pool, std::max<size_t>(2, all.size()));
+    if (!all.empty()) c->h2d(p.dev_buf.p, all.data(), all.size());
 }
 
+// The launch of one evaluation: the partition whose groups bring the round's threads
+// (points x evaluation points) closest to 2^17 from below, or the finest one; the
+// register file in LDS when the grid of 64-thread blocks is at most one block per CU
+// and the slots fit, else in scratch with 128-thread blocks.
+struct EvalLaunch {
+    size_t pi;
+    bool lds;
+    int gx, G;
+    size_t lds_bytes;
+};
+constexpr size_t EVAL_LDS_MAX = 160 * 1024 - 1024;  // LDS per CU, less the reduction's
+template <class T>
+EvalLaunch plan_eval(const Program &p, size_t len, int k, int nz) {
+    const size_t half = len >> k;
+    size_t pi = 0;
+    while (pi + 1 < p.parts.size() && half * (size_t)nz * (size_t)p.parts[pi].groups < ((size_t)1 << 17)) pi++;
+    EvalLaunch e;
+    e.pi = pi;
+    e.G = p.parts[pi].groups;
+    const int gx64 = (int)std::min<size_t>(nblk(half, 64), 1024);
+    e.lds_bytes = (size_t)p.parts[pi].n_slots * sizeof(T) * 64;
+    e.lds = (size_t)gx64 * nz * e.G <= 256 && e.lds_bytes <= EVAL_LDS_MAX;
+    e.gx = e.lds ? gx64 : (int)std::min<size_t>(nblk(half, 128), 1024);
+    return e;
+}
 template <class T>
 int launch_eval(const Program &p, const T *tab, size_t len, int k, const F *zsc, int nz, const EF *bat, const EF *eq,
                 EF *parts, hipStream_t s) {
-    const size_t half = len >> k;
-    const int gx = (int)std::min<size_t>(nblk(half, 128), 1024);
-    const dim3 grid(gx, nz);
-    const int n = (int)(p.dev.size() / 2);
-    if (p.n_slots <= 16) k_eval<T, 16><<<grid, 128, 0, s>>>(tab, len, k, zsc, p.dev_buf.p, n, bat, eq, parts);
-    else if (p.n_slots <= 64) k_eval<T, 64><<<grid, 128, 0, s>>>(tab, len, k, zsc, p.dev_buf.p, n, bat, eq, parts);
-    else k_eval<T, 256><<<grid, 128, 0, s>>>(tab, len, k, zsc, p.dev_buf.p, n, bat, eq, parts);
-    return gx;
+    const EvalLaunch e = plan_eval<T>(p, len, k, nz);
+    const Partition &q = p.parts[e.pi];
+    const dim3 grid(e.gx, nz, e.G);
+    const uint32_t *part = p.dev_buf.p + p.part_base[e.pi];
+    if (e.lds) {
+        static std::once_flag once;
+        std::call_once(once, [] {
+            (void)hipFuncSetAttribute((const void *)k_eval_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)EVAL_LDS_MAX);
+        });
+        k_eval_lds<T><<<grid, 64, e.lds_bytes, s>>>(tab, len, k, zsc, part, e.G, bat, eq, parts);
+    } else if (q.n_slots <= 16) {
+        k_eval<T, 16><<<grid, 128, 0, s>>>(tab, len, k, zsc, part, e.G, bat, eq, parts);
+    } else if (q.n_slots <= 64) {
+        k_eval<T, 64><<<grid, 128, 0, s>>>(tab, len, k, zsc, part, e.G, bat, eq, parts);
+    } else {
+        k_eval<T, 256><<<grid, 128, 0, s>>>(tab, len, k, zsc, part, e.G, bat, eq, parts);
+    }
+    return e.gx * e.G;
+}
+// partials per evaluation point launch_eval will write (for sizing the buffer)
+template <class T>
+int eval_parts(const Program &p, size_t len, int k, int nz) {
+    const EvalLaunch e = plan_eval<T>(p, len, k, nz);
+    return e.gx * e.G;
 }
 
 static int grind_bits(const Args &a, int degree) {
@@ -270,13 +472,13 @@ std::vector<EF> prove(wl_ctx *c, wh::Transcript &ts, const P2Dev *pdev, const vo
         if (!zs.empty()) {
             DBuf<F> dz(c->pool, zsc.size());
             c->h2d_async(dz.p, zsc.data(), zsc.size());
-            const size_t half = len >> k;
-            const int gx = (int)std::min<size_t>(nblk(half, 128), 1024);
-            DBuf<EF> parts(c->pool, (size_t)gx * zs.size());
+            const int np = cur_base ? eval_parts<F>(prog, len, k, (int)zs.size())
+                                    : eval_parts<EF>(prog, len, k, (int)zs.size());
+            DBuf<EF> parts(c->pool, (size_t)np * zs.size());
             if (cur_base) launch_eval(prog, (const F *)cur, len, k, dz, (int)zs.size(), dbat, eqm.p, parts, c->stream);
             else launch_eval(prog, (const EF *)cur, len, k, dz, (int)zs.size(), dbat, eqm.p, parts, c->stream);
             HIP_CHECK(hipGetLastError());
-            vals = reduce_many(c, parts, (int)zs.size(), gx);
+            vals = reduce_many(c, parts, (int)zs.size(), np);
         }
         // the round's points (z, p(z)), z = 0 .. maxz
         std::vector<F> xs;
