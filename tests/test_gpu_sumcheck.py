@@ -115,3 +115,25 @@ def test_program_options(gpu_ctx, oracle, skips, eq, rounds, grind, missing):
     s = rng.integers(0, P, 8, dtype=np.uint32)
     _run(gpu_ctx, oracle, p16, [9], skips, mles, prog, 2, bat, z, False, s, n_rounds=rounds, grinding=grind,
          missing=mm)
+
+
+def test_program_partitions(gpu_ctx, oracle):
+    """A program of 12 constraints over 2^17 points: the first round evaluates it
+    whole (2^17 threads), later rounds in 4 constraint groups (16 groups need 32
+    constraints; the Poseidon2 program above covers them), the smallest ones with
+    the register file in LDS; every round bit-exact."""
+    from whirlaway_amd.program import ConstraintProgram
+    _say("generic sumcheck, 12 constraints, n = 17 (1 and 4 constraint groups)")
+    b, p16 = _consts(oracle)
+    rng = np.random.default_rng(77)
+    n = 17
+    p = ConstraintProgram(n_inputs=3)
+    xs = [p.input(i) for i in range(3)]
+    for k in range(12):
+        i, j = rng.integers(0, 3, 2)
+        p.assert_zero(xs[i] * xs[j] * int(rng.integers(1, P)) - xs[(i + k) % 3] + int(rng.integers(0, P)))
+    mles = rng.integers(0, P, (3, 1 << n), dtype=np.uint32)
+    bat = rng.integers(0, P, (p.n_constraints, 8), dtype=np.uint32)
+    z = rng.integers(0, P, (n, 8), dtype=np.uint32)
+    s = rng.integers(0, P, 8, dtype=np.uint32)
+    _run(gpu_ctx, oracle, p16, [3], 1, mles, p, 2, bat, z, False, s)
