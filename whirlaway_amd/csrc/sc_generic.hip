@@ -74,8 +74,13 @@ __global__ void __launch_bounds__(128) k_eval(const T *tables, size_t len, int k
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < half; x += stride) {
         EF acc = ezero();
+        // the next instruction's words are fetched while this one executes (the
+        // program buffer has two words of padding after the last group)
+        uint32_t nw0 = prog[0], nw1 = prog[1];
         for (int pc = 0; pc < n_prog; pc++) {
-            const uint32_t w0 = prog[2 * pc], w1 = prog[2 * pc + 1];
+            const uint32_t w0 = nw0, w1 = nw1;
+            nw0 = prog[2 * pc + 2];
+            nw1 = prog[2 * pc + 3];
             const uint32_t op = w0 & 0xff, dst = w0 >> 8, a = w1 & 0xffff, b = w1 >> 16;
             switch (op) {
             case OP_LOAD: regs[dst] = load_folded(tables + (size_t)a * len, half, x, s, K); break;
@@ -130,8 +135,13 @@ __global__ void __launch_bounds__(64) k_eval_lds(const T *tables, size_t len, in
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < half; x += stride) {
         EF acc = ezero();
+        // the next instruction's words are fetched while this one executes (the
+        // program buffer has two words of padding after the last group)
+        uint32_t nw0 = prog[0], nw1 = prog[1];
         for (int pc = 0; pc < n_prog; pc++) {
-            const uint32_t w0 = prog[2 * pc], w1 = prog[2 * pc + 1];
+            const uint32_t w0 = nw0, w1 = nw1;
+            nw0 = prog[2 * pc + 2];
+            nw1 = prog[2 * pc + 3];
             const uint32_t op = w0 & 0xff, dst = w0 >> 8, a = w1 & 0xffff, b = w1 >> 16;
             switch (op) {
             case OP_LOAD: wr(dst, load_folded(tables + (size_t)a * len, half, x, s, K)); break;
@@ -152,6 +162,125 @@ __global__ void __launch_bounds__(64) k_eval_lds(const T *tables, size_t len, in
     }
     const EF r = block_sum(total, lds);
     if (threadIdx.x == 0) parts[((size_t)blockIdx.y * G + blockIdx.z) * gridDim.x + blockIdx.x] = r;
+}
+
+// ---- extension-field rounds with 16 lanes per point ----
+// Lane l of a 16-lane DPP row holds coefficient l mod 8 of every value of its
+// point (the row's two halves hold the same coefficients, so a rotation within the
+// row is a rotation of the 8 coefficients).  Additions are lane-local; a product
+// c_k = sum_r a_{k - r mod 8} b_r (3 when r > k, from X^8 = 3) takes the rotations
+// of a (row_ror) and the broadcasts of b and 3b (row_newbcast) into one 64-bit sum
+// per lane.  An interpreted instruction costs a few lane instructions instead of a
+// whole extension product per lane, which shortens the dependent chain of a point.
+template <int R>
+__device__ __forceinline__ uint32_t dpp_ror(uint32_t v) {
+    if constexpr (R == 0) return v;
+    else return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x120 + R, 0xF, 0xF, false);
+}
+template <int R>
+__device__ __forceinline__ uint32_t dpp_bcast(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + R, 0xF, 0xF, false);
+}
+// 64-bit sum of a_{k - r} b_r (times 3 when r > k) over r, for coefficient k of this lane
+template <int R = 0>
+__device__ __forceinline__ void lane_mul_acc(uint64_t &acc, uint32_t a, uint32_t b, uint32_t b3, int k) {
+    if constexpr (R < 8) {
+        if (R == 4) acc = fold64(acc);
+        const uint32_t ar = dpp_ror<R>(a), br = dpp_bcast<R>(b), b3r = dpp_bcast<R>(b3);
+        acc += (uint64_t)ar * (R > k ? b3r : br);
+        lane_mul_acc<R + 1>(acc, a, b, b3, k);
+    }
+}
+__device__ __forceinline__ uint32_t lane_mul(uint32_t a, uint32_t b, int k) {
+    uint64_t acc = 0;
+    lane_mul_acc(acc, a, b, ftimes3(F{b}).v, k);
+    return redc64(acc);
+}
+// sum_r a_{k - r} B_r (3 B_r when r > k) for a uniform extension constant B (B3 = 3B)
+template <int R = 0>
+__device__ __forceinline__ void lane_mul_const_acc(uint64_t &acc, uint32_t a, const EF &B, const uint32_t *B3, int k) {
+    if constexpr (R < 8) {
+        if (R == 4) acc = fold64(acc);
+        acc += (uint64_t)dpp_ror<R>(a) * (R > k ? B3[R] : B.c[R].v);
+        lane_mul_const_acc<R + 1>(acc, a, B, B3, k);
+    }
+}
+
+// k_eval for extension-field tables: 16 points per 256-thread block, register file
+// in LDS ([slot][thread], one coefficient per thread); partials as k_eval's.
+__global__ void __launch_bounds__(256) k_eval_lanes(const EF *tables, size_t len, int k, const F *zsc,
+                                                    const uint32_t *part, int G, const EF *batching, const EF *eq,
+                                                    EF *parts) {
+    extern __shared__ uint32_t lregs[];
+    __shared__ uint32_t red[4][8];
+    const int t = threadIdx.x, co = t & 7, pt = t >> 4;
+    auto rd = [&](uint32_t slot) { return lregs[slot * 256 + t]; };
+    auto wr = [&](uint32_t slot, uint32_t v) { lregs[slot * 256 + t] = v; };
+    const int K = 1 << k;
+    const size_t half = len >> k;
+    const F *s = zsc + (size_t)blockIdx.y * K;
+    const uint32_t *prog = part + (G + 1) + part[blockIdx.z];
+    const int n_prog = (int)(part[blockIdx.z + 1] - part[blockIdx.z]) / 2;
+    uint32_t total = 0;  // coefficient co of this lane's points' sum
+    const size_t stride = (size_t)gridDim.x * 16;
+    // every lane of the block runs the same number of iterations (DPP needs the row)
+    for (size_t x0 = (size_t)blockIdx.x * 16; x0 < half; x0 += stride) {
+        const size_t x = x0 + pt;
+        const bool live = x < half;
+        const size_t xl = live ? x : x0;
+        uint64_t acc = 0;
+        int n_acc = 0;
+        uint32_t nw0 = prog[0], nw1 = prog[1];
+        for (int pc = 0; pc < n_prog; pc++) {
+            const uint32_t w0 = nw0, w1 = nw1;
+            nw0 = prog[2 * pc + 2];
+            nw1 = prog[2 * pc + 3];
+            const uint32_t op = w0 & 0xff, dst = w0 >> 8, a = w1 & 0xffff, b = w1 >> 16;
+            switch (op) {
+            case OP_LOAD: {
+                const EF *tab = tables + (size_t)a * len;
+                uint64_t f = 0;
+                for (int j = 0; j < K; j++) {
+                    if (j && (j & 3) == 0) f = fold64(f);
+                    f += (uint64_t)s[j].v * tab[(size_t)j * half + xl].c[co].v;
+                }
+                wr(dst, redc64(f));
+                break;
+            }
+            case OP_CONST: wr(dst, co == 0 ? w1 : 0u); break;
+            case OP_ADD: wr(dst, fadd(F{rd(a)}, F{rd(b)}).v); break;
+            case OP_SUB: wr(dst, fsub(F{rd(a)}, F{rd(b)}).v); break;
+            case OP_MUL: wr(dst, lane_mul(rd(a), rd(b), co)); break;
+            case OP_MULC: wr(dst, fmul(F{rd(a)}, F{dpp_bcast<0>(rd(b))}).v); break;
+            case OP_NEG: wr(dst, fneg(F{rd(a)}).v); break;
+            default: {  // OP_ASSERT: batching[dst] * value into the lane's lazy sum
+                const EF B = batching[dst];
+                uint32_t B3[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) B3[i] = ftimes3(B.c[i]).v;
+                if (n_acc == 64) { acc = fold64(acc); n_acc = 0; }
+                uint64_t prod = 0;
+                lane_mul_const_acc(prod, rd(a), B, B3, co);
+                acc += fold64(prod);
+                n_acc++;
+            }
+            }
+        }
+        uint32_t v = redc64(acc);
+        if (eq) v = lane_mul(v, eq[xl].c[co].v, co);
+        if (live) total = fadd(F{total}, F{v}).v;
+    }
+    // sum over the block's 16 points: the four rows of a wave, then the four waves
+    total = fadd(F{total}, F{(uint32_t)__shfl_xor((int)total, 16, 64)}).v;
+    total = fadd(F{total}, F{(uint32_t)__shfl_xor((int)total, 32, 64)}).v;
+    if ((t & 63) < 8) red[t >> 6][t & 7] = total;
+    __syncthreads();
+    if (t < 8) {
+        uint32_t r = red[0][t];
+        for (int w = 1; w < 4; w++) r = fadd(F{r}, F{red[w][t]}).v;
+        EF *o = parts + ((size_t)blockIdx.y * G + blockIdx.z) * gridDim.x + blockIdx.x;
+        o->c[t] = F{r};
+    }
 }
 
 // out[m][i] = sum_j sc[j] in[m][j M + i], M = len / K, for every table m
@@ -355,33 +484,51 @@ void upload(wl_ctx *c, Program &p) {
         all.insert(all.end(), q.off.begin(), q.off.end());
         all.insert(all.end(), q.code.begin(), q.code.end());
     }
-    p.dev_buf = DBuf<uint32_t>(c->pool, std::max<size_t>(2, all.size()));
-    if (!all.empty()) c->h2d(p.dev_buf.p, all.data(), all.size());
+    all.push_back(0);  // padding read by the last instruction's prefetch
+    all.push_back(0);
+    p.dev_buf = DBuf<uint32_t>(c->pool, all.size());
+    c->h2d(p.dev_buf.p, all.data(), all.size());
 }
 
-// The launch of one evaluation: the partition whose groups bring the round's threads
-// (points x evaluation points) closest to 2^17 from below, or the finest one; the
-// register file in LDS when the grid of 64-thread blocks is at most one block per CU
-// and the slots fit, else in scratch with 128-thread blocks.
+// The launch of one evaluation.  Extension-field tables: 16 lanes per point
+// (k_eval_lanes) with the partition whose groups bring the round's lanes to 2^17,
+// or the finest one.  Base-field tables: the partition whose groups bring the
+// round's threads (points x evaluation points) closest to 2^17 from below, or the
+// finest one; the register file in LDS when the grid of 64-thread blocks is at most
+// one block per CU and the slots fit, else in scratch with 128-thread blocks.
+enum EvalMode { EVAL_SCRATCH, EVAL_LDS, EVAL_LANES };
 struct EvalLaunch {
     size_t pi;
-    bool lds;
+    EvalMode mode;
     int gx, G;
     size_t lds_bytes;
 };
-constexpr size_t EVAL_LDS_MAX = 160 * 1024 - 1024;  // LDS per CU, less the reduction's
+constexpr size_t EVAL_LDS_MAX = 160 * 1024 - 1024;  // LDS per CU, less the reductions'
 template <class T>
 EvalLaunch plan_eval(const Program &p, size_t len, int k, int nz) {
     const size_t half = len >> k;
+    EvalLaunch e;
+    if (std::is_same<T, EF>::value) {
+        size_t pi = 0;
+        while (pi + 1 < p.parts.size() && half * (size_t)nz * (size_t)p.parts[pi].groups * 16 < ((size_t)1 << 17))
+            pi++;
+        e.lds_bytes = (size_t)p.parts[pi].n_slots * 256 * 4;
+        if (e.lds_bytes <= EVAL_LDS_MAX) {
+            e.pi = pi;
+            e.mode = EVAL_LANES;
+            e.G = p.parts[pi].groups;
+            e.gx = (int)std::min<size_t>(nblk(half, 16), 1024);
+            return e;
+        }
+    }
     size_t pi = 0;
     while (pi + 1 < p.parts.size() && half * (size_t)nz * (size_t)p.parts[pi].groups < ((size_t)1 << 17)) pi++;
-    EvalLaunch e;
     e.pi = pi;
     e.G = p.parts[pi].groups;
     const int gx64 = (int)std::min<size_t>(nblk(half, 64), 1024);
     e.lds_bytes = (size_t)p.parts[pi].n_slots * sizeof(T) * 64;
-    e.lds = (size_t)gx64 * nz * e.G <= 256 && e.lds_bytes <= EVAL_LDS_MAX;
-    e.gx = e.lds ? gx64 : (int)std::min<size_t>(nblk(half, 128), 1024);
+    e.mode = (size_t)gx64 * nz * e.G <= 256 && e.lds_bytes <= EVAL_LDS_MAX ? EVAL_LDS : EVAL_SCRATCH;
+    e.gx = e.mode == EVAL_LDS ? gx64 : (int)std::min<size_t>(nblk(half, 128), 1024);
     return e;
 }
 template <class T>
@@ -391,7 +538,16 @@ int launch_eval(const Program &p, const T *tab, size_t len, int k, const F *zsc,
     const Partition &q = p.parts[e.pi];
     const dim3 grid(e.gx, nz, e.G);
     const uint32_t *part = p.dev_buf.p + p.part_base[e.pi];
-    if (e.lds) {
+    if (e.mode == EVAL_LANES) {
+        if constexpr (std::is_same<T, EF>::value) {
+            static std::once_flag once;
+            std::call_once(once, [] {
+                (void)hipFuncSetAttribute((const void *)k_eval_lanes, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)EVAL_LDS_MAX);
+            });
+            k_eval_lanes<<<grid, 256, e.lds_bytes, s>>>(tab, len, k, zsc, part, e.G, bat, eq, parts);
+        }
+    } else if (e.mode == EVAL_LDS) {
         static std::once_flag once;
         std::call_once(once, [] {
             (void)hipFuncSetAttribute((const void *)k_eval_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
