@@ -527,7 +527,11 @@ EvalLaunch plan_eval(const Program &p, size_t len, int k, int nz) {
     e.G = p.parts[pi].groups;
     const int gx64 = (int)std::min<size_t>(nblk(half, 64), 1024);
     e.lds_bytes = (size_t)p.parts[pi].n_slots * sizeof(T) * 64;
-    e.mode = (size_t)gx64 * nz * e.G <= 256 && e.lds_bytes <= EVAL_LDS_MAX ? EVAL_LDS : EVAL_SCRATCH;
+    // base-field slots are one word: with at most 32 KB per block several blocks share a
+    // CU, so large launches take the LDS register file too
+    const bool lds_small = (size_t)gx64 * nz * e.G <= 256 && e.lds_bytes <= EVAL_LDS_MAX;
+    const bool lds_any = std::is_same<T, F>::value && e.lds_bytes <= (32u << 10);
+    e.mode = lds_small || lds_any ? EVAL_LDS : EVAL_SCRATCH;
     e.gx = e.mode == EVAL_LDS ? gx64 : (int)std::min<size_t>(nblk(half, 128), 1024);
     return e;
 }
