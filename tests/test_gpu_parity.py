@@ -105,7 +105,8 @@ def test_sumcheck_a2b(gpu_ctx, oracle, n):
 
 
 @pytest.mark.parametrize("n_cols,log_m,log_rows", [(4, 5, 6), (8, 10, 12), (3, 10, 13), (3, 13, 14), (2, 13, 16),
-                                                   (16, 6, 15), (2, 18, 19), (1, 19, 20)])
+                                                   (16, 6, 15), (2, 18, 19), (1, 19, 20), (3, 13, 21),
+                                                   (2, 18, 21), (1, 20, 22), (2, 22, 23), (1, 23, 24)])
 def test_rs_commit(gpu_ctx, oracle, n_cols, log_m, log_rows):
     from whirlaway_amd import utils
     _say(f"rs_commit {n_cols}x2^{log_m} -> 2^{log_rows}")

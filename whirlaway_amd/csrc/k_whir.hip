@@ -341,17 +341,112 @@ __global__ void __launch_bounds__(256) k_ntt2_b(const F *Y, int log_m, int LA, c
     }
 }
 
-// Compact twiddle tables for the two-pass encoding of length 2^L:
-// [twA: w_NA^i, NA/2 | twB: w_NB^i, NB/2 | tl: w_N^i, 1024 | th: w_N^(1024 i), N/1024]
-__global__ void k_ntt_tables(const F *tw_full, int L, int LA, F *out) {
-    const int LB = L - LA;
+// Middle pass of the three-pass encoding of length 2^L, L = LA1 + LM + 10 (lengths
+// beyond the two-pass range).  Pass A of the two-pass scheme, a DFT of length
+// NA = 2^(LA1+LM) along nA, splits into two strided passes with nA = NM*nA1 + nM
+// and kA = kA1 + NA1*kM:
+//   A1 (k_ntt2_a<LA1> with stride NM*NB): Mobius over nA1, DFT along nA1
+//        -> Z[(kA1*NM + nM)*NB + nB];
+//   M  (this kernel): Mobius over nM, twiddle w_NA^(nM*kA1), DFT of length NM
+//        along nM -> Y[(kA1 + NA1*kM)*NB + nB], the layout pass B reads;
+// then pass B (k_ntt2_b<10>) runs in place on Y.  The twiddle couples nM with kA1,
+// so the whole Mobius over nM precedes it; the phases are those of pass B with
+// rows strided by NB instead of contiguous.  Block: T = NTT_TILE_A / NM
+// consecutive nB at one kA1 (consecutive threads take consecutive nB, so global
+// accesses are coalesced and LDS rows, xor-swizzled by t, hit distinct banks);
+// grid (NA1 * NB / T, cols).  th[i] = w_NA^i (i < NA), twM[i] = w_NM^i.
+template <int LM>
+__global__ void __launch_bounds__(512) k_ntt3_m(const F *Z, int log_m, int LA1, const F *twM, const F *th, F *Y) {
+    constexpr int LB = 10, NB = 1 << LB;
+    constexpr int NM = 1 << LM, T = NTT_TILE_A / NM, BPK = NB / T;  // blocks per kA1
+    constexpr int R1 = (LM + 1) / 2, R2 = LM - R1;  // high / low natural bits of nM
+    static_assert(T <= NB && NM >= 64, "tile shape");
+    __shared__ F tile[NTT_TILE_A];
+    const int L = LA1 + LM + LB;
+    const uint32_t NAm = (1u << (LA1 + LM)) - 1;
+    const uint32_t kA1 = blockIdx.x / BPK;
+    const int nB0 = (blockIdx.x % BPK) * T;
+    const F *z = Z + ((size_t)blockIdx.y << L) + (size_t)kA1 * NM * NB + nB0;
+    F *y = Y + ((size_t)blockIdx.y << L) + nB0;
+    auto at = [](int t, uint32_t x) { return t * NM + (x ^ (uint32_t)(t & (NM - 1))); };
+    // M1: natural low R2 bits of nM in registers (bit q of nM is bit LB + q of n)
+    uint32_t maskl = 0;
+#pragma unroll
+    for (int q = 0; q < R2; q++)
+        if (LB + q < log_m) maskl |= 1u << q;
+    for (int gid = threadIdx.x; gid < (T << R1); gid += blockDim.x) {
+        const int t = gid % T, h = gid / T;
+        F a[1 << R2];
+#pragma unroll
+        for (int j = 0; j < (1 << R2); j++) a[j] = z[(size_t)((h << R2) | j) * NB + t];
+        reg_mobius<R2>(a, maskl);
+#pragma unroll
+        for (int j = 0; j < (1 << R2); j++) tile[at(t, (h << R2) | j)] = a[j];
+    }
+    __syncthreads();
+    // M2: natural high R1 bits in registers; Mobius, twiddle, DIT stages 1..R1
+    uint32_t maskh = 0;
+#pragma unroll
+    for (int b = 0; b < R1; b++)
+        if (LB + R2 + b < log_m) maskh |= 1u << (R1 - 1 - b);
+    const F step = th[(kA1 << R2) & NAm];  // w_NA^(kA1 * 2^R2)
+    for (int gid = threadIdx.x; gid < (T << R2); gid += blockDim.x) {
+        const int t = gid % T, l = gid / T;
+        F a[1 << R1];
+#pragma unroll
+        for (int s = 0; s < (1 << R1); s++) a[s] = tile[at(t, (brevc(s, R1) << R2) | l)];
+        reg_mobius<R1>(a, maskh);
+        // nM = j * 2^R2 + l: w_NA^(kA1 nM) from base w_NA^(kA1 l) and the step above
+        F cur = th[(kA1 * (uint32_t)l) & NAm];
+#pragma unroll
+        for (int j = 0; j < (1 << R1); j++) {
+            const int s = brevc(j, R1);
+            a[s] = fmul(a[s], cur);
+            cur = fmul(cur, step);
+        }
+        reg_dit<R1, LM, 0>(a, 0, twM);
+#pragma unroll
+        for (int s = 0; s < (1 << R1); s++) tile[at(t, (brevc(s, R1) << R2) | l)] = a[s];
+    }
+    __syncthreads();
+    // M3: storage bits [R1, LM) in registers; DIT stages R1+1..LM; kM = storage position
+    for (int gid = threadIdx.x; gid < (T << R1); gid += blockDim.x) {
+        const int t = gid % T, c = gid / T;
+        const uint32_t hi = brevc(c, R1) << R2;
+        F a[1 << R2];
+#pragma unroll
+        for (int s = 0; s < (1 << R2); s++) a[s] = tile[at(t, hi | brevc(s, R2))];
+        reg_dit<R2, LM, R1>(a, (uint32_t)c, twM);
+#pragma unroll
+        for (int s = 0; s < (1 << R2); s++) {
+            const uint32_t kM = ((uint32_t)s << R1) | (uint32_t)c;
+            y[(size_t)(kA1 + (kM << LA1)) * NB + t] = a[s];
+        }
+    }
+}
+// the split NA = NA1 * NM of the three-pass encoding's pass A
+static inline void ntt3_split(int LA, int *LA1, int *LM) {
+    *LA1 = LA / 2;
+    *LM = LA - *LA1;
+}
+
+// Compact twiddle tables for the two- and three-pass encodings of length 2^L:
+// [twA: w_NA^i, NA/2 | twB: w_NB^i, NB/2 | tl: w_N^i, 1024 | th: w_N^(1024 i), N/1024].
+// With three passes (LA1 < LA) the twA region holds twA1: w_NA1^i (NA1/2 words),
+// then twM: w_NM^i (NM/2 words), then zeros.
+__global__ void k_ntt_tables(const F *tw_full, int L, int LA, int LA1, F *out) {
+    const int LB = L - LA, LM = LA - LA1;
     const int na = 1 << (LA - 1), nb = 1 << (LB - 1), nh = 1 << (L - 10);
     const size_t N = (size_t)1 << L;
     auto w = [&](size_t e) -> F { return tw_full[e & (N - 1)]; };
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < na + nb + 1024 + nh; i += gridDim.x * blockDim.x) {
         F v;
-        if (i < na) v = w((size_t)i << LB);
-        else if (i < na + nb) v = w((size_t)(i - na) << LA);
+        if (i < na) {
+            if (LA1 == LA) v = w((size_t)i << LB);
+            else if (i < (1 << (LA1 - 1))) v = w((size_t)i << (L - LA1));
+            else if (i < (1 << (LA1 - 1)) + (1 << (LM - 1))) v = w((size_t)(i - (1 << (LA1 - 1))) << (L - LM));
+            else v = F{0};
+        } else if (i < na + nb) v = w((size_t)(i - na) << LA);
         else if (i < na + nb + 1024) v = w((size_t)(i - na - nb));
         else v = w((size_t)(i - na - nb - 1024) << 10);
         out[i] = v;
@@ -363,16 +458,19 @@ size_t ntt_tables_words(int L, int variant) {
     return ((size_t)1 << (LA - 1)) + ((size_t)1 << (LB - 1)) + 1024 + ((size_t)1 << (L - 10));
 }
 void ntt_tables(const F *tw_full, int L, F *out, hipStream_t s, int variant) {
-    int LA, LB;
+    int LA, LB, LA1 = 0, LM = 0;
     ntt_split(L, &LA, &LB, variant);
-    k_ntt_tables<<<8, 256, 0, s>>>(tw_full, L, LA, out);
+    if (L > NTT2_MAX_L) ntt3_split(LA, &LA1, &LM);
+    else LA1 = LA;
+    k_ntt_tables<<<L > NTT2_MAX_L ? 64 : 8, 256, 0, s>>>(tw_full, L, LA, LA1, out);
 }
 // LB = min(L - 4, 10) by default (variant 0 or negative); variants 1 and 2 take
-// LB = 9 and 8 where that leaves 4 <= LA <= 10
+// LB = 9 and 8 where that leaves 4 <= LA <= 10.  Beyond 2^20 rows (three passes)
+// LB = 10 and LA = L - 10 for every variant.
 bool ntt_split(int L, int *LA, int *LB, int variant) {
-    if (L < NTT2_MIN_L || L > NTT2_MAX_L) { *LA = 0; *LB = L; return false; }
+    if (L < NTT2_MIN_L || L > NTT3_MAX_L) { *LA = 0; *LB = L; return false; }
     *LB = L - 4 < 10 ? L - 4 : 10;
-    if (variant > 0 && variant < NTT_VARIANTS) {
+    if (variant > 0 && variant < NTT_VARIANTS && L <= NTT2_MAX_L) {
         const int lb = 10 - variant, la = L - lb;
         if (la >= 4 && la <= 10) *LB = lb;
     }
@@ -396,6 +494,24 @@ static void launch_b(const F *Y, int n, int log_m, int LA, const F *tabs, F *cw,
     (void)L;
 }
 
+template <int LM>
+static void launch_m(const F *Z, int n, int log_m, int LA1, const F *twM, const F *th, F *Y, hipStream_t s) {
+    constexpr int T = NTT_TILE_A / (1 << LM);
+    k_ntt3_m<LM><<<dim3((unsigned)((((size_t)1 << LA1) << 10) / T), n), 512, 0, s>>>(Z, log_m, LA1, twM, th, Y);
+}
+static void launch_a_any(int LA, const F *cols, int n, int log_m, int LB, const F *twA, F *Y, hipStream_t s) {
+    switch (LA) {
+        case 4: launch_a<4>(cols, n, log_m, LB, twA, Y, s); break;
+        case 5: launch_a<5>(cols, n, log_m, LB, twA, Y, s); break;
+        case 6: launch_a<6>(cols, n, log_m, LB, twA, Y, s); break;
+        case 7: launch_a<7>(cols, n, log_m, LB, twA, Y, s); break;
+        case 8: launch_a<8>(cols, n, log_m, LB, twA, Y, s); break;
+        case 9: launch_a<9>(cols, n, log_m, LB, twA, Y, s); break;
+        case 10: launch_a<10>(cols, n, log_m, LB, twA, Y, s); break;
+        default: throw std::runtime_error("rs_encode: pass A length out of range");
+    }
+}
+
 int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int L, F *cw, F *tmp,
               const F *tw_full, const F *tabs, hipStream_t s, bool zero_fill, int variant) {
     size_t rows = (size_t)1 << L;
@@ -410,17 +526,24 @@ int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int L, F *cw,
         k_ntt_small<<<n_nonzero, 256, sh, s>>>(cols, log_m, L, tw_full, cw);
         return 0;
     }
-    if (two) {
-        const F *twA = tabs;
-        switch (LA) {
-            case 4: launch_a<4>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
-            case 5: launch_a<5>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
-            case 6: launch_a<6>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
-            case 7: launch_a<7>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
-            case 8: launch_a<8>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
-            case 9: launch_a<9>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
-            default: launch_a<10>(cols, n_nonzero, log_m, LB, twA, tmp, s); break;
+    if (two && L > NTT2_MAX_L) {
+        // three passes: A1 cols -> tmp, M tmp -> cw, B in place on cw (each block of
+        // pass B reads its rows whole before it writes them)
+        int LA1, LM;
+        ntt3_split(LA, &LA1, &LM);
+        const F *twA1 = tabs, *twM = tabs + ((size_t)1 << (LA1 - 1));
+        const F *th = tabs + ((size_t)1 << (LA - 1)) + ((size_t)1 << (LB - 1)) + 1024;
+        launch_a_any(LA1, cols, n_nonzero, log_m, LM + LB, twA1, tmp, s);
+        switch (LM) {
+            case 6: launch_m<6>(tmp, n_nonzero, log_m, LA1, twM, th, cw, s); break;
+            case 7: launch_m<7>(tmp, n_nonzero, log_m, LA1, twM, th, cw, s); break;
+            default: throw std::runtime_error("rs_encode: middle pass length out of range");
         }
+        launch_b<10>(cw, n_nonzero, log_m, LA, tabs, cw, s);
+        return LA;
+    }
+    if (two) {
+        launch_a_any(LA, cols, n_nonzero, log_m, LB, tabs, tmp, s);
         if (LB == 8) launch_b<8>(tmp, n_nonzero, log_m, LA, tabs, cw, s);
         else if (LB == 9) launch_b<9>(tmp, n_nonzero, log_m, LA, tabs, cw, s);
         else launch_b<10>(tmp, n_nonzero, log_m, LA, tabs, cw, s);

@@ -168,8 +168,9 @@ int rs_encode(const F *cols, int n_cols, int n_nonzero, int log_m, int log_rows,
               F *tmp, const F *tw_full, const F *tabs, hipStream_t s, bool zero_fill = true, int variant = 0);
 // lengths up to 2^12 are encoded in one block per column, 2^14 .. 2^20 by the
 // two-pass register kernels with L = LA + LB (LB = 10 by default; variants 1, 2:
-// LB = 9, 8), others by LDS four-step kernels
-constexpr int NTT2_MIN_L = 14, NTT2_MAX_L = 20;
+// LB = 9, 8), 2^21 .. 2^24 by three register passes (pass A split in two strided
+// passes, LB = 10), 2^13 by LDS four-step kernels
+constexpr int NTT2_MIN_L = 14, NTT2_MAX_L = 20, NTT3_MAX_L = 24;
 constexpr int NTT_VARIANTS = 3, LEAF_VARIANTS = 4, TREE_VARIANTS = 6;
 bool ntt_split(int L, int *LA, int *LB, int variant = 0);
 size_t ntt_tables_words(int L, int variant = 0);
