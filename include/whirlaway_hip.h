@@ -254,6 +254,14 @@ int wl_kernel_timing_read(wl_ctx *ctx, int kernel_class, long *launches, double 
  * the reference; used by the benchmark beside its VALU roofline). */
 int wl_fmul_rate(wl_ctx *ctx, double *fmul_per_s);
 
+/* Host-side check of the device Poseidon2 form (partial rounds as a convolution,
+ * paired diagonal entries) against the round-by-round permutation: random round
+ * constants from `seed`, n_states random states plus field-extreme ones, for
+ * width 16 (mismatches[0]) and 24 (mismatches[1]).  No GPU is used; no
+ * counterpart in the reference (it checks this build's own rewriting of
+ * Poseidon2Koalabear's permutation). */
+int wl_p2_conv_check(uint32_t n_states, uint64_t seed, uint64_t mismatches[2]);
+
 /* Device memory of a context's pool: bytes held (live + cached) and the peak of
  * live bytes since the last reset (no counterpart in the reference). */
 void wl_mem_stats(wl_ctx *ctx, uint64_t *held, uint64_t *peak_live, int reset_peak);

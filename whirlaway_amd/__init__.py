@@ -72,6 +72,7 @@ def lib():
         u64p = ctypes.POINTER(u64)
         L.wl_mem_stats.argtypes = [vp, u64p, u64p, ctypes.c_int]
         L.wl_fmul_rate.argtypes = [vp, dp]
+        L.wl_p2_conv_check.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
         L.wl_job_footprint.restype = u64
         L.wl_job_footprint.argtypes = [ctypes.c_uint32, vp]
         L.wl_air_settings_validate.argtypes = [ctypes.c_uint32, ctypes.c_uint32, vp]

@@ -64,9 +64,38 @@ constexpr PartialConv<W, RP> make_pconv(const uint32_t *diag_mont) {
         }
         t.csum[k - 1] = cmont(cs);
     }
+    // h = d^RP / 2 of each pair (d = the entry DiagIdx::plus(j))
+    for (int j = 0; j < DiagIdx<W>::NP; j++) {
+        const uint64_t d = from_cmont(diag_mont[DiagIdx<W>::plus(j)]);
+        uint64_t h = cinv2k(1);
+        for (int k = 0; k < RP; k++) h = cmulmod(h, d);
+        t.hpow[j] = cmont(h);
+    }
     t.wm1 = cmont(W - 1);
     return t;
 }
+// DiagIdx lists every entry 1..W-1 once, each pair as d and -d
+template <int W>
+constexpr bool diag_idx_ok(const uint32_t *diag_mont) {
+    using D = DiagIdx<W>;
+    int seen[W] = {};
+    for (int j = 0; j < D::NP; j++) {
+        const int a = D::plus(j), b = D::minus(j);
+        if (a < 1 || a >= W || b < 1 || b >= W) return false;
+        seen[a]++, seen[b]++;
+        if ((from_cmont(diag_mont[a]) + from_cmont(diag_mont[b])) % P != 0) return false;
+    }
+    for (int j = 0; j < D::NS; j++) {
+        const int a = D::single(j);
+        if (a < 1 || a >= W) return false;
+        seen[a]++;
+    }
+    for (int i = 1; i < W; i++)
+        if (seen[i] != 1) return false;
+    return 2 * D::NP + D::NS == W - 1;
+}
+static_assert(diag_idx_ok<16>(DIAG.d16), "width-16 diagonal pairs");
+static_assert(diag_idx_ok<24>(DIAG.d24), "width-24 diagonal pairs");
 constexpr PartialConv<16, 20> PCONV16 = make_pconv<16, 20>(DIAG.d16);
 constexpr PartialConv<24, 23> PCONV24 = make_pconv<24, 23>(DIAG.d24);
 __device__ __constant__ static const PartialConv<16, 20> DPCONV16 = make_pconv<16, 20>(DIAG.d16);

@@ -474,13 +474,42 @@ KB_HD void p2_end(F *s, const C &k) {
 // with A(k) = sum_{i>=1} d_i^k s_i(0) and C(k) = sum_{i>=1} d_i^k, and the final
 // s_i(RP) comes from the closed form.  Every sum is a dot product of canonical
 // values with Montgomery constants: 64-bit multiply-adds, a fold64 after every
-// four, one reduction.  About 2,500 instructions for width 24 against 4,400 for
-// the round-by-round diagonal (which multiplies by shifts but reduces every
-// element every round).
+// four, one reduction.
+//
+// Most entries d_i (i >= 1) come in pairs d, -d (DiagIdx).  For a pair (a, b) =
+// (s_+(0), s_-(0)) the terms d^k a + (-d)^k b = d^k (a + (-1)^k b) take one
+// product with P = a + b (k even) or M = a - b (k odd).  In the closed form, with
+// E and O the sums of d^e S(u) over the even and the odd exponents e = RP-1-u,
+//   s_+(RP) = E + O + h (P + M) + S(RP-1),
+//   s_-(RP) = E - O + (-1)^RP h (P - M) + S(RP-1),      h = d^RP / 2,
+// i.e. s_+/- = E' +/- O' + S(RP-1) with E' = E + h M, O' = O + h P for RP odd
+// (E' = E + h P, O' = O + h M for RP even): both elements of a pair cost the
+// products of one.
+template <int W>
+struct DiagIdx;
+// width 16: pairs (3,6) 1/2, (4,7) 3, (5,8) 4, (9,12) 2^-8, (10,13) 1/8, (11,15) 2^-24;
+// singles 1, 2, -1/16
+template <>
+struct DiagIdx<16> {
+    static constexpr int NP = 6, NS = 3;
+    KB_HD static constexpr int plus(int j) { constexpr int t[6] = {3, 4, 5, 9, 10, 11}; return t[j]; }
+    KB_HD static constexpr int minus(int j) { constexpr int t[6] = {6, 7, 8, 12, 13, 15}; return t[j]; }
+    KB_HD static constexpr int single(int j) { constexpr int t[3] = {1, 2, 14}; return t[j]; }
+};
+// width 24: pairs (3,6) 1/2, (4,7) 3, (5,8) 4, (9,16) 2^-8, (11,17) 1/8, (12,18) 1/16,
+// (13,19) 1/32, (14,20) 1/64, (15,23) 2^-24; singles 1, 2, 1/4, -2^-7, -2^-9
+template <>
+struct DiagIdx<24> {
+    static constexpr int NP = 9, NS = 5;
+    KB_HD static constexpr int plus(int j) { constexpr int t[9] = {3, 4, 5, 9, 11, 12, 13, 14, 15}; return t[j]; }
+    KB_HD static constexpr int minus(int j) { constexpr int t[9] = {6, 7, 8, 16, 17, 18, 19, 20, 23}; return t[j]; }
+    KB_HD static constexpr int single(int j) { constexpr int t[5] = {1, 2, 10, 21, 22}; return t[j]; }
+};
 template <int W, int RP>
 struct PartialConv {
     uint32_t dpow[RP][W - 1];  // d_i^k (Montgomery) at [k - 1][i - 1], k = 1..RP
     uint32_t csum[RP];         // C(k) (Montgomery) at [k - 1], k = 1..RP - 1
+    uint32_t hpow[W];          // d^RP / 2 (Montgomery) of pair j (d = the entry of DiagIdx::plus(j))
     uint32_t wm1;              // W - 1 (Montgomery)
 };
 // sum_j x_j c_j for canonical x_j and Montgomery c_j: products below P^2, four of
@@ -497,10 +526,17 @@ struct Lazy64 {
 };
 template <int W, int RP, typename C, typename PC>
 KB_HD void p2_partial_conv(F *s, const C &k, const PC &pc) {
+    using D = DiagIdx<W>;
     uint64_t r64 = 0;
 #pragma unroll
     for (int i = 1; i < W; i++) r64 += s[i].v;
     F R{fred37(r64)};
+    F pa[D::NP], ma[D::NP];  // P = s_+ + s_-, M = s_+ - s_- of each pair, canonical
+#pragma unroll
+    for (int j = 0; j < D::NP; j++) {
+        pa[j] = fadd(s[D::plus(j)], s[D::minus(j)]);
+        ma[j] = fsub(s[D::plus(j)], s[D::minus(j)]);
+    }
     F S[RP];
     F x0 = s[0];
 #pragma unroll
@@ -509,11 +545,13 @@ KB_HD void p2_partial_conv(F *s, const C &k, const PC &pc) {
         S[t] = fadd(y, R);
         x0 = fsub(S[t], fdbl(y));
         if (t + 1 < RP) {
-            // A(t+1) from the entering state (kept in s[1..W)) and the convolution
-            // of the earlier sums, in one dot product
+            // A(t+1) from the entering state and the convolution of the earlier
+            // sums, in one dot product
             Lazy64 a;
 #pragma unroll
-            for (int i = 1; i < W; i++) a.add(s[i].v, pc.dpow[t][i - 1]);
+            for (int j = 0; j < D::NP; j++) a.add(((t + 1) & 1) ? ma[j].v : pa[j].v, pc.dpow[t][D::plus(j) - 1]);
+#pragma unroll
+            for (int j = 0; j < D::NS; j++) a.add(s[D::single(j)].v, pc.dpow[t][D::single(j) - 1]);
             a.add(S[t].v, pc.wm1);
 #pragma unroll
             for (int u = 0; u < t; u++) a.add(S[u].v, pc.csum[t - u - 1]);
@@ -522,12 +560,29 @@ KB_HD void p2_partial_conv(F *s, const C &k, const PC &pc) {
     }
     s[0] = x0;
 #pragma unroll
-    for (int i = 1; i < W; i++) {
+    for (int j = 0; j < D::NS; j++) {
+        const int i = D::single(j);
         Lazy64 a;
         a.add(s[i].v, pc.dpow[RP - 1][i - 1]);
 #pragma unroll
         for (int u = 0; u + 1 < RP; u++) a.add(S[u].v, pc.dpow[RP - 2 - u][i - 1]);
         s[i] = fadd(a.get(), S[RP - 1]);
+    }
+#pragma unroll
+    for (int j = 0; j < D::NP; j++) {
+        const int ip = D::plus(j);
+        Lazy64 e, o;
+#pragma unroll
+        for (int u = 0; u + 1 < RP; u++) {
+            const int ex = RP - 1 - u;
+            if (ex & 1) o.add(S[u].v, pc.dpow[ex - 1][ip - 1]);
+            else e.add(S[u].v, pc.dpow[ex - 1][ip - 1]);
+        }
+        e.add((RP & 1) ? ma[j].v : pa[j].v, pc.hpow[j]);
+        o.add((RP & 1) ? pa[j].v : ma[j].v, pc.hpow[j]);
+        const F ev = e.get(), ov = o.get();
+        s[ip] = fadd(fadd(ev, ov), S[RP - 1]);
+        s[D::minus(j)] = fadd(fsub(ev, ov), S[RP - 1]);
     }
 }
 // Poseidon2 with the partial rounds by convolution (same permutation as p2_permute)

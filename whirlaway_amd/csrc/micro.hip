@@ -51,9 +51,52 @@ __global__ void __launch_bounds__(256) k_fmul_probe(uint32_t *out, int iters) {
     out[g] = x.v ^ y.v ^ z.v ^ w.v;
 }
 
+// Host-side check of the Poseidon2 partial rounds by convolution against the
+// round-by-round form, for one width: random round constants (xorshift from
+// `seed`), `n` random states plus field-extreme states.  Returns the number of
+// states whose permutations differ.
+template <int W, int RP, typename C>
+uint64_t conv_mismatches(const PartialConv<W, RP> &pc, const uint32_t *diag, uint32_t n, uint64_t &rng) {
+    auto next = [&]() {
+        rng ^= rng << 13, rng ^= rng >> 7, rng ^= rng << 17;
+        return (uint32_t)(rng % P);
+    };
+    C k;
+    for (int r = 0; r < 4; r++)
+        for (int i = 0; i < W; i++) k.ext_init[r][i] = to_mont(next()), k.ext_term[r][i] = to_mont(next());
+    for (int r = 0; r < RP; r++) k.internal[r] = to_mont(next());
+    const uint32_t extremes[] = {0u, 1u, P - 1, P - 2, P / 2, (P + 1) / 2, 1u << 24, (1u << 24) - 1, 1u << 30};
+    const uint32_t n_ext = sizeof(extremes) / sizeof(extremes[0]);
+    uint64_t bad = 0;
+    for (uint32_t st = 0; st < n + n_ext * 3; st++) {
+        F a[W], b[W];
+        for (int i = 0; i < W; i++) {
+            uint32_t v;
+            if (st < n) v = next();
+            else if (st < n + n_ext) v = extremes[st - n];  // constant state
+            else if (st < n + 2 * n_ext) v = i & 1 ? extremes[st - n - n_ext] : 0u;
+            else v = (i == (int)(st % W)) ? extremes[st - n - 2 * n_ext] : P - 1;
+            a[i] = b[i] = to_mont(v);
+        }
+        p2_permute<W, RP>(a, k, diag);
+        p2_permute_conv<W, RP>(b, k, pc);
+        for (int i = 0; i < W; i++)
+            if (a[i].v != b[i].v) { bad++; break; }
+    }
+    return bad;
+}
+
 }  // namespace
 
 extern "C" {
+
+int wl_p2_conv_check(uint32_t n_states, uint64_t seed, uint64_t *mismatches) {
+    if (!mismatches) { wl_set_error("null argument"); return WL_ERR_ARG; }
+    uint64_t rng = seed * 0x9e3779b97f4a7c15ull + 1;
+    mismatches[0] = conv_mismatches<16, 20, P2Consts16>(PCONV16, DIAG.d16, n_states, rng);
+    mismatches[1] = conv_mismatches<24, 23, P2Consts24>(PCONV24, DIAG.d24, n_states, rng);
+    return WL_OK;
+}
 
 int wl_fmul_rate(wl_ctx *c, double *fmul_per_s) {
     if (!fmul_per_s) { wl_set_error("null argument"); return WL_ERR_ARG; }
