@@ -43,7 +43,7 @@ MEASURED_TRAFFIC_N17 = {
 # leaf class's live permutation rate times this count, over the VALU issue rate of
 # 1024 SIMDs x 2.4 GHz at 4 cycles per wave64 instruction, is `valu_issue`: the share of
 # the issue rate the leaf hashing uses (the F-mul `frac` counts only its products).
-VALU_INSTR_PER_P24 = 7006
+VALU_INSTR_PER_P24 = 6438
 PHASES = ["commit", "zerocheck_round0", "zerocheck_rounds", "batching", "inner_sumcheck", "whir_open",
           "host_wait_for_device"]
 MODELS = {
