@@ -186,6 +186,29 @@ def test_proof_settings_variants(gpu_ctx, oracle, n, kw):
     assert b.verify(got, **kw) == 0
 
 
+# Rate 1/8 with a first folding factor of 4: the first codeword has
+# 2^(n + 8 + 3 - 4) rows, 2^21 at n = 14 and 2^22 at n = 15, beyond the two-pass
+# register NTT, so these proofs take the three-pass encoding.
+@pytest.mark.parametrize("n,kw", [(14, dict(sec=90, f0=4, f=2, rate=3, skips=4, rs0=2)),
+                                  (15, dict(sec=100, f0=4, f=4, rate=3, skips=4, rs0=3))])
+def test_proof_three_pass_encoding(gpu_ctx, oracle, n, kw):
+    import whirlaway_amd as wa
+    _say(f"full proof n={n} settings {kw} (first codeword beyond 2^20 rows)")
+    b = oracle.Bench(n, seed=5)
+    st = wa.AirSettings(security_bits=kw["sec"], folding_factor_first=kw["f0"], folding_factor=kw["f"],
+                        log_inv_rate=kw["rate"], univariate_skips=kw["skips"],
+                        rs_domain_initial_reduction=kw["rs0"])
+    table = wa.AirTable(gpu_ctx, b.air_consts(), n, univariate_skips=kw["skips"])
+    got = table.prove(st, b.p2_consts(16), b.p2_consts(24), b.trace())
+    table.close()
+    _say(f"  oracle proof n={n}")
+    want = b.prove(**kw)
+    assert got.size == want.size
+    diff = np.nonzero(got != want)[0]
+    assert diff.size == 0, f"first mismatch at {diff[:5]}"
+    assert b.verify(got, **kw) == 0
+
+
 @pytest.mark.parametrize("n,kw", [(8, {}), (12, {}), SETTINGS_VARIANTS[1]])
 def test_device_transcript_rounds(oracle, monkeypatch, n, kw):
     """Zerocheck rounds whose transcript runs on the device (one wave's duplex
