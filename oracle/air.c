@@ -70,6 +70,7 @@ static void matrix_down_folded(const ef_t *p, int n, ef_t *out) {
         for (int i = n - k; i < n; i++) prod = ef_mul(prod, p[i]);
         or_eval_eq_scaled(p, n - k - 1, prod, eq);
         size_t M = (size_t)1 << (n - k - 1);
+#pragma omp parallel for schedule(static) if (M >= 4096)
         for (size_t i = 0; i < M; i++) {
             size_t idx = (i << (k + 1)) + ((size_t)1 << k);
             out[idx] = ef_add(out[idx], eq[i]);
@@ -194,6 +195,7 @@ void or_air_prove_ts(const or_air_settings_t *s, const or_p2_bench_t *b, or_ts_t
     }
     ef_t alpha2 = or_ts_sample_ef(ts);
     ef_t *mixed = (ef_t *)malloc(N * sizeof(ef_t));
+#pragma omp parallel for schedule(static)
     for (size_t x = 0; x < N; x++) {
         ef_t up = x == N - 1 ? batched[N - 2] : batched[x];
         ef_t dn2 = x == N - 1 ? batched[N - 1] : batched[x + 1];
@@ -205,6 +207,7 @@ void or_air_prove_ts(const or_air_settings_t *s, const or_p2_bench_t *b, or_ts_t
     ef_t *eqr = (ef_t *)malloc(M * sizeof(ef_t));
     or_eval_eq(ch + 1, n - skips, eqr);
     ef_t sub[64];
+#pragma omp parallel for schedule(dynamic, 1)
     for (int j = 0; j < K; j++) {
         ef_t acc = ef_zero();
         for (size_t i = 0; i < M; i++) acc = ef_add(acc, ef_mul(mixed[(size_t)j * M + i], eqr[i]));
@@ -219,6 +222,7 @@ void or_air_prove_ts(const or_air_settings_t *s, const or_p2_bench_t *b, or_ts_t
     ef_t *md = (ef_t *)malloc(N * sizeof(ef_t));
     matrix_up_folded(pt, n, mu);
     matrix_down_folded(pt, n, md);
+#pragma omp parallel for schedule(static)
     for (size_t x = 0; x < N; x++) mu[x] = ef_add(mu[x], ef_mul(md[x], alpha2));
     free(md);
     ef_t inner_sum = or_mle_eval_ef(mixed, n, pt);
@@ -401,6 +405,7 @@ void or_gair_prove(const or_air_settings_t *s, const or_gair_t *a, const f_t *wi
     }
     ef_t alpha2 = or_ts_sample_ef(ts);
     ef_t *mixed = (ef_t *)malloc(N * sizeof(ef_t));
+#pragma omp parallel for schedule(static)
     for (size_t x = 0; x < N; x++) {
         ef_t up = x == N - 1 ? batched[N - 2] : batched[x];
         ef_t dn = x == N - 1 ? batched[N - 1] : batched[x + 1];
@@ -411,6 +416,7 @@ void or_gair_prove(const or_air_settings_t *s, const or_gair_t *a, const f_t *wi
     ef_t *eqr = (ef_t *)malloc(M * sizeof(ef_t));
     or_eval_eq(ch + 1, n - skips, eqr);
     ef_t sub[64];
+#pragma omp parallel for schedule(dynamic, 1)
     for (int j = 0; j < K; j++) {
         ef_t acc = ef_zero();
         for (size_t i = 0; i < M; i++) acc = ef_add(acc, ef_mul(mixed[(size_t)j * M + i], eqr[i]));
@@ -425,6 +431,7 @@ void or_gair_prove(const or_air_settings_t *s, const or_gair_t *a, const f_t *wi
     ef_t *md = (ef_t *)malloc(N * sizeof(ef_t));
     matrix_up_folded(pt, n, mu);
     matrix_down_folded(pt, n, md);
+#pragma omp parallel for schedule(static)
     for (size_t x = 0; x < N; x++) mu[x] = ef_add(mu[x], ef_mul(md[x], alpha2));
     free(md);
     ef_t inner_sum = or_mle_eval_ef(mixed, n, pt);

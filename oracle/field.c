@@ -60,16 +60,20 @@ ef_t ef_neg(ef_t a) {
 }
 
 /* Schoolbook product reduced by X^8 = W. */
+/* Schoolbook product, X^8 = W.  Each product of canonical values is below
+ * p^2 < 2^62, so a 64-bit sum holds four of them: the coefficient sums are kept in
+ * two halves by the first factor's index (i < 4, i >= 4), each reduced once. */
 ef_t ef_mul(ef_t a, ef_t b) {
-    unsigned __int128 lo[8] = {0}, hi[8] = {0};
+    uint64_t lo[2][8] = {{0}}, hi[2][8] = {{0}};
     for (int i = 0; i < 8; i++)
         for (int j = 0; j < 8; j++) {
             uint64_t p = (uint64_t)a.c[i] * b.c[j];
-            if (i + j < 8) lo[i + j] += p; else hi[i + j - 8] += p;
+            if (i + j < 8) lo[i >> 2][i + j] += p; else hi[i >> 2][i + j - 8] += p;
         }
     ef_t r;
     for (int k = 0; k < 8; k++) {
-        f_t l = (f_t)(lo[k] % OR_P), h = (f_t)(hi[k] % OR_P);
+        f_t l = f_add((f_t)(lo[0][k] % OR_P), (f_t)(lo[1][k] % OR_P));
+        f_t h = f_add((f_t)(hi[0][k] % OR_P), (f_t)(hi[1][k] % OR_P));
         r.c[k] = f_add(l, f_mul(h, OR_EF_W));
     }
     return r;

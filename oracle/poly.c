@@ -21,30 +21,60 @@ size_t or_log2_up(size_t x) {
     return r;
 }
 
+/* Levels of 2^12 entries and more read the previous level from a copy so that
+ * their entries can be computed in parallel; smaller ones double in place, from
+ * the top entry down.  Both give the same table. */
+#define OR_PAR_MIN 4096
 void or_eval_eq_scaled(const ef_t *p, int n, ef_t scale, ef_t *out) {
     out[0] = scale;
+    ef_t *prev = NULL;
     for (int k = 0; k < n; k++) {
         size_t half = (size_t)1 << k;
-        for (size_t j = half; j-- > 0;) {
-            ef_t hi = ef_mul(out[j], p[k]);
-            out[2 * j] = ef_sub(out[j], hi);
+        if (half < OR_PAR_MIN) {
+            for (size_t j = half; j-- > 0;) {
+                ef_t hi = ef_mul(out[j], p[k]);
+                out[2 * j] = ef_sub(out[j], hi);
+                out[2 * j + 1] = hi;
+            }
+            continue;
+        }
+        prev = (ef_t *)realloc(prev, half * sizeof(ef_t));
+        memcpy(prev, out, half * sizeof(ef_t));
+#pragma omp parallel for schedule(static)
+        for (size_t j = 0; j < half; j++) {
+            ef_t hi = ef_mul(prev[j], p[k]);
+            out[2 * j] = ef_sub(prev[j], hi);
             out[2 * j + 1] = hi;
         }
     }
+    free(prev);
 }
 
 void or_eval_eq(const ef_t *p, int n, ef_t *out) { or_eval_eq_scaled(p, n, ef_one(), out); }
 
 void or_eval_eq_f(const f_t *p, int n, f_t *out) {
     out[0] = 1;
+    f_t *prev = NULL;
     for (int k = 0; k < n; k++) {
         size_t half = (size_t)1 << k;
-        for (size_t j = half; j-- > 0;) {
-            f_t hi = f_mul(out[j], p[k]);
-            out[2 * j] = f_sub(out[j], hi);
+        if (half < OR_PAR_MIN) {
+            for (size_t j = half; j-- > 0;) {
+                f_t hi = f_mul(out[j], p[k]);
+                out[2 * j] = f_sub(out[j], hi);
+                out[2 * j + 1] = hi;
+            }
+            continue;
+        }
+        prev = (f_t *)realloc(prev, half * sizeof(f_t));
+        memcpy(prev, out, half * sizeof(f_t));
+#pragma omp parallel for schedule(static)
+        for (size_t j = 0; j < half; j++) {
+            f_t hi = f_mul(prev[j], p[k]);
+            out[2 * j] = f_sub(prev[j], hi);
             out[2 * j + 1] = hi;
         }
     }
+    free(prev);
 }
 
 /* MultilinearPoint::eq_poly_outside: prod_i (a_i b_i + (1-a_i)(1-b_i)) */
@@ -64,6 +94,7 @@ ef_t or_mle_eval_ef(const ef_t *evals, int n, const ef_t *point) {
     memcpy(t, evals, N * sizeof(ef_t));
     for (int k = 0; k < n; k++) {
         size_t h = N >> (k + 1);
+#pragma omp parallel for schedule(static) if (h >= OR_PAR_MIN)
         for (size_t i = 0; i < h; i++) t[i] = ef_add(t[i], ef_mul(point[k], ef_sub(t[i + h], t[i])));
     }
     ef_t r = t[0];
@@ -76,10 +107,12 @@ ef_t or_mle_eval_f(const f_t *evals, int n, const ef_t *point) {
     if (n == 0) return ef_from_f(evals[0]);
     size_t h = N >> 1;
     ef_t *t = (ef_t *)malloc(h * sizeof(ef_t));
+#pragma omp parallel for schedule(static) if (h >= OR_PAR_MIN)
     for (size_t i = 0; i < h; i++)
         t[i] = ef_add(ef_from_f(evals[i]), ef_mul_f(point[0], f_sub(evals[i + h], evals[i])));
     for (int k = 1; k < n; k++) {
         size_t hh = N >> (k + 1);
+#pragma omp parallel for schedule(static) if (hh >= OR_PAR_MIN)
         for (size_t i = 0; i < hh; i++) t[i] = ef_add(t[i], ef_mul(point[k], ef_sub(t[i + hh], t[i])));
     }
     ef_t r = t[0];

@@ -53,11 +53,23 @@ ef_t or_chal_sample_ef(or_chal_t *c) {
     return r;
 }
 
+/* The smallest witness, searched in rounds of consecutive candidates (64, then
+ * four times more per round up to 2^16) whose members are tried in parallel: the
+ * first round with a hit holds the smallest one. */
 f_t or_grind_witness(const or_chal_t *c, int bits) {
-    for (f_t w = 0; w < OR_P; w++) {
-        or_chal_t t = *c;
-        or_chal_observe(&t, w);
-        if (or_chal_sample_bits(&t, bits) == 0) return w;
+    uint64_t base = 0, span = 64;
+    while (base < OR_P) {
+        uint64_t end = base + span < OR_P ? base + span : OR_P;
+        f_t best = OR_P;
+#pragma omp parallel for schedule(static) reduction(min : best) if (span > 64)
+        for (uint64_t w = base; w < end; w++) {
+            or_chal_t t = *c;
+            or_chal_observe(&t, (f_t)w);
+            if (or_chal_sample_bits(&t, bits) == 0 && (f_t)w < best) best = (f_t)w;
+        }
+        if (best < OR_P) return best;
+        base = end;
+        if (span < 65536) span *= 4;
     }
     abort();
 }
