@@ -27,6 +27,11 @@ struct Partition {
     int n_slots = 0;
     std::vector<uint32_t> off;
     std::vector<uint32_t> code;
+    // the same groups as bundles (k_eval_bundles): a header {op | count << 8, 0} and
+    // `count` (1..4) entries {dst, operands} of independent instructions of one op
+    int bn_slots = 0;
+    std::vector<uint32_t> boff;
+    std::vector<uint32_t> bcode;
 };
 struct Program {
     int n_inputs = 0;
@@ -36,6 +41,7 @@ struct Program {
     std::vector<Partition> parts;    // 1, 4 and 16 groups (as many as the constraints allow)
     wl::DBuf<uint32_t> dev_buf;      // every partition on the device: offsets, then code
     std::vector<size_t> part_base;   // word offset of each partition in dev_buf
+    std::vector<size_t> bpart_base;  // word offset of each partition's bundles: offsets, then code
 };
 
 // Validates the SSA program and compiles it; throws std::runtime_error on a

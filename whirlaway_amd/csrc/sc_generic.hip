@@ -12,6 +12,7 @@
 // challenge's scalars in one launch.  All arithmetic is exact in F_p, so the
 // block/tree order of the sums does not change the result.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <mutex>
 #include <type_traits>
 #include "../../include/whirlaway_hip.h"
@@ -23,6 +24,7 @@ using namespace wl;
 namespace wg {
 
 static inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+constexpr int BUNDLE_WORDS = 10;  // a bundle: 2 header words and up to 4 entries of 2
 
 __device__ __forceinline__ F d_add(F a, F b) { return fadd(a, b); }
 __device__ __forceinline__ EF d_add(const EF &a, const EF &b) { return eadd(a, b); }
@@ -283,6 +285,162 @@ __global__ void __launch_bounds__(256) k_eval_lanes(const EF *tables, size_t len
     }
 }
 
+// k_eval_lanes over bundles (Partition::bcode): each step runs up to four
+// independent instructions of one op, their operand reads first, then the
+// arithmetic (the products' DPP chains interleave), then the writes.  The next
+// bundle's words are fetched while the current one runs.
+template <int C>
+using BundleN = std::integral_constant<int, C>;
+__global__ void __launch_bounds__(256) k_eval_bundles(const EF *tables, size_t len, int k, const F *zsc,
+                                                      const uint32_t *bpart, int G, const EF *batching,
+                                                      const EF *eq, EF *parts) {
+    extern __shared__ uint32_t lregs[];
+    __shared__ uint32_t red[4][8];
+    const int t = threadIdx.x, co = t & 7, pt = t >> 4;
+    auto rd = [&](uint32_t slot) { return lregs[slot * 256 + t]; };
+    auto wr = [&](uint32_t slot, uint32_t v) { lregs[slot * 256 + t] = v; };
+    const int K = 1 << k;
+    const size_t half = len >> k;
+    const F *s = zsc + (size_t)blockIdx.y * K;
+    const uint32_t *prog = bpart + (G + 1) + bpart[blockIdx.z];
+    const int n_words = (int)(bpart[blockIdx.z + 1] - bpart[blockIdx.z]);
+    uint32_t total = 0;  // coefficient co of this lane's points' sum
+    const size_t stride = (size_t)gridDim.x * 16;
+    // every lane of the block runs the same number of iterations (DPP needs the row)
+    for (size_t x0 = (size_t)blockIdx.x * 16; x0 < half; x0 += stride) {
+        const size_t x = x0 + pt;
+        const bool live = x < half;
+        const size_t xl = live ? x : x0;
+        uint64_t acc = 0;
+        int n_acc = 0;
+        uint32_t nb[BUNDLE_WORDS];
+#pragma unroll
+        for (int i = 0; i < BUNDLE_WORDS; i++) nb[i] = prog[i];
+        for (int pc = 0; pc < n_words;) {
+            uint32_t e[BUNDLE_WORDS];
+#pragma unroll
+            for (int i = 0; i < BUNDLE_WORDS; i++) e[i] = nb[i];
+            const uint32_t op = e[0] & 0xff, cnt = e[0] >> 8;
+            const int next = pc + 2 + 2 * (int)cnt;
+#pragma unroll
+            for (int i = 0; i < BUNDLE_WORDS; i++) nb[i] = prog[next + i];
+            auto run = [&](auto fn) {
+                switch (cnt) {
+                case 1: fn(BundleN<1>{}); break;
+                case 2: fn(BundleN<2>{}); break;
+                case 3: fn(BundleN<3>{}); break;
+                default: fn(BundleN<4>{}); break;
+                }
+            };
+            switch (op) {
+            case OP_LOAD:
+                run([&](auto CN) {
+                    constexpr int NC = decltype(CN)::value;
+                    uint64_t f[NC];
+#pragma unroll
+                    for (int i = 0; i < NC; i++) f[i] = 0;
+                    for (int j = 0; j < K; j++) {
+#pragma unroll
+                        for (int i = 0; i < NC; i++) {
+                            const EF *tab = tables + (size_t)e[3 + 2 * i] * len;
+                            if (j && (j & 3) == 0) f[i] = fold64(f[i]);
+                            f[i] += (uint64_t)s[j].v * tab[(size_t)j * half + xl].c[co].v;
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < NC; i++) wr(e[2 + 2 * i], redc64(f[i]));
+                });
+                break;
+            case OP_CONST:
+                run([&](auto CN) {
+                    constexpr int NC = decltype(CN)::value;
+#pragma unroll
+                    for (int i = 0; i < NC; i++) wr(e[2 + 2 * i], co == 0 ? e[3 + 2 * i] : 0u);
+                });
+                break;
+            case OP_ADD:
+            case OP_SUB:
+                run([&](auto CN) {
+                    constexpr int NC = decltype(CN)::value;
+                    uint32_t a[NC], b[NC];
+#pragma unroll
+                    for (int i = 0; i < NC; i++) a[i] = rd(e[3 + 2 * i] & 0xffff), b[i] = rd(e[3 + 2 * i] >> 16);
+#pragma unroll
+                    for (int i = 0; i < NC; i++)
+                        wr(e[2 + 2 * i], op == OP_ADD ? fadd(F{a[i]}, F{b[i]}).v : fsub(F{a[i]}, F{b[i]}).v);
+                });
+                break;
+            case OP_NEG:
+                run([&](auto CN) {
+                    constexpr int NC = decltype(CN)::value;
+                    uint32_t a[NC];
+#pragma unroll
+                    for (int i = 0; i < NC; i++) a[i] = rd(e[3 + 2 * i] & 0xffff);
+#pragma unroll
+                    for (int i = 0; i < NC; i++) wr(e[2 + 2 * i], fneg(F{a[i]}).v);
+                });
+                break;
+            case OP_MUL:
+                run([&](auto CN) {
+                    constexpr int NC = decltype(CN)::value;
+                    uint32_t a[NC], b[NC], r[NC];
+#pragma unroll
+                    for (int i = 0; i < NC; i++) a[i] = rd(e[3 + 2 * i] & 0xffff), b[i] = rd(e[3 + 2 * i] >> 16);
+#pragma unroll
+                    for (int i = 0; i < NC; i++) r[i] = lane_mul(a[i], b[i], co);
+#pragma unroll
+                    for (int i = 0; i < NC; i++) wr(e[2 + 2 * i], r[i]);
+                });
+                break;
+            case OP_MULC:
+                run([&](auto CN) {
+                    constexpr int NC = decltype(CN)::value;
+                    uint32_t a[NC], c[NC];
+#pragma unroll
+                    for (int i = 0; i < NC; i++) a[i] = rd(e[3 + 2 * i] & 0xffff), c[i] = rd(e[3 + 2 * i] >> 16);
+#pragma unroll
+                    for (int i = 0; i < NC; i++) wr(e[2 + 2 * i], fmul(F{a[i]}, F{dpp_bcast<0>(c[i])}).v);
+                });
+                break;
+            default:  // OP_ASSERT: batching[rank] * value into the lane's lazy sum
+                run([&](auto CN) {
+                    constexpr int NC = decltype(CN)::value;
+                    uint32_t a[NC];
+                    EF B[NC];
+#pragma unroll
+                    for (int i = 0; i < NC; i++) a[i] = rd(e[3 + 2 * i]), B[i] = batching[e[2 + 2 * i]];
+#pragma unroll
+                    for (int i = 0; i < NC; i++) {
+                        uint32_t B3[8];
+#pragma unroll
+                        for (int q = 0; q < 8; q++) B3[q] = ftimes3(B[i].c[q]).v;
+                        if (n_acc == 64) { acc = fold64(acc); n_acc = 0; }
+                        uint64_t prod = 0;
+                        lane_mul_const_acc(prod, a[i], B[i], B3, co);
+                        acc += fold64(prod);
+                        n_acc++;
+                    }
+                });
+            }
+            pc = next;
+        }
+        uint32_t v = redc64(acc);
+        if (eq) v = lane_mul(v, eq[xl].c[co].v, co);
+        if (live) total = fadd(F{total}, F{v}).v;
+    }
+    // sum over the block's 16 points: the four rows of a wave, then the four waves
+    total = fadd(F{total}, F{(uint32_t)__shfl_xor((int)total, 16, 64)}).v;
+    total = fadd(F{total}, F{(uint32_t)__shfl_xor((int)total, 32, 64)}).v;
+    if ((t & 63) < 8) red[t >> 6][t & 7] = total;
+    __syncthreads();
+    if (t < 8) {
+        uint32_t r = red[0][t];
+        for (int w = 1; w < 4; w++) r = fadd(F{r}, F{red[w][t]}).v;
+        EF *o = parts + ((size_t)blockIdx.y * G + blockIdx.z) * gridDim.x + blockIdx.x;
+        o->c[t] = F{r};
+    }
+}
+
 // out[m][i] = sum_j sc[j] in[m][j M + i], M = len / K, for every table m
 template <class T>
 __global__ void k_fold_tables(const T *in, size_t len, int K, const EF *sc, EF *out, size_t total) {
@@ -378,8 +536,148 @@ int emit_slots(const uint32_t *code, size_t n_instr, int n_inputs, const std::ve
     return std::max(1, n_slots);
 }
 
+// The instructions j with keep[j] set as bundles for k_eval_bundles.  A list
+// schedule over the dependence graph (input loads are nodes too, placed at their
+// first user): each step takes the ready node first in program order and up to
+// three more ready nodes of the same op, earliest first, so that independent work
+// (the products of an internal layer, the additions of a balanced sum, the loads
+// of a round's inputs) runs in one dispatch with its memory and LDS latencies
+// overlapped, while values stay live about as long as in program order.  Slots are assigned over the bundle order: a
+// bundle reads all its operands before it writes, so a result may take the slot an
+// operand of the same bundle frees.  Returns the register-file size.
+int emit_bundles(const uint32_t *code, size_t n_instr, int n_inputs, const std::vector<char> &keep,
+                 const std::vector<int> &assert_rank, std::vector<uint32_t> &dev) {
+    const size_t NN = n_instr + (size_t)n_inputs;  // node n_instr + v loads input v
+    auto node_of = [&](uint32_t v) { return v < (uint32_t)n_inputs ? n_instr + v : (size_t)(v - n_inputs); };
+    auto is_const = [&](uint32_t v) { return v >= (uint32_t)n_inputs && code[4 * (v - n_inputs)] == OP_CONST; };
+    // operands as nodes (a repeated operand once)
+    auto operands = [&](size_t j, size_t *o) {
+        const uint32_t *w = code + 4 * j;
+        const int nops = n_operands(w[0]);
+        int n = 0;
+        for (int q = 0; q < nops; q++) {
+            const size_t u = node_of(w[1 + q]);
+            if (n == 0 || o[0] != u) o[n++] = u;
+        }
+        return n;
+    };
+    // live nodes: the kept instructions on a path to an ASSERT, and the inputs they read
+    std::vector<char> live(NN, 0);
+    for (size_t j = n_instr; j-- > 0;) {
+        if (!keep[j]) continue;
+        if (code[4 * j] == OP_ASSERT) live[j] = 1;
+        if (!live[j]) continue;
+        size_t o[2];
+        const int n = operands(j, o);
+        for (int q = 0; q < n; q++) live[o[q]] = 1;
+    }
+    std::vector<std::vector<uint32_t>> users(NN);
+    std::vector<int> ndeps(NN, 0);
+    for (size_t j = 0; j < n_instr; j++) {
+        if (!live[j]) continue;
+        size_t o[2];
+        const int n = operands(j, o);
+        for (int q = 0; q < n; q++) users[o[q]].push_back((uint32_t)j);
+        ndeps[j] = n;
+    }
+    auto kind = [&](size_t u) -> uint32_t {
+        if (u >= n_instr) return OP_LOAD;
+        const uint32_t *w = code + 4 * u;
+        if (w[0] == OP_MUL && (is_const(w[1]) || is_const(w[2]))) return OP_MULC;
+        return w[0];
+    };
+    // program position: an instruction's index, an input load's first user
+    std::vector<long> pos(NN, 0);
+    for (size_t u = 0; u < NN; u++) {
+        if (u < n_instr) pos[u] = (long)u;
+        else pos[u] = users[u].empty() ? 0 : (long)users[u].front();
+    }
+    // list schedule
+    std::vector<std::vector<uint32_t>> ready(8);
+    for (size_t u = 0; u < NN; u++)
+        if (live[u] && ndeps[u] == 0) ready[kind(u)].push_back((uint32_t)u);
+    auto better = [&](uint32_t a, uint32_t b) { return pos[a] != pos[b] ? pos[a] < pos[b] : a < b; };
+    std::vector<std::vector<uint32_t>> bundles;
+    std::vector<uint32_t> bundle_op;
+    for (;;) {
+        int best_k = -1;
+        uint32_t best = 0;
+        for (int k = 0; k < 8; k++)
+            for (uint32_t u : ready[k])
+                if (best_k < 0 || better(u, best)) best_k = k, best = u;
+        if (best_k < 0) break;
+        std::vector<uint32_t> &r = ready[best_k];
+        std::sort(r.begin(), r.end(), better);
+        const size_t cnt = std::min<size_t>(4, r.size());
+        std::vector<uint32_t> b(r.begin(), r.begin() + cnt);
+        r.erase(r.begin(), r.begin() + cnt);
+        for (uint32_t u : b)
+            for (uint32_t v : users[u])
+                if (--ndeps[v] == 0) ready[kind(v)].push_back(v);
+        bundles.push_back(std::move(b));
+        bundle_op.push_back((uint32_t)best_k);
+    }
+    // slots over the bundle order
+    std::vector<long> last(NN, -1);
+    for (size_t bi = 0; bi < bundles.size(); bi++)
+        for (uint32_t u : bundles[bi]) {
+            size_t o[2];
+            const int n = u < n_instr ? operands(u, o) : 0;
+            for (int q = 0; q < n; q++) last[o[q]] = (long)bi;
+        }
+    std::vector<int> slot(NN, -1), free_slots;
+    int n_slots = 0;
+    auto take = [&]() {
+        if (!free_slots.empty()) { int s2 = free_slots.back(); free_slots.pop_back(); return s2; }
+        return n_slots++;
+    };
+    for (size_t bi = 0; bi < bundles.size(); bi++) {
+        const std::vector<uint32_t> &b = bundles[bi];
+        const uint32_t op = bundle_op[bi];
+        // operand slots, read before the bundle frees or writes any
+        std::vector<uint32_t> w1(b.size(), 0);
+        for (size_t i = 0; i < b.size(); i++) {
+            const size_t u = b[i];
+            if (u >= n_instr) { w1[i] = (uint32_t)(u - n_instr); continue; }
+            const uint32_t *w = code + 4 * u;
+            if (op == OP_CONST) w1[i] = to_mont(w[1]).v;
+            else if (op == OP_MULC) {
+                const bool c2 = is_const(w[2]);
+                const uint32_t xs = (uint32_t)slot[node_of(c2 ? w[1] : w[2])];
+                const uint32_t cs = (uint32_t)slot[node_of(c2 ? w[2] : w[1])];
+                w1[i] = xs | (cs << 16);
+            } else {
+                const int nops = n_operands(w[0]);
+                const uint32_t a = nops >= 1 ? (uint32_t)slot[node_of(w[1])] : 0;
+                const uint32_t bb = nops >= 2 ? (uint32_t)slot[node_of(w[2])] : 0;
+                w1[i] = a | (bb << 16);
+            }
+        }
+        for (size_t i = 0; i < b.size(); i++) {
+            size_t o[2];
+            const int n = b[i] < n_instr ? operands(b[i], o) : 0;
+            for (int q = 0; q < n; q++)
+                if (last[o[q]] == (long)bi && slot[o[q]] >= 0) {
+                    free_slots.push_back(slot[o[q]]);
+                    slot[o[q]] = -2;
+                }
+        }
+        dev.push_back(op | ((uint32_t)b.size() << 8));
+        dev.push_back(0);
+        for (size_t i = 0; i < b.size(); i++) {
+            const size_t u = b[i];
+            uint32_t w0;
+            if (op == OP_ASSERT) w0 = (uint32_t)assert_rank[u];
+            else w0 = (uint32_t)(slot[u] = take());
+            dev.push_back(w0);
+            dev.push_back(w1[i]);
+        }
+    }
+    return std::max(1, n_slots);
+}
+
 // One partition (see compile): false when the program has too few constraints for G groups.
-bool build_partition(const uint32_t *code, size_t n_instr, int n_inputs, int G, Program &out) {
+bool build_partition(const uint32_t *code, size_t n_instr, int n_inputs, int G, bool bundles, Program &out) {
     std::vector<char> is_assert(n_instr, 0);
     std::vector<int> assert_rank(n_instr, -1);
     int n_asserts = 0;
@@ -411,6 +709,10 @@ bool build_partition(const uint32_t *code, size_t n_instr, int n_inputs, int G, 
         part.off.push_back((uint32_t)part.code.size());
         const int ns = emit_slots(code, n_instr, n_inputs, keep, assert_rank, part.code);
         part.n_slots = std::max(part.n_slots, ns);
+        if (bundles) {
+            part.boff.push_back((uint32_t)part.bcode.size());
+            part.bn_slots = std::max(part.bn_slots, emit_bundles(code, n_instr, n_inputs, keep, assert_rank, part.bcode));
+        }
         part.groups++;
         std::fill(keep.begin(), keep.end(), 0);
         group_w = 0;
@@ -433,9 +735,73 @@ bool build_partition(const uint32_t *code, size_t n_instr, int n_inputs, int G, 
     }
     if (part.groups == 0 || std::count(keep.begin(), keep.end(), 1) > 0) close_group();
     part.off.push_back((uint32_t)part.code.size());
+    if (bundles) part.boff.push_back((uint32_t)part.bcode.size());
     if (part.n_slots > 256) throw std::runtime_error("program keeps more than 256 values live");
     out.parts.push_back(std::move(part));
     return true;
+}
+
+// Sums written as chains of additions (s = s + x_i) rebalanced into trees: an ADD
+// whose only use is another ADD is folded into that ADD's sum, and each sum of
+// L >= 3 terms is emitted as L - 1 additions pairing the terms level by level.
+// Field addition is exact, so every value the program defines is unchanged; the
+// dependent depth of an L-term sum falls from L - 1 to about log2 L, which lets
+// the bundled interpreter run its additions side by side.
+std::vector<uint32_t> rebalance_sums(const uint32_t *code, size_t n, int n_inputs) {
+    std::vector<int> uses(n, 0);
+    std::vector<long> user(n, -1);
+    for (size_t j = 0; j < n; j++) {
+        const uint32_t *w = code + 4 * j;
+        for (int o = 0; o < n_operands(w[0]); o++)
+            if (w[1 + o] >= (uint32_t)n_inputs) {
+                const size_t d = w[1 + o] - n_inputs;
+                uses[d]++;
+                user[d] = (long)j;
+            }
+    }
+    std::vector<char> inner(n, 0);
+    for (size_t j = 0; j < n; j++)
+        inner[j] = code[4 * j] == OP_ADD && uses[j] == 1 && code[4 * user[j]] == OP_ADD;
+    std::vector<uint32_t> out, map(n_inputs + n, 0);
+    for (int v = 0; v < n_inputs; v++) map[v] = (uint32_t)v;
+    auto emit = [&](uint32_t op, uint32_t a, uint32_t b) {
+        out.push_back(op), out.push_back(a), out.push_back(b), out.push_back(0);
+        return (uint32_t)(n_inputs + out.size() / 4 - 1);
+    };
+    std::vector<uint32_t> stack, leaves, level, next;
+    for (size_t j = 0; j < n; j++) {
+        const uint32_t *w = code + 4 * j;
+        if (inner[j]) continue;  // part of the sum of its (later) user
+        if (w[0] != OP_ADD) {
+            const int nops = n_operands(w[0]);
+            const uint32_t a = nops >= 1 ? map[w[1]] : w[1], b = nops >= 2 ? map[w[2]] : w[2];
+            map[n_inputs + j] = emit(w[0], a, b);
+            continue;
+        }
+        // the terms of this sum, left to right
+        leaves.clear();
+        stack.assign({w[2], w[1]});
+        while (!stack.empty()) {
+            const uint32_t v = stack.back();
+            stack.pop_back();
+            if (v >= (uint32_t)n_inputs && inner[v - n_inputs]) {
+                const uint32_t *u = code + 4 * (v - n_inputs);
+                stack.push_back(u[2]);
+                stack.push_back(u[1]);
+            } else {
+                leaves.push_back(map[v]);
+            }
+        }
+        level = leaves;
+        while (level.size() > 1) {
+            next.clear();
+            for (size_t i = 0; i + 1 < level.size(); i += 2) next.push_back(emit(OP_ADD, level[i], level[i + 1]));
+            if (level.size() & 1) next.push_back(level.back());
+            level.swap(next);
+        }
+        map[n_inputs + j] = level[0];
+    }
+    return out;
 }
 
 }  // namespace
@@ -465,13 +831,27 @@ void compile(const uint32_t *code_in, size_t n_instr_in, int n_inputs, Program &
     }
     out.n_inputs = n_inputs;
     out.parts.clear();
+    // the bundled code (extension-field rounds) from the program with its sums
+    // rebalanced; the one-instruction code from the program as written, whose
+    // long-lived balanced-sum terms measured slower in the base-field round
+    const std::vector<uint32_t> bal = rebalance_sums(code_in, n_instr_in, n_inputs);
+    Program bprog;
     // Partitions of the constraints into G groups (G = 1, 4, 16): consecutive
     // ASSERTs, each group the union of its constraints' dependency cones (shared
     // subexpressions are computed in every group that needs them), closed once it
     // holds its share of the program's weighted instructions (a product weighs 8).
     // Rounds with few points evaluate a point's constraints in G threads at once.
-    for (int G : {1, 4, 16})
-        if (!build_partition(code_in, n_instr_in, n_inputs, G, out)) break;
+    for (int G : {1, 4, 16}) {
+        if (!build_partition(code_in, n_instr_in, n_inputs, G, false, out)) break;
+        if (!build_partition(bal.data(), bal.size() / 4, n_inputs, G, true, bprog))
+            throw std::runtime_error("program partition differs after rebalancing its sums");
+        Partition &q = out.parts.back();
+        Partition &b = bprog.parts.back();
+        if (b.groups != q.groups) throw std::runtime_error("program partition differs after rebalancing its sums");
+        q.bn_slots = b.bn_slots;
+        q.boff = std::move(b.boff);
+        q.bcode = std::move(b.bcode);
+    }
     out.n_slots = out.parts[0].n_slots;
     out.dev = out.parts[0].code;
 }
@@ -486,6 +866,13 @@ void upload(wl_ctx *c, Program &p) {
     }
     all.push_back(0);  // padding read by the last instruction's prefetch
     all.push_back(0);
+    p.bpart_base.clear();
+    for (const Partition &q : p.parts) {
+        p.bpart_base.push_back(all.size());
+        all.insert(all.end(), q.boff.begin(), q.boff.end());
+        all.insert(all.end(), q.bcode.begin(), q.bcode.end());
+        all.insert(all.end(), BUNDLE_WORDS, 0u);  // read by the last bundle's prefetch
+    }
     p.dev_buf = DBuf<uint32_t>(c->pool, all.size());
     c->h2d(p.dev_buf.p, all.data(), all.size());
 }
@@ -496,7 +883,7 @@ void upload(wl_ctx *c, Program &p) {
 // round's threads (points x evaluation points) closest to 2^17 from below, or the
 // finest one; the register file in LDS when the grid of 64-thread blocks is at most
 // one block per CU and the slots fit, else in scratch with 128-thread blocks.
-enum EvalMode { EVAL_SCRATCH, EVAL_LDS, EVAL_LANES };
+enum EvalMode { EVAL_SCRATCH, EVAL_LDS, EVAL_LANES, EVAL_BUNDLES };
 struct EvalLaunch {
     size_t pi;
     EvalMode mode;
@@ -512,10 +899,12 @@ EvalLaunch plan_eval(const Program &p, size_t len, int k, int nz) {
         size_t pi = 0;
         while (pi + 1 < p.parts.size() && half * (size_t)nz * (size_t)p.parts[pi].groups * 16 < ((size_t)1 << 17))
             pi++;
+        const size_t bundle_bytes = (size_t)p.parts[pi].bn_slots * 256 * 4;
         e.lds_bytes = (size_t)p.parts[pi].n_slots * 256 * 4;
-        if (e.lds_bytes <= EVAL_LDS_MAX) {
+        if (bundle_bytes <= EVAL_LDS_MAX || e.lds_bytes <= EVAL_LDS_MAX) {
             e.pi = pi;
-            e.mode = EVAL_LANES;
+            e.mode = bundle_bytes <= EVAL_LDS_MAX ? EVAL_BUNDLES : EVAL_LANES;
+            if (e.mode == EVAL_BUNDLES) e.lds_bytes = bundle_bytes;
             e.G = p.parts[pi].groups;
             e.gx = (int)std::min<size_t>(nblk(half, 16), 1024);
             return e;
@@ -542,7 +931,17 @@ int launch_eval(const Program &p, const T *tab, size_t len, int k, const F *zsc,
     const Partition &q = p.parts[e.pi];
     const dim3 grid(e.gx, nz, e.G);
     const uint32_t *part = p.dev_buf.p + p.part_base[e.pi];
-    if (e.mode == EVAL_LANES) {
+    if (e.mode == EVAL_BUNDLES) {
+        if constexpr (std::is_same<T, EF>::value) {
+            static std::once_flag once;
+            std::call_once(once, [] {
+                (void)hipFuncSetAttribute((const void *)k_eval_bundles, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)EVAL_LDS_MAX);
+            });
+            const uint32_t *bpart = p.dev_buf.p + p.bpart_base[e.pi];
+            k_eval_bundles<<<grid, 256, e.lds_bytes, s>>>(tab, len, k, zsc, bpart, e.G, bat, eq, parts);
+        }
+    } else if (e.mode == EVAL_LANES) {
         if constexpr (std::is_same<T, EF>::value) {
             static std::once_flag once;
             std::call_once(once, [] {
