@@ -137,3 +137,32 @@ def test_program_partitions(gpu_ctx, oracle):
     z = rng.integers(0, P, (n, 8), dtype=np.uint32)
     s = rng.integers(0, P, 8, dtype=np.uint32)
     _run(gpu_ctx, oracle, p16, [3], 1, mles, p, 2, bat, z, False, s)
+
+
+def test_program_wide_sums(gpu_ctx, oracle):
+    """Constraints over a 300-term sum written as a chain of additions (which the
+    compiler rebalances into a tree for the bundled extension-field rounds) and a
+    chain with subtractions: 41 constraints, so 1, 4 and 16 constraint groups;
+    every round bit-exact against the oracle, which evaluates the program as
+    written."""
+    from whirlaway_amd.program import ConstraintProgram
+    _say("generic sumcheck, 300-term sums, n = 9")
+    b, p16 = _consts(oracle)
+    rng = np.random.default_rng(303)
+    n, m = 9, 300
+    p = ConstraintProgram(n_inputs=m)
+    xs = [p.input(i) for i in range(m)]
+    acc = xs[0]
+    for i in range(1, m):
+        acc = acc + xs[i]
+    alt = xs[1]
+    for i in range(2, 60):
+        alt = alt - xs[i] if i % 3 else alt + xs[i]
+    for k in range(40):
+        p.assert_zero(acc * xs[k] - xs[k + 1] + int(rng.integers(0, P)))
+    p.assert_zero(alt * acc)
+    mles = rng.integers(0, P, (m, 1 << n), dtype=np.uint32)
+    bat = rng.integers(0, P, (p.n_constraints, 8), dtype=np.uint32)
+    z = rng.integers(0, P, (n, 8), dtype=np.uint32)
+    s = rng.integers(0, P, 8, dtype=np.uint32)
+    _run(gpu_ctx, oracle, p16, [5], 1, mles, p, 2, bat, z, False, s)

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <mutex>
+#include <queue>
 #include <type_traits>
 #include "../../include/whirlaway_hip.h"
 #include "sc_generic.h"
@@ -592,28 +593,26 @@ int emit_bundles(const uint32_t *code, size_t n_instr, int n_inputs, const std::
         if (u < n_instr) pos[u] = (long)u;
         else pos[u] = users[u].empty() ? 0 : (long)users[u].front();
     }
-    // list schedule
-    std::vector<std::vector<uint32_t>> ready(8);
+    // list schedule: a ready queue per op, earliest program position first
+    using Key = std::pair<long, uint32_t>;
+    std::vector<std::priority_queue<Key, std::vector<Key>, std::greater<Key>>> ready(8);
     for (size_t u = 0; u < NN; u++)
-        if (live[u] && ndeps[u] == 0) ready[kind(u)].push_back((uint32_t)u);
-    auto better = [&](uint32_t a, uint32_t b) { return pos[a] != pos[b] ? pos[a] < pos[b] : a < b; };
+        if (live[u] && ndeps[u] == 0) ready[kind(u)].push({pos[u], (uint32_t)u});
     std::vector<std::vector<uint32_t>> bundles;
     std::vector<uint32_t> bundle_op;
     for (;;) {
         int best_k = -1;
-        uint32_t best = 0;
         for (int k = 0; k < 8; k++)
-            for (uint32_t u : ready[k])
-                if (best_k < 0 || better(u, best)) best_k = k, best = u;
+            if (!ready[k].empty() && (best_k < 0 || ready[k].top() < ready[best_k].top())) best_k = k;
         if (best_k < 0) break;
-        std::vector<uint32_t> &r = ready[best_k];
-        std::sort(r.begin(), r.end(), better);
-        const size_t cnt = std::min<size_t>(4, r.size());
-        std::vector<uint32_t> b(r.begin(), r.begin() + cnt);
-        r.erase(r.begin(), r.begin() + cnt);
+        std::vector<uint32_t> b;
+        while (b.size() < 4 && !ready[best_k].empty()) {
+            b.push_back(ready[best_k].top().second);
+            ready[best_k].pop();
+        }
         for (uint32_t u : b)
             for (uint32_t v : users[u])
-                if (--ndeps[v] == 0) ready[kind(v)].push_back(v);
+                if (--ndeps[v] == 0) ready[kind(v)].push({pos[v], v});
         bundles.push_back(std::move(b));
         bundle_op.push_back((uint32_t)best_k);
     }
@@ -677,6 +676,9 @@ int emit_bundles(const uint32_t *code, size_t n_instr, int n_inputs, const std::
 }
 
 // One partition (see compile): false when the program has too few constraints for G groups.
+// bundles: emit only the bundled code (Partition::bcode; left empty, and the
+// partition's bundles unused, when they would keep more than 256 values live),
+// else only the one-instruction slot code.
 bool build_partition(const uint32_t *code, size_t n_instr, int n_inputs, int G, bool bundles, Program &out) {
     std::vector<char> is_assert(n_instr, 0);
     std::vector<int> assert_rank(n_instr, -1);
@@ -706,12 +708,12 @@ bool build_partition(const uint32_t *code, size_t n_instr, int n_inputs, int G, 
     std::vector<char> keep(n_instr, 0);
     long group_w = 0;
     auto close_group = [&]() {
-        part.off.push_back((uint32_t)part.code.size());
-        const int ns = emit_slots(code, n_instr, n_inputs, keep, assert_rank, part.code);
-        part.n_slots = std::max(part.n_slots, ns);
         if (bundles) {
             part.boff.push_back((uint32_t)part.bcode.size());
             part.bn_slots = std::max(part.bn_slots, emit_bundles(code, n_instr, n_inputs, keep, assert_rank, part.bcode));
+        } else {
+            part.off.push_back((uint32_t)part.code.size());
+            part.n_slots = std::max(part.n_slots, emit_slots(code, n_instr, n_inputs, keep, assert_rank, part.code));
         }
         part.groups++;
         std::fill(keep.begin(), keep.end(), 0);
@@ -734,16 +736,23 @@ bool build_partition(const uint32_t *code, size_t n_instr, int n_inputs, int G, 
         if (G > 1 && part.groups < G - 1 && group_w * G >= total) close_group();
     }
     if (part.groups == 0 || std::count(keep.begin(), keep.end(), 1) > 0) close_group();
-    part.off.push_back((uint32_t)part.code.size());
-    if (bundles) part.boff.push_back((uint32_t)part.bcode.size());
-    if (part.n_slots > 256) throw std::runtime_error("program keeps more than 256 values live");
+    if (bundles) {
+        part.boff.push_back((uint32_t)part.bcode.size());
+        if (part.bn_slots > 256) {  // beyond the operand encoding: no bundles for this partition
+            part.bcode.clear();
+            part.boff.clear();
+        }
+    } else {
+        part.off.push_back((uint32_t)part.code.size());
+        if (part.n_slots > 256) throw std::runtime_error("program keeps more than 256 values live");
+    }
     out.parts.push_back(std::move(part));
     return true;
 }
 
 // Sums written as chains of additions (s = s + x_i) rebalanced into trees: an ADD
 // whose only use is another ADD is folded into that ADD's sum, and each sum of
-// L >= 3 terms is emitted as L - 1 additions pairing the terms level by level.
+// L >= 3 terms is emitted as L - 1 additions combining equal-size partial sums.
 // Field addition is exact, so every value the program defines is unchanged; the
 // dependent depth of an L-term sum falls from L - 1 to about log2 L, which lets
 // the bundled interpreter run its additions side by side.
@@ -792,12 +801,25 @@ std::vector<uint32_t> rebalance_sums(const uint32_t *code, size_t n, int n_input
                 leaves.push_back(map[v]);
             }
         }
-        level = leaves;
+        // pairs of partial sums of equal size combine as soon as both exist (a
+        // binary counter over the terms): depth ~log2 L, and only ~log2 L partial
+        // sums are live at a time
+        level.clear();  // the partial sums
+        next.clear();   // their sizes (powers of two, decreasing)
+        for (uint32_t v : leaves) {
+            level.push_back(v);
+            next.push_back(1);
+            while (next.size() >= 2 && next[next.size() - 1] == next[next.size() - 2]) {
+                const uint32_t r = emit(OP_ADD, level[level.size() - 2], level.back());
+                level.pop_back(), next.pop_back();
+                level.back() = r;
+                next.back() *= 2;
+            }
+        }
         while (level.size() > 1) {
-            next.clear();
-            for (size_t i = 0; i + 1 < level.size(); i += 2) next.push_back(emit(OP_ADD, level[i], level[i + 1]));
-            if (level.size() & 1) next.push_back(level.back());
-            level.swap(next);
+            const uint32_t r = emit(OP_ADD, level[level.size() - 2], level.back());
+            level.pop_back(), next.pop_back();
+            level.back() = r;
         }
         map[n_inputs + j] = level[0];
     }
@@ -843,14 +865,16 @@ void compile(const uint32_t *code_in, size_t n_instr_in, int n_inputs, Program &
     // Rounds with few points evaluate a point's constraints in G threads at once.
     for (int G : {1, 4, 16}) {
         if (!build_partition(code_in, n_instr_in, n_inputs, G, false, out)) break;
-        if (!build_partition(bal.data(), bal.size() / 4, n_inputs, G, true, bprog))
-            throw std::runtime_error("program partition differs after rebalancing its sums");
         Partition &q = out.parts.back();
-        Partition &b = bprog.parts.back();
-        if (b.groups != q.groups) throw std::runtime_error("program partition differs after rebalancing its sums");
-        q.bn_slots = b.bn_slots;
-        q.boff = std::move(b.boff);
-        q.bcode = std::move(b.bcode);
+        // bundles only where the rebalanced program splits into as many groups and
+        // they fit the operand encoding; otherwise k_eval_lanes runs this partition
+        if (build_partition(bal.data(), bal.size() / 4, n_inputs, G, true, bprog) &&
+            bprog.parts.back().groups == q.groups && !bprog.parts.back().bcode.empty()) {
+            Partition &b = bprog.parts.back();
+            q.bn_slots = b.bn_slots;
+            q.boff = std::move(b.boff);
+            q.bcode = std::move(b.bcode);
+        }
     }
     out.n_slots = out.parts[0].n_slots;
     out.dev = out.parts[0].code;
@@ -899,7 +923,7 @@ EvalLaunch plan_eval(const Program &p, size_t len, int k, int nz) {
         size_t pi = 0;
         while (pi + 1 < p.parts.size() && half * (size_t)nz * (size_t)p.parts[pi].groups * 16 < ((size_t)1 << 17))
             pi++;
-        const size_t bundle_bytes = (size_t)p.parts[pi].bn_slots * 256 * 4;
+        const size_t bundle_bytes = p.parts[pi].bcode.empty() ? SIZE_MAX : (size_t)p.parts[pi].bn_slots * 256 * 4;
         e.lds_bytes = (size_t)p.parts[pi].n_slots * 256 * 4;
         if (bundle_bytes <= EVAL_LDS_MAX || e.lds_bytes <= EVAL_LDS_MAX) {
             e.pi = pi;
