@@ -78,6 +78,13 @@ void wl_destroy(wl_ctx *ctx);
  * this way.  No counterpart in the reference. */
 int wl_set_device_waits(wl_ctx *ctx, int enabled);
 
+/* Bound of each device-side wait, in seconds (0 restores the default, 10 s).  A
+ * wait that reaches it ends its kernel without the host value; the proof then
+ * fails with "a device wait for a host value timed out", and the context serves
+ * later calls.  No counterpart in the reference (its prover does not wait on the
+ * device). */
+int wl_set_wait_timeout(wl_ctx *ctx, double seconds);
+
 /* AirTable::new for the Poseidon2-16 KoalaBear AIR (crates/air/src/table.rs:32-56,
  * called at src/examples/poseidon2.rs:140-146).  round_constants: the AIR's
  * RoundConstants (148 canonical u32: beginning[4][16], partial[20],

@@ -41,6 +41,7 @@ def lib():
         L.wl_create.argtypes = [ctypes.c_int]
         L.wl_destroy.argtypes = [vp]
         L.wl_set_device_waits.argtypes = [vp, ctypes.c_int]
+        L.wl_set_wait_timeout.argtypes = [vp, ctypes.c_double]
         L.wl_air_table_new.restype = vp
         L.wl_air_table_new.argtypes = [vp, u32p, ctypes.c_uint32, ctypes.c_uint32]
         L.wl_air_table_free.argtypes = [vp]

@@ -63,6 +63,10 @@ class Context:
         device-side waits (the default), or only once the challenge is known."""
         check(lib().wl_set_device_waits(self.ptr, int(bool(enabled))))
 
+    def set_wait_timeout(self, seconds):
+        """wl_set_wait_timeout: the bound of each device-side wait (0: the default)."""
+        check(lib().wl_set_wait_timeout(self.ptr, float(seconds)))
+
     def phase_times(self, reset=True):
         out = (ctypes.c_double * 7)()
         lib().wl_phase_times(self.ptr, out, int(reset))

@@ -390,6 +390,8 @@ struct wl_ctx {
     uint32_t *mbox = nullptr, *mbox_dev = nullptr, *args_dev = nullptr;
     uint32_t mbox_next = 0;
     uint64_t await_ticks = 0;
+    double await_limit_s = 10.0;  // bound of each device wait (wl_set_wait_timeout)
+    int await_khz = 0;
     struct HostSlot { uint32_t id; uint32_t *host; uint32_t *dev; };
     void ensure_mbox() {
         if (mbox) return;
@@ -400,7 +402,12 @@ struct wl_ctx {
         HIP_CHECK(hipMalloc((void **)&args_dev, (size_t)MB_SLOTS * MB_WORDS * 4));
         int khz = 0;
         HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
-        await_ticks = (uint64_t)(khz > 0 ? khz : 100000) * 1000ull * 10ull;  // 10 s
+        await_khz = khz > 0 ? khz : 100000;
+        set_await_limit(await_limit_s);
+    }
+    void set_await_limit(double seconds) {
+        await_limit_s = seconds;
+        if (await_khz) await_ticks = (uint64_t)((double)await_khz * 1000.0 * seconds);
     }
     // enqueue a wait for the next host value (<= MB_WORDS words), readable by later
     // kernels at .dev

@@ -1161,6 +1161,13 @@ int wl_set_device_waits(wl_ctx *c, int enabled) {
     return WL_OK;
 }
 
+int wl_set_wait_timeout(wl_ctx *c, double seconds) {
+    if (!c) { g_err = "null context"; return WL_ERR_ARG; }
+    if (!(seconds >= 0.0 && seconds <= 3600.0)) { g_err = "timeout must be in [0, 3600] seconds"; return WL_ERR_ARG; }
+    c->set_await_limit(seconds == 0.0 ? 10.0 : seconds);
+    return WL_OK;
+}
+
 int wl_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
