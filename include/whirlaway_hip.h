@@ -264,7 +264,10 @@ int wl_fmul_rate(wl_ctx *ctx, double *fmul_per_s);
 /* Host-side check of the device Poseidon2 form (partial rounds as a convolution,
  * paired diagonal entries) against the round-by-round permutation: random round
  * constants from `seed`, n_states random states plus field-extreme ones, for
- * width 16 (mismatches[0]) and 24 (mismatches[1]).  No GPU is used; no
+ * width 16 (mismatches[0]) and 24 (mismatches[1]).  Width 16 also counts states
+ * on which the transcript's AVX2 permutation differs, and inputs (field extremes
+ * and n_states random ones) on which its partial-round cube differs from the
+ * Montgomery cube.  No GPU is used; no
  * counterpart in the reference (it checks this build's own rewriting of
  * Poseidon2Koalabear's permutation). */
 int wl_p2_conv_check(uint32_t n_states, uint64_t seed, uint64_t mismatches[2]);

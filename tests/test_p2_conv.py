@@ -1,9 +1,11 @@
 """Poseidon2 as the device kernels compute it -- the partial rounds as a
 convolution with the internal diagonal's entries taken in pairs (d, -d) -- must
-equal the round-by-round permutation (kb.h p2_permute, the form the host
-transcript uses and which the proof parity tests pin against the oracle).  The
-check runs on the host through the library, for widths 16 and 24, on random
-states and round constants and on field-extreme states.  Reference:
+equal the round-by-round permutation (kb.h p2_permute, which the proof parity
+tests pin against the oracle), and so must the host transcript's AVX2
+Poseidon2-16 (host_p16.h, whose partial rounds cube by one reduction mod 2^64).
+The check runs on the host through the library, for widths 16 and 24, on random
+states and round constants and on field-extreme states (and, for the AVX2 cube,
+on field-extreme inputs).  Reference:
 Poseidon2KoalaBear<16>/<24> as named at src/examples/poseidon2.rs:22-23."""
 import ctypes
 

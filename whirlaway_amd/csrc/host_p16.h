@@ -7,6 +7,8 @@
 // lanes), the external layer's 4x4 blocks are 128-bit-lane shuffles, and the
 // internal layer multiplies by the Montgomery images of the diagonal.  Same
 // results as kb::p2_permute<16, 20> (the proofs are compared bit for bit).
+// The partial rounds' S-box is a scalar chain, so its cube takes one Montgomery
+// reduction by 2^64 instead of two by 2^32 (cube64).
 #pragma once
 #include <stdint.h>
 #include "common.h"
@@ -51,6 +53,24 @@ struct HostP16 {
     }
     static __m256i ld(const kb::F *p) { return _mm256_loadu_si256((const __m256i *)p); }
 
+    static constexpr uint64_t pinv64() {  // P^-1 mod 2^64 by Newton's iteration (3 -> 96 bits)
+        uint64_t x = kb::P;
+        for (int i = 0; i < 5; i++) x *= 2 - (uint64_t)kb::P * x;
+        return x;
+    }
+    // Montgomery cube of t < P: t^3 < 2^93 < 2^64 P, so one reduction by 2^64 gives
+    // t^3 / 2^64 mod P, which for t = a 2^32 is (a^3) 2^32, the Montgomery image of
+    // a^3.  With m = lo64(T) P^-1, T - m P is a multiple of 2^64 and
+    // hi(T) - hi(m P) lies in (-P, 2^29): one conditional add of P.
+    static uint32_t cube64(uint32_t t) {
+        const uint64_t t2 = (uint64_t)t * t;
+        const unsigned __int128 T = (unsigned __int128)t2 * t;
+        const uint64_t m = (uint64_t)T * pinv64();
+        const uint64_t mh = (uint64_t)(((unsigned __int128)m * kb::P) >> 64);
+        const uint32_t d = (uint32_t)((uint64_t)(T >> 64) - mh);
+        return kb::umin(d, d + kb::P);
+    }
+
     static void permute(kb::F *s, const kb::P2Consts16 &k) {
         __m256i lo = ld(s), hi = ld(s + 8);
         const __m256i dlo = _mm256_loadu_si256((const __m256i *)kb::DIAG.d16);
@@ -66,7 +86,7 @@ struct HostP16 {
         // x0' = s + d0 y = S - y (d0 = -2) and x_i' = s + d_i x_i (i >= 1), and
         // S' = 15 s + sum_{i>=1} d_i x_i.  The vector products d_i x_i and their sum
         // depend only on x, so they overlap the scalar S-box chain, whose step is
-        // one subtraction, the round constant and the cube.
+        // one subtraction (from S + rc, formed while the cube runs) and the cube.
         const __m256i lane0 = _mm256_setr_epi32(-1, 0, 0, 0, 0, 0, 0, 0);
         auto hsum_rest = [&](__m256i a, __m256i b) {  // sum of lanes 1..15 of (a, b)
             __m256i u = vadd(_mm256_andnot_si256(lane0, a), b);
@@ -80,10 +100,12 @@ struct HostP16 {
         kb::F rest = hsum_rest(lo, hi);
         __m256i vlo = vmul(lo, dlo), vhi = vmul(hi, dhi);
         kb::F w = hsum_rest(vlo, vhi);
+        kb::F t = kb::fadd(x0, k.internal[0]);  // the S-box input x0 + rc
         for (int r = 0; r < 20; r++) {
-            const kb::F y = kb::fcube(kb::fadd(x0, k.internal[r]));
+            const kb::F y{cube64(t.v)};
             const kb::F sr = kb::fadd(rest, y);
-            x0 = kb::fsub(rest, y);
+            if (r + 1 < 20) t = kb::fsub(kb::fadd(rest, k.internal[r + 1]), y);
+            else x0 = kb::fsub(rest, y);
             rest = kb::fadd(kb::fmul(fifteen, sr), w);
             const __m256i sv = _mm256_set1_epi32((int)sr.v);
             lo = vadd(sv, vlo);
@@ -104,6 +126,7 @@ struct HostP16 {
         _mm256_storeu_si256((__m256i *)(s + 8), hi);
     }
 };
+static_assert((uint64_t)kb::P * HostP16::pinv64() == 1, "P^-1 mod 2^64");
 
 }  // namespace wh
 #endif

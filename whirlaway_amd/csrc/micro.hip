@@ -78,11 +78,28 @@ uint64_t conv_mismatches(const PartialConv<W, RP> &pc, const uint32_t *diag, uin
             else v = (i == (int)(st % W)) ? extremes[st - n - 2 * n_ext] : P - 1;
             a[i] = b[i] = to_mont(v);
         }
+        F h[W];
+        for (int i = 0; i < W; i++) h[i] = a[i];
         p2_permute<W, RP>(a, k, diag);
         p2_permute_conv<W, RP>(b, k, pc);
-        for (int i = 0; i < W; i++)
-            if (a[i].v != b[i].v) { bad++; break; }
+        bool differ = false;
+        for (int i = 0; i < W; i++) differ |= a[i].v != b[i].v;
+#if !defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (W == 16) {  // the transcript's AVX2 permutation on the same state
+            wh::HostP16::permute(h, k);
+            for (int i = 0; i < W; i++) differ |= a[i].v != h[i].v;
+        }
+#endif
+        bad += differ;
     }
+#if !defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (W == 16) {  // its partial-round cube on field extremes and random inputs
+        for (uint32_t j = 0; j < n + 64; j++) {
+            const uint32_t t = j < 32 ? j : (j < 64 ? P - 1 - (j - 32) : next());
+            bad += wh::HostP16::cube64(t) != fcube(F{t}).v;
+        }
+    }
+#endif
     return bad;
 }
 
