@@ -93,21 +93,26 @@ __global__ void k_publish(const uint32_t *src, size_t n, uint32_t *dst, volatile
     } else {
         for (size_t i = t; i < n; i += stride) dst[i] = pub_word(src[i], i, n_canon);
     }
+    // Every wave's system fence waits for its own stores; after the barrier thread 0
+    // of a lone block raises the flag.  (Each fence also writes L2 back, ~0.15 us
+    // per wave, so the launch gives a lone block only as many waves as its data.)
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x != 0) return;
     if (gridDim.x > 1) {
         if (atomicAdd(counter, 1u) != gridDim.x - 1) return;
         __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence_system();
     }
-    __threadfence_system();
     *flag = seq;
 }
 void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
              uint32_t *counter, hipStream_t s, size_t n_canon) {
     // one block up to 16 KiB, then a block per 16 KiB, at most 64
     const int blocks = (int)std::min<size_t>(64, std::max<size_t>(1, (n + 4095) / 4096));
-    k_publish<<<blocks, 1024, 0, s>>>(src, n, host_dst, flag, seq, counter, n_canon);
+    // a lone block: one thread per four 16-byte words, in whole waves
+    const int threads = blocks > 1 ? 1024 : (int)std::min<size_t>(1024, std::max<size_t>(64, ((n + 15) / 16 + 63) / 64 * 64));
+    k_publish<<<blocks, threads, 0, s>>>(src, n, host_dst, flag, seq, counter, n_canon);
 }
 // rows of row_words words, stride_words apart in device memory, packed into host memory
 __global__ void k_publish_rows(const uint32_t *src, int rows, int row_words, size_t stride_words, uint32_t *dst,
@@ -115,16 +120,15 @@ __global__ void k_publish_rows(const uint32_t *src, int rows, int row_words, siz
     const int n = rows * row_words;
     for (int i = threadIdx.x; i < n; i += blockDim.x)
         dst[i] = src[(size_t)(i / row_words) * stride_words + (size_t)(i % row_words)];
-    __threadfence_system();
+    __threadfence_system();  // each wave's stores complete (as in k_publish)
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence_system();
-        *flag = seq;
-    }
+    if (threadIdx.x == 0) *flag = seq;
 }
 void publish_rows(const uint32_t *src, int rows, int row_words, size_t stride_words, uint32_t *host_dst,
                   volatile uint32_t *flag, uint32_t seq, hipStream_t s) {
-    k_publish_rows<<<1, 1024, 0, s>>>(src, rows, row_words, stride_words, host_dst, flag, seq);
+    const int n = rows * row_words;  // eight words per thread, in whole waves
+    const int threads = std::min(1024, std::max(64, ((n + 7) / 8 + 63) / 64 * 64));
+    k_publish_rows<<<1, threads, 0, s>>>(src, rows, row_words, stride_words, host_dst, flag, seq);
 }
 
 // one block: item i summed by wave i mod (waves per block), no block barriers,
@@ -145,16 +149,32 @@ __device__ __forceinline__ void reduce_items(const EF *in, int n_items, int n_pa
         if (lane == 0) dst[it] = acc;
     }
 }
-// one block: the item sums written to host memory, then the flag
+// One block: the item sums written to host memory, then the flag.  Up to
+// REDUCE_LDS_ITEMS sums go through LDS and wave 0 alone stores them, fences and
+// raises the flag (a system fence also writes L2 back, ~0.15 us per wave that
+// issues one); more sums are stored by the waves that made them.
+constexpr int REDUCE_LDS_ITEMS = 64;
+__device__ __forceinline__ void reduce_and_publish(const EF *in, int n_items, int n_parts, EF *dst,
+                                                   volatile uint32_t *flag, uint32_t seq) {
+    __shared__ EF sums[REDUCE_LDS_ITEMS];
+    if (n_items <= REDUCE_LDS_ITEMS) {
+        reduce_items(in, n_items, n_parts, sums);
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            for (int i = threadIdx.x; i < n_items; i += 64) dst[i] = sums[i];
+            __threadfence_system();
+            if (threadIdx.x == 0) *flag = seq;
+        }
+    } else {
+        reduce_items(in, n_items, n_parts, dst);
+        __threadfence_system();
+        __syncthreads();
+        if (threadIdx.x == 0) *flag = seq;
+    }
+}
 __global__ void k_reduce_publish(const EF *in, int n_items, int n_parts, EF *dst, volatile uint32_t *flag,
                                  uint32_t seq) {
-    reduce_items(in, n_items, n_parts, dst);
-    __threadfence_system();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence_system();
-        *flag = seq;
-    }
+    reduce_and_publish(in, n_items, n_parts, dst, flag, seq);
 }
 // thread 0 waits (bounded) until *aflag >= id, then the block copies `words` words
 // of the host value into device memory; on timeout id goes to *err instead
@@ -184,13 +204,7 @@ __device__ void await_copy(const uint32_t *aflag, uint32_t id, const uint32_t *s
 __global__ void k_reduce_publish_await(const EF *in, int n_items, int n_parts, EF *dst, volatile uint32_t *flag,
                                        uint32_t seq, const uint32_t *aflag, uint32_t id, const uint32_t *src,
                                        uint32_t *adst, int words, uint64_t timeout_ticks, uint32_t *err) {
-    reduce_items(in, n_items, n_parts, dst);
-    __threadfence_system();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence_system();
-        *flag = seq;
-    }
+    reduce_and_publish(in, n_items, n_parts, dst, flag, seq);
     await_copy(aflag, id, src, adst, words, timeout_ticks, err);
 }
 void reduce_publish_await(const EF *in, int n_items, int n_parts, EF *host_dst, volatile uint32_t *flag,
