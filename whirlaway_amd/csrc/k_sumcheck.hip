@@ -110,8 +110,9 @@ void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_
              uint32_t *counter, hipStream_t s, size_t n_canon) {
     // one block up to 16 KiB, then a block per 16 KiB, at most 64
     const int blocks = (int)std::min<size_t>(64, std::max<size_t>(1, (n + 4095) / 4096));
-    // a lone block: one thread per four 16-byte words, in whole waves
-    const int threads = blocks > 1 ? 1024 : (int)std::min<size_t>(1024, std::max<size_t>(64, ((n + 15) / 16 + 63) / 64 * 64));
+    // one thread per four 16-byte words: 256 threads for a block's 16 KiB, and a lone
+    // block in whole waves
+    const int threads = blocks > 1 ? 256 : (int)std::min<size_t>(1024, std::max<size_t>(64, ((n + 15) / 16 + 63) / 64 * 64));
     k_publish<<<blocks, threads, 0, s>>>(src, n, host_dst, flag, seq, counter, n_canon);
 }
 // rows of row_words words, stride_words apart in device memory, packed into host memory
