@@ -94,16 +94,17 @@ __global__ void k_publish(const uint32_t *src, size_t n, uint32_t *dst, volatile
         for (size_t i = t; i < n; i += stride) dst[i] = pub_word(src[i], i, n_canon);
     }
     // Every wave's system fence waits for its own stores; after the barrier thread 0
-    // of a lone block raises the flag.  (Each fence also writes L2 back, ~0.15 us
-    // per wave, so the launch gives a lone block only as many waves as its data.)
+    // releases them to the host with its own fence and raises the flag.  (Each fence
+    // also writes L2 back, ~0.15 us per wave, so the launch gives a lone block only
+    // as many waves as its data.)
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x != 0) return;
     if (gridDim.x > 1) {
         if (atomicAdd(counter, 1u) != gridDim.x - 1) return;
         __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence_system();
     }
+    __threadfence_system();
     *flag = seq;
 }
 void publish(const uint32_t *src, size_t n, uint32_t *host_dst, volatile uint32_t *flag, uint32_t seq,
@@ -121,9 +122,12 @@ __global__ void k_publish_rows(const uint32_t *src, int rows, int row_words, siz
     const int n = rows * row_words;
     for (int i = threadIdx.x; i < n; i += blockDim.x)
         dst[i] = src[(size_t)(i / row_words) * stride_words + (size_t)(i % row_words)];
-    __threadfence_system();  // each wave's stores complete (as in k_publish)
+    __threadfence_system();  // as in k_publish
     __syncthreads();
-    if (threadIdx.x == 0) *flag = seq;
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        *flag = seq;
+    }
 }
 void publish_rows(const uint32_t *src, int rows, int row_words, size_t stride_words, uint32_t *host_dst,
                   volatile uint32_t *flag, uint32_t seq, hipStream_t s) {
@@ -151,9 +155,10 @@ __device__ __forceinline__ void reduce_items(const EF *in, int n_items, int n_pa
     }
 }
 // One block: the item sums written to host memory, then the flag.  Up to
-// REDUCE_LDS_ITEMS sums go through LDS and wave 0 alone stores them, fences and
-// raises the flag (a system fence also writes L2 back, ~0.15 us per wave that
-// issues one); more sums are stored by the waves that made them.
+// REDUCE_LDS_ITEMS sums go through LDS and wave 0 alone stores and fences them (a
+// system fence also writes L2 back, ~0.15 us per wave that issues one); more sums
+// are stored and fenced by the waves that made them.  Thread 0 then releases them
+// with its own fence and raises the flag.
 constexpr int REDUCE_LDS_ITEMS = 64;
 __device__ __forceinline__ void reduce_and_publish(const EF *in, int n_items, int n_parts, EF *dst,
                                                    volatile uint32_t *flag, uint32_t seq) {
@@ -164,13 +169,15 @@ __device__ __forceinline__ void reduce_and_publish(const EF *in, int n_items, in
         if (threadIdx.x < 64) {
             for (int i = threadIdx.x; i < n_items; i += 64) dst[i] = sums[i];
             __threadfence_system();
-            if (threadIdx.x == 0) *flag = seq;
         }
     } else {
         reduce_items(in, n_items, n_parts, dst);
         __threadfence_system();
-        __syncthreads();
-        if (threadIdx.x == 0) *flag = seq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        *flag = seq;
     }
 }
 __global__ void k_reduce_publish(const EF *in, int n_items, int n_parts, EF *dst, volatile uint32_t *flag,
